@@ -1,0 +1,214 @@
+/*
+ * sod2d_hip.h -- C-ABI of the MI355X (gfx950) implementation of SOD2D's
+ * compressible spectral-element right-hand side and Runge-Kutta stage.
+ *
+ * Drop-in boundary.  Each operator entry point replaces one Fortran subroutine
+ * of the reference and takes the same arguments with the same meaning, layout
+ * (column-major, the reference's dimension order) and 1-based index values;
+ * the Fortran shim sod2d_amd/fortran/mod_sod2d_hip.f90 keeps the reference's
+ * subroutine interfaces and forwards to these functions.  Reference citations
+ * are relative to the LIUJUNLE97/sod2d source tree.
+ *
+ * Pointer arguments of the operator entry points are DEVICE pointers (the
+ * Fortran shim maps host arrays to their device mirrors with sodhip_deviceptr,
+ * the equivalent of OpenACC's host_data use_device).  All work is enqueued on
+ * the context's stream; entry points return after enqueueing.  Functions that
+ * hand data to the host synchronise.
+ *
+ * Every function returns 0 on success or a SODHIP_ERR_* code; the message of
+ * the last error of the calling thread is available from sodhip_last_error.
+ * The reference aborts (stop 1 / MPI_Abort) where this layer returns an error.
+ */
+#ifndef SOD2D_HIP_H
+#define SOD2D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SODHIP_OK 0
+#define SODHIP_ERR_ARG 1
+#define SODHIP_ERR_HIP 2
+#define SODHIP_ERR_NOT_PRESENT 3
+#define SODHIP_ERR_UNSUPPORTED 4
+#define SODHIP_ERR_ALLOC 5
+
+typedef struct sodhip_ctx_s *sodhip_ctx;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+int sodhip_version(void);
+/* copies the last error message (NUL-terminated, truncated to len) */
+int sodhip_last_error(char *buf, int len);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side case setup (CPU only; no device needed).  These restate the     */
+/* reference's set-up routines that produce the operators' inputs.           */
+/* ------------------------------------------------------------------------ */
+
+/* Element tables of order porder (2 <= porder <= 8):
+ *   a2ijk(nnode)                 mesh_a2ijk = gmsh2ijk (mod_ijk_indices.f90:61-114)
+ *   invAtoIJK(p+1,p+1,p+1), gmshAtoI/J/K(nnode)   atoIJKInverse (mod_geom.f90:127-148)
+ *   xgp(ngaus,3), wgp(ngaus)     GaussLobattoLegendre_hex (quadrature_rules.f90:51-78)
+ *   dNgp(3,nnode,ngaus), dlxigp_ip(ngaus,3,p+1)
+ *                                hex_highorder (elem_hex.f90:22-43) as driven by
+ *                                CFDSolverBase_evalShapeFunctions (CFDSolverBase.f90:1468-1619)
+ * Any output pointer may be NULL. */
+int sodhip_basis_tables(int porder, int *a2ijk, int *invAtoIJK, int *gmshAtoI, int *gmshAtoJ,
+                        int *gmshAtoK, double *xgp, double *wgp, double *dNgp, double *dlxigp_ip);
+
+/* Structured box of nx*ny*nz HEX((p+1)^3) elements on [0,L]^3 in SOD2D's partition
+ * format (mod_mpi_mesh.f90:15-48): npoin nodes including periodic slaves,
+ * connecParOrig/connecParWork (nelem,nnode) 1-based, workingNodesPar (npoin_w),
+ * coordPar (npoin,3).  Element-local node a follows the gmsh high-order order.
+ * periodic != 0 folds the x=L, y=L, z=L planes onto x=0, y=0, z=0 (slave ->
+ * master, as create_working_lists_parallel, mod_meshConversorTool.f90:4917-5008).
+ * jitter > 0 moves every interior-free node (non-periodic meshes only) by
+ * U(-jitter,jitter) times the local minimum GLL spacing per axis, from seed. */
+int sodhip_box_mesh_sizes(int porder, int nx, int ny, int nz, int periodic, int *nelem, int *npoin,
+                          int *npoin_w);
+int sodhip_box_mesh(int porder, int nx, int ny, int nz, int periodic, const double *lengths,
+                    double jitter, unsigned long long seed, int *connec_orig, int *connec_work,
+                    int *lpoin_w, double *coord);
+
+/* elem_jacobian (jacobian_oper.f90:7-140): He(3,3,ngaus,nelem), gpvol(1,ngaus,nelem)
+ * from connecParOrig and coordPar.  Returns SODHIP_ERR_ARG on a non-positive
+ * Jacobian determinant. */
+int sodhip_elem_jacobian(int porder, int nelem, int npoin, const int *connec, const double *coord,
+                         double *He, double *gpvol);
+/* lumped_mass_spectral + invMl (mass_matrix.f90:11-42; CFDSolverBase.f90:1771-1782) */
+int sodhip_lumped_mass(int porder, int nelem, int npoin, const int *connec_work,
+                       const double *gpvol, int npoin_w, const int *lpoin_w, double *Ml,
+                       double *invMl);
+/* char_length (mod_geom.f90:13-37) -> helem(nelem); char_length_spectral
+ * (mod_geom.f90:39-66) -> helem_l(nelem,nnode); both from connecParOrig */
+int sodhip_char_length(int porder, int nelem, int npoin, const int *connec_orig,
+                       const double *coord, double *helem, double *helem_l);
+
+/* ------------------------------------------------------------------------ */
+/* Device context                                                            */
+/* ------------------------------------------------------------------------ */
+int sodhip_create(int device, sodhip_ctx *ctx);
+int sodhip_destroy(sodhip_ctx ctx);
+/* stream (hipStream_t) on which all work of ctx is enqueued; NULL = null stream */
+int sodhip_set_stream(sodhip_ctx ctx, void *stream);
+int sodhip_synchronize(sodhip_ctx ctx);
+
+/* OpenACC-style present table for callers that hold host arrays (the Fortran
+ * shim): enter data create/copyin, update device/host, exit data, and the
+ * host -> device address lookup (NULL when not present). */
+int sodhip_enter_data(sodhip_ctx ctx, const void *host, size_t bytes, int copyin);
+int sodhip_update_device(sodhip_ctx ctx, const void *host, size_t bytes);
+int sodhip_update_host(sodhip_ctx ctx, void *host, size_t bytes);
+int sodhip_exit_data(sodhip_ctx ctx, const void *host, int copyout);
+void *sodhip_deviceptr(sodhip_ctx ctx, const void *host);
+
+/* Kernel timing with HIP events on the context stream: when enabled, every
+ * launch of kernel class `kind` is bracketed by events; sodhip_prof_get returns
+ * the summed duration (ms) and the number of launches since the last reset.
+ * kind: 0 element operator kernel, 1 node gather/update kernel, 2 sensor,
+ * 3 other nodal kernels. */
+int sodhip_prof_enable(sodhip_ctx ctx, int enable);
+int sodhip_prof_get(sodhip_ctx ctx, int kind, double *total_ms, long long *count);
+int sodhip_prof_reset(sodhip_ctx ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in operators (device pointers, reference layouts)                    */
+/* ------------------------------------------------------------------------ */
+
+/* full_convec_ijk (elem_convec.f90:227-470).  initialize != 0 zeroes Rmass,
+ * Rmom, Rener first; fact multiplies the element residual (the reference's
+ * optional `fact`; the caller resolves its SAVE'd default).  connec(nelem,nnode),
+ * He(3,3,ngaus,nelem), gpvol(1,ngaus,nelem), dlxigp_ip(ngaus,3,p+1),
+ * invAtoIJK(p+1,p+1,p+1), gmshAto*(nnode), nodal u(npoin,3), q(npoin,3), rho, pr,
+ * E (the energy slot; updateF passes the enthalpy aux_h, time_integ_ls.f90:605-606),
+ * Rmass(npoin), Rmom(npoin,3), Rener(npoin).  Assembly is a deterministic
+ * node-ordered gather (no atomics): results do not depend on scheduling. */
+int sodhip_full_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                               const double *He, const double *gpvol, const double *dlxigp_ip,
+                               const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                               const int *gmshAtoK, const double *u, const double *q,
+                               const double *rho, const double *pr, const double *E,
+                               double *Rmass, double *Rmom, double *Rener, int initialize,
+                               double fact);
+int sodhip_full_convec_ijk_f32(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                               const float *He, const float *gpvol, const float *dlxigp_ip,
+                               const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                               const int *gmshAtoK, const float *u, const float *q,
+                               const float *rho, const float *pr, const float *E, float *Rmass,
+                               float *Rmom, float *Rener, int initialize, float fact);
+
+/* full_diffusion_ijk (elem_diffu.f90:12-223).  mu_e and mu_sgs are (nelem,ngaus);
+ * mu_sgs may be NULL (identically zero).  Ml is not used by the reference and
+ * is not an argument here.  c_rho and c_ener are the module globals of
+ * mod_numerical_params.f90 (default 1). */
+int sodhip_full_diffusion_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                  const int *connec, const double *He, const double *gpvol,
+                                  const double *dlxigp_ip, const int *invAtoIJK,
+                                  const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                                  double Cp, double Pr, const double *rho_n, const double *rho,
+                                  const double *u, const double *Tem, const double *mu_fluid,
+                                  const double *mu_e, const double *mu_sgs, double c_rho,
+                                  double c_ener, double *Rmass, double *Rmom, double *Rener,
+                                  int initialize, double fact);
+int sodhip_full_diffusion_ijk_f32(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                  const int *connec, const float *He, const float *gpvol,
+                                  const float *dlxigp_ip, const int *invAtoIJK,
+                                  const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                                  float Cp, float Pr, const float *rho_n, const float *rho,
+                                  const float *u, const float *Tem, const float *mu_fluid,
+                                  const float *mu_e, const float *mu_sgs, float c_rho,
+                                  float c_ener, float *Rmass, float *Rmom, float *Rener,
+                                  int initialize, float fact);
+
+/* generic_scalar_convec_ijk (elem_convec.f90:472-594): q is the flux f(npoin,3),
+ * eta the transported scalar, u the velocity; output Rconvec(npoin). */
+int sodhip_generic_scalar_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                         const int *connec, const double *He, const double *gpvol,
+                                         const double *dlxigp_ip, const int *invAtoIJK,
+                                         const int *gmshAtoI, const int *gmshAtoJ,
+                                         const int *gmshAtoK, const double *q, const double *eta,
+                                         const double *u, double *Rconvec, int initialize,
+                                         double fact);
+
+/* Module globals read by smart_visc_spectral (mod_numerical_params.f90:1-175). */
+typedef struct {
+    double ce, cmax, cglob, Rgas; /* Rgas = nscbc_Rgas_inf */
+    int flag_normalise_entropy, flag_drop_c_in_envit;
+} sodhip_visc_params;
+
+/* smart_visc_spectral (mod_entropy_viscosity.f90:14-111) with the filter tables
+ * of init_filters (mod_filters.f90:15-55).  helem(nelem,nnode) and
+ * helem_k(nelem) as in the reference's dummy arguments (callers pass helem_l and
+ * helem).  Outputs mu_e(nelem,ngaus) and mue_l(nelem,nnode).  The entropy
+ * normalisation is a deterministic two-level FP64 reduction. */
+int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                   const int *connec, const int *lpoin_w, const double *Reta,
+                                   double gamma_gas, const double *rho, const double *u,
+                                   const double *Tem, const double *eta, const double *helem,
+                                   const double *helem_k, const int *invAtoIJK,
+                                   const sodhip_visc_params *prm, double *mu_e, double *mue_l);
+
+/* lumped_solver_scal_opt / lumped_solver_vect_opt (mod_solver.f90:40-104) */
+int sodhip_lumped_solver_scal_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                  const double *invMl, double *R);
+int sodhip_lumped_solver_vect_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                  const double *invMl, double *R);
+
+/* RK stage update with the lumped-mass inverse fused in (the reference has these
+ * as inline loops: mod_solver.f90:40-104 then time_integ_ls.f90:323-331,350-358):
+ * over working nodes, R <- R*invMl (written back, as the reference's in-place
+ * lumped solve), then y <- lam1*y1 + lams*y + gam*dt*R for rho, E and q(:,1:3). */
+int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                               const double *invMl, double dt, double lam1, double lams,
+                               double gam, const double *rho1, const double *q1,
+                               const double *E1, double *rho, double *q, double *E,
+                               double *Rmass, double *Rmom, double *Rener);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,329 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by the cpu_baseline leg of ``bench.py``; the product never imports it.
+
+Arrays are numpy arrays in the reference's Fortran layout (``order='F'``) with the
+reference's dimension order; index tables hold the reference's 1-based values.
+The functions mirror the C entry points in ``sod2d_oracle.h``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+    return _LIB
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim <= 1), a.dtype
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and (a.flags.f_contiguous or a.ndim <= 1), a.dtype
+    return a.ctypes.data_as(_I)
+
+
+def fzeros(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+class ViscParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("ce", "cmax", "cglob", "Rgas", "c_rho", "c_ener", "ce_comp", "factor_comp")] + \
+               [("flag_normalise_entropy", C.c_int), ("flag_drop_c_in_envit", C.c_int)]
+
+
+class TGVParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("Cp", "Prt", "M", "Re", "rho0", "gamma_gas", "Rgas", "to", "po", "mu_factor")]
+
+
+# ---------------------------------------------------------------- basis
+def gll(p):
+    xi = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    lib().or_gll(p, _d(xi), _d(w))
+    return xi, w
+
+
+def lagrange(p, xi, x):
+    out = np.zeros(p + 1)
+    lib().or_lagrange(p, _d(np.ascontiguousarray(xi, dtype=np.float64)), C.c_double(x), _d(out))
+    return out
+
+
+def lagrange_deriv(p, xi, x):
+    out = np.zeros(p + 1)
+    lib().or_lagrange_deriv(p, _d(np.ascontiguousarray(xi, dtype=np.float64)), C.c_double(x),
+                            _d(out))
+    return out
+
+
+def gmsh_hex_table(p):
+    t = fzeros(((p + 1) ** 3, 3), np.int32)
+    lib().or_gmsh_hex_table(p, _i(t))
+    return t
+
+
+def a2ijk(p):
+    a = np.zeros((p + 1) ** 3, dtype=np.int32)
+    lib().or_a2ijk(p, _i(a))
+    return a
+
+
+def atoijk_inverse(p, a2):
+    n1 = p + 1
+    inv = fzeros((n1, n1, n1), np.int32)
+    gI = np.zeros(n1 ** 3, np.int32)
+    gJ = np.zeros(n1 ** 3, np.int32)
+    gK = np.zeros(n1 ** 3, np.int32)
+    lib().or_atoijk_inverse(p, _i(a2), _i(inv), _i(gI), _i(gJ), _i(gK))
+    return inv, gI, gJ, gK
+
+
+def shape(p, a2):
+    ng = (p + 1) ** 3
+    xgp = fzeros((ng, 3))
+    wgp = np.zeros(ng)
+    Ngp = fzeros((ng, ng))
+    dNgp = fzeros((3, ng, ng))
+    dl = fzeros((ng, 3, p + 1))
+    lib().or_shape(p, _i(a2), _d(xgp), _d(wgp), _d(Ngp), _d(dNgp), _d(dl))
+    return xgp, wgp, Ngp, dNgp, dl
+
+
+class Tables:
+    """All element tables of order p, built by the oracle."""
+
+    def __init__(self, p):
+        self.p = p
+        self.nnode = (p + 1) ** 3
+        self.a2ijk = a2ijk(p)
+        self.invAtoIJK, self.gI, self.gJ, self.gK = atoijk_inverse(p, self.a2ijk)
+        self.xgp, self.wgp, self.Ngp, self.dNgp, self.dlxigp_ip = shape(p, self.a2ijk)
+
+
+# ---------------------------------------------------------------- geometry
+def elem_jacobian(p, connec, coord, dNgp, wgp):
+    nelem = connec.shape[0]
+    ng = (p + 1) ** 3
+    gpvol = fzeros((1, ng, nelem))
+    He = fzeros((3, 3, ng, nelem))
+    lib().or_elem_jacobian(p, nelem, coord.shape[0], _i(connec), _d(coord), _d(dNgp), _d(wgp),
+                           _d(gpvol), _d(He))
+    return gpvol, He
+
+
+def lumped_mass(p, connec, gpvol, npoin):
+    Ml = np.zeros(npoin)
+    lib().or_lumped_mass(p, connec.shape[0], npoin, _i(connec), _d(gpvol), _d(Ml))
+    return Ml
+
+
+def inv_ml(Ml, lpoin_w):
+    invMl = np.zeros_like(Ml)
+    lib().or_inv_ml(Ml.shape[0], lpoin_w.shape[0], _i(lpoin_w), _d(Ml), _d(invMl))
+    return invMl
+
+
+def char_length(p, connec, coord):
+    h = np.zeros(connec.shape[0])
+    lib().or_char_length(p, connec.shape[0], coord.shape[0], _i(connec), _d(coord), _d(h))
+    return h
+
+
+def char_length_spectral(p, connec, coord):
+    h = fzeros(connec.shape)
+    lib().or_char_length_spectral(p, connec.shape[0], coord.shape[0], _i(connec), _d(coord),
+                                  _d(h))
+    return h
+
+
+# ---------------------------------------------------------------- operators
+def full_convec_ijk(T, connec, He, gpvol, u, q, rho, pr, E, Rmass=None, Rmom=None, Rener=None,
+                    initialize=True, fact=1.0, nthreads=1):
+    npoin = rho.shape[0]
+    if Rmass is None:
+        Rmass, Rmom, Rener = np.zeros(npoin), fzeros((npoin, 3)), np.zeros(npoin)
+    lib().or_full_convec_ijk(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
+                             _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ), _i(T.gK),
+                             _d(u), _d(q), _d(rho), _d(pr), _d(E), _d(Rmass), _d(Rmom), _d(Rener),
+                             int(bool(initialize)), C.c_double(fact), int(nthreads))
+    return Rmass, Rmom, Rener
+
+
+def full_diffusion_ijk(T, connec, He, gpvol, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs,
+                       Rmass=None, Rmom=None, Rener=None, initialize=True, fact=1.0,
+                       c_rho=1.0, c_ener=1.0, nthreads=1):
+    npoin = rho.shape[0]
+    if Rmass is None:
+        Rmass, Rmom, Rener = np.zeros(npoin), fzeros((npoin, 3)), np.zeros(npoin)
+    lib().or_full_diffusion_ijk(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
+                                _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ), _i(T.gK),
+                                C.c_double(Cp), C.c_double(Pr), _d(rho_n), _d(rho), _d(u), _d(Tem),
+                                _d(mu_fluid), _d(mu_e), _d(mu_sgs), C.c_double(c_rho),
+                                C.c_double(c_ener), _d(Rmass), _d(Rmom), _d(Rener),
+                                int(bool(initialize)), C.c_double(fact), int(nthreads))
+    return Rmass, Rmom, Rener
+
+
+def generic_scalar_convec_ijk(T, connec, He, gpvol, q, eta, u, Rconvec=None, initialize=True,
+                              fact=1.0):
+    npoin = eta.shape[0]
+    if Rconvec is None:
+        Rconvec = np.zeros(npoin)
+    lib().or_generic_scalar_convec_ijk(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
+                                       _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ),
+                                       _i(T.gK), _d(q), _d(eta), _d(u), _d(Rconvec),
+                                       int(bool(initialize)), C.c_double(fact))
+    return Rconvec
+
+
+def convert_ijk(p):
+    c = np.zeros(p + 3, np.int32)
+    lib().or_convert_ijk(p, _i(c))
+    return c
+
+
+def visc_params(Rgas, ce=0.1, cmax=0.5, cglob=1.0, c_rho=1.0, c_ener=1.0, ce_comp=1.0,
+                factor_comp=1.0, flag_normalise_entropy=1, flag_drop_c_in_envit=0):
+    return ViscParams(ce, cmax, cglob, Rgas, c_rho, c_ener, ce_comp, factor_comp,
+                      flag_normalise_entropy, flag_drop_c_in_envit)
+
+
+def smart_visc_spectral(T, connec, lpoin_w, Reta, gamma_gas, rho, u, Tem, eta, helem, helem_k,
+                        prm, mu_e, mue_l=None):
+    nelem = connec.shape[0]
+    if mue_l is None:
+        mue_l = fzeros(mu_e.shape)
+    lib().or_smart_visc_spectral(T.p, nelem, rho.shape[0], lpoin_w.shape[0], _i(connec),
+                                 _i(lpoin_w), _d(Reta), C.c_double(gamma_gas), _d(rho), _d(u),
+                                 _d(Tem), _d(eta), _d(helem), _d(helem_k), _i(T.invAtoIJK),
+                                 C.byref(prm), _d(mu_e), _d(mue_l))
+    return mu_e, mue_l
+
+
+def thermo_entropy(lpoin_w, gamma_gas, p_inf, rho_inf, rho, pr, u, eta, feta):
+    lib().or_thermo_entropy(rho.shape[0], lpoin_w.shape[0], _i(lpoin_w), C.c_double(gamma_gas),
+                            C.c_double(p_inf), C.c_double(rho_inf), _d(rho), _d(pr), _d(u),
+                            _d(eta), _d(feta))
+
+
+# ---------------------------------------------------------------- nodal
+def lumped_solver_scal(lpoin_w, invMl, R):
+    lib().or_lumped_solver_scal(R.shape[0], lpoin_w.shape[0], _i(lpoin_w), _d(invMl), _d(R))
+
+
+def lumped_solver_vect(lpoin_w, invMl, R):
+    lib().or_lumped_solver_vect(R.shape[0], lpoin_w.shape[0], _i(lpoin_w), _d(invMl), _d(R))
+
+
+def sutherland(Tem, mu_factor, mu_fluid):
+    lib().or_sutherland(Tem.shape[0], _d(Tem), _d(mu_factor), _d(mu_fluid))
+
+
+def eos_stage(lpoin_w, gamma_gas, Rgas, rho, q, E, u, e_int, pr, aux_h, Tem):
+    lib().or_eos_stage(rho.shape[0], lpoin_w.shape[0], _i(lpoin_w), C.c_double(gamma_gas),
+                       C.c_double(Rgas), _d(rho), _d(q), _d(E), _d(u), _d(e_int), _d(pr),
+                       _d(aux_h), _d(Tem))
+
+
+def eos_step(lpoin_w, gamma_gas, Rgas, rho, q, E, u, e_int, pr, csound, machno, Tem):
+    lib().or_eos_step(rho.shape[0], lpoin_w.shape[0], _i(lpoin_w), C.c_double(gamma_gas),
+                      C.c_double(Rgas), _d(rho), _d(q), _d(E), _d(u), _d(e_int), _d(pr),
+                      _d(csound), _d(machno), _d(Tem))
+
+
+def rk_ls_coeffs(stages=5, variant=1):
+    L = fzeros((6, 6))
+    G = fzeros((6, 6))
+    rc = lib().or_rk_ls_coeffs(stages, variant, _d(L), _d(G))
+    if rc != 0:
+        raise ValueError("unsupported LS-RK scheme")
+    return L, G
+
+
+def rk4_coeffs():
+    a, b, c = np.zeros(4), np.zeros(4), np.zeros(4)
+    lib().or_rk4_coeffs(_d(a), _d(b), _d(c))
+    return a, b, c
+
+
+def rk_ls_update(lpoin_w, l1, ls, gdt, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener):
+    lib().or_rk_ls_update(rho.shape[0], lpoin_w.shape[0], _i(lpoin_w), C.c_double(l1),
+                          C.c_double(ls), C.c_double(gdt), _d(rho1), _d(q1), _d(E1), _d(rho),
+                          _d(q), _d(E), _d(Rmass), _d(Rmom), _d(Rener))
+
+
+# ---------------------------------------------------------------- diagnostics
+def volavg_ek(T, connec, gpvol, rho0, rho, u):
+    f = lib().or_volavg_ek
+    f.restype = C.c_double
+    return f(T.p, connec.shape[0], rho.shape[0], _i(connec), _d(gpvol), _d(T.Ngp),
+             C.c_double(rho0), _d(rho), _d(u))
+
+
+def visc_dissipation(T, connec, rho0, mu_fluid, mu_e, u, volT, gpvol, He):
+    eS, eD, eT = C.c_double(), C.c_double(), C.c_double()
+    lib().or_visc_dissipation(T.p, connec.shape[0], u.shape[0], _i(connec), C.c_double(rho0),
+                              _d(mu_fluid), _d(mu_e), _d(u), C.c_double(volT), _d(gpvol), _d(He),
+                              _d(T.dNgp), C.byref(eS), C.byref(eD), C.byref(eT))
+    return eS.value, eD.value, eT.value
+
+
+def max_mach(lpoin_w, machno):
+    f = lib().or_max_mach
+    f.restype = C.c_double
+    return f(machno.shape[0], lpoin_w.shape[0], _i(lpoin_w), _d(machno))
+
+
+def adapt_dt_cfl(p, connec, helem, u, csound, cfl_conv, cfl_diff, mu_fluid, mu_sgs, mu_e, rho):
+    f = lib().or_adapt_dt_cfl
+    f.restype = C.c_double
+    return f(p, connec.shape[0], u.shape[0], _i(connec), _d(helem), _d(u), _d(csound),
+             C.c_double(cfl_conv), C.c_double(cfl_diff), _d(mu_fluid), _d(mu_sgs), _d(mu_e),
+             _d(rho))
+
+
+# ---------------------------------------------------------------- TGV
+def tgv_params(Cp=1004.0, Prt=0.71, M=0.1, Re=1600.0, rho0=1.0, gamma_gas=1.4):
+    t = TGVParams(Cp, Prt, M, Re, rho0, gamma_gas, 0, 0, 0, 0)
+    lib().or_tgv_params_derive(C.byref(t))
+    return t
+
+
+def tgv_ic(coord, t):
+    npoin = coord.shape[0]
+    u, q = fzeros((npoin, 3)), fzeros((npoin, 3))
+    rho, pr, E, e_int, Tem, csound, eta, machno = (np.zeros(npoin) for _ in range(8))
+    lib().or_tgv_ic(npoin, _d(coord), C.byref(t), _d(u), _d(q), _d(rho), _d(pr), _d(E), _d(e_int),
+                    _d(Tem), _d(csound), _d(eta), _d(machno))
+    return dict(u=u, q=q, rho=rho, pr=pr, E=E, e_int=e_int, Tem=Tem, csound=csound, eta=eta,
+                machno=machno)

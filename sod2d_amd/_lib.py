@@ -1,0 +1,109 @@
+"""Loader and ctypes signatures of libsod2d_hip.so (the C-ABI in include/sod2d_hip.h).
+
+The library is built in-tree by ``python -m sod2d_amd.build``.  There is no
+fallback: if the library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsod2d_hip.so")
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+F = C.c_float
+SZ = C.c_size_t
+
+
+class SodhipError(RuntimeError):
+    pass
+
+
+class ViscParams(C.Structure):
+    """sodhip_visc_params: module globals read by smart_visc_spectral."""
+    _fields_ = [("ce", D), ("cmax", D), ("cglob", D), ("Rgas", D),
+                ("flag_normalise_entropy", I), ("flag_drop_c_in_envit", I)]
+
+
+class StageParams(C.Structure):
+    """sodhip_stage_params: physical constants of the fused RK stage."""
+    _fields_ = [("gamma_gas", D), ("Rgas", D), ("Cp", D), ("Pr", D), ("mu_factor", D),
+                ("c_rho", D), ("c_ener", D), ("fact", D)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "sodhip_version": (I, []),
+    "sodhip_last_error": (I, [C.c_char_p, I]),
+    "sodhip_basis_tables": (I, [I, P, P, P, P, P, P, P, P, P]),
+    "sodhip_box_mesh_sizes": (I, [I, I, I, I, I, P, P, P]),
+    "sodhip_box_mesh": (I, [I, I, I, I, I, P, D, C.c_ulonglong, P, P, P, P]),
+    "sodhip_elem_jacobian": (I, [I, I, I, P, P, P, P]),
+    "sodhip_lumped_mass": (I, [I, I, I, P, P, I, P, P, P]),
+    "sodhip_char_length": (I, [I, I, I, P, P, P, P]),
+    "sodhip_create": (I, [I, P]),
+    "sodhip_destroy": (I, [P]),
+    "sodhip_set_stream": (I, [P, P]),
+    "sodhip_synchronize": (I, [P]),
+    "sodhip_enter_data": (I, [P, P, SZ, I]),
+    "sodhip_update_device": (I, [P, P, SZ]),
+    "sodhip_update_host": (I, [P, P, SZ]),
+    "sodhip_exit_data": (I, [P, P, I]),
+    "sodhip_deviceptr": (P, [P, P]),
+    "sodhip_prof_enable": (I, [P, I]),
+    "sodhip_prof_get": (I, [P, I, C.POINTER(D), C.POINTER(C.c_longlong)]),
+    "sodhip_prof_reset": (I, [P]),
+    "sodhip_full_convec_ijk_f64": (I, [P, I, I, I] + [P] * 16 + [I, D]),
+    "sodhip_full_convec_ijk_f32": (I, [P, I, I, I] + [P] * 16 + [I, F]),
+    "sodhip_full_diffusion_ijk_f64": (I, [P, I, I, I] + [P] * 8 + [D, D] + [P] * 7 + [D, D]
+                                      + [P] * 3 + [I, D]),
+    "sodhip_full_diffusion_ijk_f32": (I, [P, I, I, I] + [P] * 8 + [F, F] + [P] * 7 + [F, F]
+                                      + [P] * 3 + [I, F]),
+    "sodhip_generic_scalar_convec_ijk_f64": (I, [P, I, I, I] + [P] * 11 + [P, I, D]),
+    "sodhip_smart_visc_spectral_f64": (I, [P, I, I, I, I, P, P, P, D] + [P] * 7
+                                       + [C.POINTER(ViscParams), P, P]),
+    "sodhip_lumped_solver_scal_f64": (I, [P, I, I, P, P, P]),
+    "sodhip_lumped_solver_vect_f64": (I, [P, I, I, P, P, P]),
+    "sodhip_rk_stage_update_f64": (I, [P, I, I, P, P, D, D, D, D] + [P] * 9),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libsod2d_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SodhipError(f"{LIB_PATH} is missing: build it with `python -m sod2d_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names declared in include/sod2d_hip.h (parsed from the header)."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "sod2d_hip.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(sodhip_[a-z0-9_]+)\s*\(", txt)))
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().sodhip_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        raise SodhipError(f"{what or 'sodhip call'} failed (code {rc}): {last_error()}")

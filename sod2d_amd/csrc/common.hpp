@@ -1,0 +1,106 @@
+// common.hpp -- shared definitions of the sod2d_hip library (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sod2d_hip.h"
+
+namespace sodhip {
+
+// ---------------------------------------------------------------- errors
+int set_error(int code, const char *fmt, ...);
+
+#define SODHIP_TRY_HIP(expr)                                                               \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return ::sodhip::set_error(SODHIP_ERR_HIP, "%s failed: %s (%s:%d)", #expr,     \
+                                       hipGetErrorString(_e), __FILE__, __LINE__);         \
+    } while (0)
+
+#define SODHIP_REQUIRE(cond, ...)                                                          \
+    do {                                                                                   \
+        if (!(cond)) return ::sodhip::set_error(SODHIP_ERR_ARG, __VA_ARGS__);              \
+    } while (0)
+
+// ---------------------------------------------------------------- element sizes
+constexpr int kMaxP = 4; // device kernels are instantiated for porder 2..4
+
+inline int nnode_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Node-ordered incidence (CSR) of a connectivity: for every node n (0-based),
+// entries row[n]..row[n+1]-1 of col hold e*nnode+a for each (e,a) with
+// connec(e,a) == n+1, in ascending (e,a) order -- the order in which the
+// reference's serial element loop adds element contributions.
+struct MeshCache {
+    const int *connec_key = nullptr; // device pointer of the caller's connec
+    int porder = 0, nelem = 0, npoin = 0;
+    int *connecT = nullptr; // (nelem, nnode) element-major, 0-based node ids
+    int *row = nullptr;     // npoin+1
+    int *col = nullptr;     // nelem*nnode
+    // per-element-node index of the node's CSR row start is not needed: the
+    // gather kernel walks rows.
+};
+
+struct PresentEntry {
+    void *dev = nullptr;
+    size_t bytes = 0;
+};
+
+struct ProfSlot {
+    double total_ms = 0.0;
+    long long count = 0;
+};
+
+constexpr int kProfKinds = 4;
+
+} // namespace sodhip
+
+struct sodhip_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<uintptr_t, sodhip::PresentEntry> present; // host start address -> mirror
+    std::vector<std::unique_ptr<sodhip::MeshCache>> meshes;
+    sodhip::DevBuf scratch;  // per element-node residual partials
+    sodhip::DevBuf scratch2; // reduction partials
+    sodhip::DevBuf hostbuf;  // pinned staging (small reductions)
+    bool prof = false;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    sodhip::ProfSlot slots[sodhip::kProfKinds];
+    int num_cu = 256;
+};
+
+namespace sodhip {
+
+// grows ctx->scratch to at least `bytes`
+int ensure_buf(DevBuf &b, size_t bytes);
+// mesh cache lookup/build for (connec, porder, nelem, npoin)
+int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+             MeshCache **out);
+
+// launch timing helpers: prof_begin returns a token (or -1); prof_end closes it
+int prof_begin(sodhip_ctx ctx, int kind);
+void prof_end(sodhip_ctx ctx, int token);
+
+// XCD-aware grid: number of blocks for a grid-stride element loop
+int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu);
+
+} // namespace sodhip

@@ -1,0 +1,797 @@
+// ops_element.hip -- element kernels of the split-form convection, the viscous
+// operator and the generic scalar convection, the node-ordered assembly gather,
+// and their drop-in C-ABI entry points.
+//
+// Layout and mapping (gfx950):
+//  * one element per workgroup of NT = roundup((p+1)^3, 64) lanes; lane a owns
+//    element-local node / quadrature point a (the reference's gmsh order);
+//  * the element's nodal fields are staged field-major in LDS; each lane builds
+//    once, in registers, the LDS offsets of the p other nodes on its three GLL
+//    lines and the matching rows of the derivative table (and of its transpose
+//    for the weak divergence of the viscous fluxes);
+//  * blocks walk the elements in a grid-stride loop whose block -> element map
+//    gives each XCD (blockIdx % 8) one contiguous eighth of the elements, so
+//    nodes shared by neighbouring elements are re-read from the same L2;
+//  * element residuals go to a per element-node scratch array (SoA); a second
+//    kernel gathers them per node through the node-ordered incidence table in
+//    the reference's serial (element, node) order -- deterministic, no atomics.
+//
+// Arithmetic restatement: the split forms are evaluated through identities that
+// hold for arbitrary nodal inputs (linearity of the discrete derivative and
+// reordering of finite sums), which remove redundant work of the reference loop:
+//  * the enthalpy-slot and kinetic-energy terms of the energy equation share one
+//    derivative of H = E + k and of rho*H;
+//  * advective derivatives a.grad(f) use contravariant components
+//    sum_j He(j,k) a_j at the quadrature point;
+//  * flux divergences sum_j d_j(s u_j) are evaluated from the line values of s
+//    times the contravariant velocity sum_j He_g(j,k) u_j(m) at the line node m.
+#include <cstdio>
+
+#include "common.hpp"
+
+namespace sodhip {
+
+template <int P> struct EG {
+    static constexpr int N1 = P + 1;
+    static constexpr int NN = N1 * N1 * N1;
+    static constexpr int NT = (NN + 63) / 64 * 64;
+};
+
+// Per-lane line tables: the P other nodes of the lane's three GLL lines.
+template <int P, typename T> struct LineTab {
+    int off[3][P];
+    T ds[3], d[3][P];   // d/dxi_k at the lane's point: own and other coefficients
+    T dsT[3], dT[3][P]; // transposed coefficients for the weak divergence
+};
+
+template <int P, typename T>
+__device__ __forceinline__ void build_linetab(int a, const T *__restrict__ dl,
+                                              const int *__restrict__ inv,
+                                              const int *__restrict__ gI,
+                                              const int *__restrict__ gJ,
+                                              const int *__restrict__ gK, LineTab<P, T> &t)
+{
+    constexpr int N1 = EG<P>::N1, NN = EG<P>::NN;
+    // indices are clamped so that malformed tables cannot address outside the
+    // element's arrays
+    const int own[3] = {min(max(gI[a] - 1, 0), P), min(max(gJ[a] - 1, 0), P),
+                        min(max(gK[a] - 1, 0), P)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t.ds[k] = dl[a + NN * (k + 3 * own[k])];
+        t.dsT[k] = dl[a + NN * (k + 3 * own[k])];
+#pragma unroll
+        for (int o = 0; o < P; ++o) {
+            const int m = o + (o >= own[k] ? 1 : 0);
+            int idx[3] = {own[0], own[1], own[2]};
+            idx[k] = m;
+            const int b = min(max(inv[idx[0] + N1 * (idx[1] + N1 * idx[2])] - 1, 0), NN - 1);
+            t.off[k][o] = b;
+            t.d[k][o] = dl[a + NN * (k + 3 * m)];
+            t.dT[k][o] = dl[b + NN * (k + 3 * own[k])];
+        }
+    }
+}
+
+// XCD-aware grid-stride element range of this block.
+struct ElemRange {
+    int begin, end, stride;
+};
+__device__ __forceinline__ ElemRange elem_range(int nelem)
+{
+    const int nblk = gridDim.x, b = blockIdx.x;
+    const int per = nblk >> 3; // grid is a multiple of 8
+    const int xcd = b & 7, r = b >> 3;
+    const int chunk = (nelem + 7) >> 3;
+    int e0 = xcd * chunk;
+    int e1 = min(nelem, e0 + chunk);
+    return {e0 + r, e1, per};
+}
+
+template <typename T> struct RhsArgs {
+    int nelem, npoin;
+    size_t EN;
+    const int *connecT;
+    const T *He, *gpvol, *dl;
+    const int *inv, *gI, *gJ, *gK;
+    // primitive inputs (drop-in operators)
+    const T *u, *q, *rho, *pr, *E;
+    const T *rho_n, *Tem, *mu_fluid, *mu_e, *mu_sgs;
+    T Cp, Pr, c_rho, c_ener;
+    // conserved inputs (fused stage): FP64 state
+    const double *cr, *cq, *cE;
+    double gamma, Rgas, mu_factor;
+    T *out; // SoA [5][EN]: mass, mom1, mom2, mom3, energy
+};
+
+// Sutherland's law (mod_fluid_viscosity.f90:21-41)
+template <typename T> __device__ __forceinline__ T sutherland(T mu_factor, T Tk)
+{
+    return mu_factor * T(0.000001458) * (Tk * sqrt(Tk)) / (Tk + T(110.4));
+}
+
+// CONV: split-form convection (full_convec_ijk); DIFF: viscous operator
+// (full_diffusion_ijk); CONS: inputs are the conserved FP64 state (the fused
+// stage computes the primitives of updateF's EOS block in-kernel) instead of the
+// primitive arrays.
+template <int P, typename T, bool CONV, bool DIFF, bool CONS>
+__global__ __launch_bounds__(EG<P>::NT) void k_rhs(RhsArgs<T> A)
+{
+    constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
+    // pass-1 fields
+    constexpr int F_RHO = 0, F_U = 1, F_T = 4;
+    constexpr int F_C = DIFF ? 5 : 4; // first convection-only field
+    constexpr int F_P = F_C, F_H = F_C + 1, F_RH = F_C + 2, F_Q = F_C + 3;
+    constexpr int NF1 = CONV ? F_C + 6 : F_C;
+    constexpr int NF2 = DIFF ? 18 : 0;
+    __shared__ T s1[NF1 * NT];
+    __shared__ T s2[(NF2 > 0 ? NF2 : 1) * NT];
+
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    LineTab<P, T> lt;
+    build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+
+    const ElemRange er = elem_range(A.nelem);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? A.connecT[ea] : 0;
+        // ---------------- own node values
+        T rho, u[3], Tk = T(0), muf = T(0), rhon = T(0), mue = T(0), musgs = T(0);
+        T p = T(0), H = T(0), rhoH = T(0), q[3] = {T(0), T(0), T(0)};
+        if constexpr (CONS) {
+            const double r = A.cr[n];
+            const double qq[3] = {A.cq[n], A.cq[n + A.npoin], A.cq[n + 2 * (size_t)A.npoin]};
+            const double EE = A.cE[n];
+            double ud[3], umag = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                ud[d] = qq[d] / r;
+                umag += ud[d] * ud[d];
+            }
+            const double eint = EE / r - 0.5 * umag;
+            const double pd = r * (A.gamma - 1.0) * eint;
+            const double hd = (A.gamma / (A.gamma - 1.0)) * pd / r;
+            const double Td = pd / (r * A.Rgas);
+            rho = T(r);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                u[d] = T(ud[d]);
+                q[d] = T(qq[d]);
+            }
+            p = T(pd);
+            const T kin = T(0.5 * umag);
+            H = T(hd) + kin;
+            rhoH = rho * H;
+            Tk = T(Td);
+            rhon = rho;
+            muf = T(sutherland<double>(A.mu_factor, Td));
+            mue = A.mu_e[e + (size_t)A.nelem * a];
+        } else {
+            rho = A.rho[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] = A.u[n + (size_t)A.npoin * d];
+            if constexpr (CONV) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) q[d] = A.q[n + (size_t)A.npoin * d];
+                p = A.pr[n];
+                const T kin = u[0] * u[0] * T(0.5) + u[1] * u[1] * T(0.5) + u[2] * u[2] * T(0.5);
+                H = A.E[n] + kin;
+                rhoH = rho * H;
+            }
+            if constexpr (DIFF) {
+                Tk = A.Tem[n];
+                rhon = A.rho_n[n];
+                muf = A.mu_fluid[n];
+                mue = A.mu_e[e + (size_t)A.nelem * a];
+                musgs = A.mu_sgs ? A.mu_sgs[e + (size_t)A.nelem * a] : T(0);
+            }
+        }
+        // ---------------- stage nodal fields in LDS
+        if (act) {
+            s1[F_RHO * NT + a] = rho;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s1[(F_U + d) * NT + a] = u[d];
+            if constexpr (DIFF) s1[F_T * NT + a] = Tk;
+            if constexpr (CONV) {
+                s1[F_P * NT + a] = p;
+                s1[F_H * NT + a] = H;
+                s1[F_RH * NT + a] = rhoH;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s1[(F_Q + d) * NT + a] = q[d];
+            }
+        }
+        // element geometry at the lane's point
+        T He[3][3];
+        const T *hp = A.He + 9 * ea;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) He[i][j] = act ? hp[i + 3 * j] : T(0);
+        const T vol = act ? A.gpvol[ea] : T(0);
+        __syncthreads();
+
+        // contravariant components at the point: Ut_k = sum_j He(j,k) u_j
+        T Ut[3], Qt[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
+            Qt[k] = He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2];
+        }
+        // convection accumulators
+        T urho = 0, gp[3] = {0, 0, 0}, qH = 0, urhoH = 0, qgu[3] = {0, 0, 0}, ugq[3] = {0, 0, 0};
+        T divQ = 0, divFq[3] = {0, 0, 0}, divFuu[3] = {0, 0, 0}, divRHu = 0, divHu = 0, divUw = 0;
+        // diffusion accumulators: physical gradients
+        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const T c0 = lt.ds[k];
+            T dr = c0 * rho, du[3], dT = c0 * Tk, dp = c0 * p, dH = c0 * H, dRH = c0 * rhoH, dq[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                du[d] = c0 * u[d];
+                dq[d] = c0 * q[d];
+            }
+            if constexpr (CONV) {
+                const T Wo = c0 * Ut[k];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    divFq[d] += Wo * q[d];
+                    divFuu[d] += Wo * u[d];
+                }
+                divRHu += Wo * rhoH;
+                divHu += Wo * H;
+                divUw += Wo;
+            }
+#pragma unroll
+            for (int o = 0; o < P; ++o) {
+                const int b = lt.off[k][o];
+                const T c = lt.d[k][o];
+                const T rb = s1[F_RHO * NT + b];
+                T ub[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) ub[d] = s1[(F_U + d) * NT + b];
+                dr += c * rb;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) du[d] += c * ub[d];
+                if constexpr (DIFF) dT += c * s1[F_T * NT + b];
+                if constexpr (CONV) {
+                    const T pb = s1[F_P * NT + b], Hb = s1[F_H * NT + b], RHb = s1[F_RH * NT + b];
+                    T qb[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) qb[d] = s1[(F_Q + d) * NT + b];
+                    dp += c * pb;
+                    dH += c * Hb;
+                    dRH += c * RHb;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) dq[d] += c * qb[d];
+                    const T W = c * (He[0][k] * ub[0] + He[1][k] * ub[1] + He[2][k] * ub[2]);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        divFq[d] += W * qb[d];
+                        divFuu[d] += W * ub[d];
+                    }
+                    divRHu += W * RHb;
+                    divHu += W * Hb;
+                    divUw += W;
+                }
+            }
+            if constexpr (CONV) {
+                urho += Ut[k] * dr;
+                qH += Qt[k] * dH;
+                urhoH += Ut[k] * dRH;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    gp[i] += He[i][k] * dp;
+                    qgu[i] += Qt[k] * du[i];
+                    ugq[i] += Ut[k] * dq[i];
+                }
+                divQ += He[0][k] * dq[0] + He[1][k] * dq[1] + He[2][k] * dq[2];
+            }
+            if constexpr (DIFF) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    gR[i] += He[i][k] * dr;
+                    gT[i] += He[i][k] * dT;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) gU[i][j] += He[j][k] * du[i];
+                }
+            }
+        }
+
+        T Rm = 0, Rq[3] = {0, 0, 0}, Re = 0;
+        if constexpr (CONV) {
+            Rm = T(0.5) * (divQ + rho * divUw) + T(0.5) * urho;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                Rq[i] = T(0.25) * (divFq[i] + u[i] * divQ + q[i] * divUw + rho * divFuu[i]) + gp[i] +
+                        T(0.25) * (u[i] * urho + qgu[i] + ugq[i]);
+            Re = T(0.25) * (divRHu + H * divQ + rhoH * divUw + rho * divHu) +
+                 T(0.25) * (qH + urhoH + H * urho);
+            Rm *= vol;
+            Re *= vol;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Rq[i] *= vol;
+        }
+
+        if constexpr (DIFF) {
+            // pass 1: viscous fluxes at the point, mapped to contravariant
+            // components G_k(F) = gpvol * sum_d He(d,k) F_d
+            const T mufg = muf + rhon * musgs;
+            const T mut = mufg + mue;
+            const T divU = gU[0][0] + gU[1][1] + gU[2][2];
+            const T tt = mufg * T(2.0 / 3.0) * divU;
+            T tau[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tau[i][i] -= tt;
+            T tu[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tu[i] = tau[i][0] * u[0] + tau[i][1] * u[1] + tau[i][2] * u[2];
+            T vHe[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) vHe[i][j] = vol * He[i][j];
+            T G[18];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                G[0 + k] = vHe[0][k] * gR[0] + vHe[1][k] * gR[1] + vHe[2][k] * gR[2];
+                G[3 + k] = vHe[0][k] * gT[0] + vHe[1][k] * gT[1] + vHe[2][k] * gT[2];
+                G[6 + k] = vHe[0][k] * tu[0] + vHe[1][k] * tu[1] + vHe[2][k] * tu[2];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    G[9 + 3 * r + k] = vHe[0][k] * tau[r][0] + vHe[1][k] * tau[r][1] + vHe[2][k] * tau[r][2];
+            }
+            if (act) {
+#pragma unroll
+                for (int f = 0; f < 18; ++f) s2[f * NT + a] = G[f];
+            }
+            __syncthreads();
+            // pass 2: weak divergence WD(F)(a) = sum_k sum_{b on k-line} dl(b,k,I_k(a)) G_k(F)(b)
+            T W6[6];
+#pragma unroll
+            for (int fl = 0; fl < 6; ++fl) {
+                T acc = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int f = 3 * fl + k;
+                    acc += lt.dsT[k] * G[f];
+#pragma unroll
+                    for (int o = 0; o < P; ++o) acc += lt.dT[k][o] * s2[f * NT + lt.off[k][o]];
+                }
+                W6[fl] = acc;
+            }
+            const T nu_e = A.c_rho * mue / rhon;
+            const T kappa = muf * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr +
+                            A.Cp * rhon * musgs / T(0.9);
+            Rm += nu_e * W6[0];
+            Re += W6[2] + kappa * W6[1];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Rq[i] += W6[3 + i];
+        }
+
+        if (act) {
+            T *o = A.out;
+            o[ea] = Rm;
+            o[A.EN + ea] = Rq[0];
+            o[2 * A.EN + ea] = Rq[1];
+            o[3 * A.EN + ea] = Rq[2];
+            o[4 * A.EN + ea] = Re;
+        }
+        __syncthreads();
+    }
+}
+
+// generic_scalar_convec_ijk (elem_convec.f90:472-594):
+// Re = gpvol [ 1/2 (div f + eta div u) + 1/2 u.grad(eta) ]
+template <int P, typename T>
+__global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const T *__restrict__ f,
+                                                             const T *__restrict__ eta)
+{
+    constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
+    __shared__ T s[7 * NT]; // eta, u(3), f(3)
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    LineTab<P, T> lt;
+    build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const ElemRange er = elem_range(A.nelem);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? A.connecT[ea] : 0;
+        const T et = eta[n];
+        T u[3], fv[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = A.u[n + (size_t)A.npoin * d];
+            fv[d] = f[n + (size_t)A.npoin * d];
+        }
+        if (act) {
+            s[a] = et;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                s[(1 + d) * NT + a] = u[d];
+                s[(4 + d) * NT + a] = fv[d];
+            }
+        }
+        T He[3][3];
+        const T *hp = A.He + 9 * ea;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) He[i][j] = act ? hp[i + 3 * j] : T(0);
+        const T vol = act ? A.gpvol[ea] : T(0);
+        __syncthreads();
+        T gE[3] = {0, 0, 0}, divU = 0, divF = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const T c0 = lt.ds[k];
+            T de = c0 * et, du[3], df[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                du[d] = c0 * u[d];
+                df[d] = c0 * fv[d];
+            }
+#pragma unroll
+            for (int o = 0; o < P; ++o) {
+                const int b = lt.off[k][o];
+                const T c = lt.d[k][o];
+                de += c * s[b];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    du[d] += c * s[(1 + d) * NT + b];
+                    df[d] += c * s[(4 + d) * NT + b];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                gE[i] += He[i][k] * de;
+                divU += He[i][k] * du[i];
+                divF += He[i][k] * df[i];
+            }
+        }
+        T r = T(0.5) * (divF + et * divU);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) r += T(0.5) * (u[i] * gE[i]);
+        if (act) A.out[ea] = vol * r;
+        __syncthreads();
+    }
+}
+
+// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(col_j).
+template <typename TS, typename TR>
+__global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict__ row,
+                                                const int *__restrict__ col,
+                                                const TS *__restrict__ scr, size_t EN, int nf,
+                                                TR *__restrict__ R0, TR *__restrict__ R1,
+                                                TR *__restrict__ R2, TR *__restrict__ R3,
+                                                TR *__restrict__ R4, int init, TR fact)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npoin) return;
+    const int j0 = row[n], j1 = row[n + 1];
+    TR *R[5] = {R0, R1, R2, R3, R4};
+    for (int f = 0; f < nf; ++f) {
+        TR acc = init ? TR(0) : R[f][n];
+        const TS *s = scr + (size_t)f * EN;
+        for (int j = j0; j < j1; ++j) acc += fact * TR(s[col[j]]);
+        R[f][n] = acc;
+    }
+}
+
+// ---------------------------------------------------------------- launch helpers
+template <typename T>
+static int launch_rhs(sodhip_ctx ctx, int porder, bool conv, bool diff, bool cons,
+                      const RhsArgs<T> &A, int grid)
+{
+#define SODHIP_RHS_CASE(PP)                                                                \
+    case PP:                                                                               \
+        if (conv && diff && cons)                                                          \
+            k_rhs<PP, T, true, true, true><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A);       \
+        else if (conv && !diff && !cons)                                                   \
+            k_rhs<PP, T, true, false, false><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A);     \
+        else if (!conv && diff && !cons)                                                   \
+            k_rhs<PP, T, false, true, false><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A);     \
+        else                                                                               \
+            return set_error(SODHIP_ERR_UNSUPPORTED, "unsupported operator combination");  \
+        break;
+    switch (porder) {
+        SODHIP_RHS_CASE(2)
+        SODHIP_RHS_CASE(3)
+        SODHIP_RHS_CASE(4)
+    default:
+        return set_error(SODHIP_ERR_UNSUPPORTED, "porder %d not supported (2..4)", porder);
+    }
+#undef SODHIP_RHS_CASE
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+template <typename TS, typename TR>
+static int launch_gather(sodhip_ctx ctx, const MeshCache *m, const TS *scr, size_t EN, int nf,
+                         TR *R0, TR *R1, TR *R2, TR *R3, TR *R4, int init, TR fact)
+{
+    const int bs = 256;
+    const int grid = (m->npoin + bs - 1) / bs;
+    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, m->col, scr, EN, nf, R0, R1,
+                                                   R2, R3, R4, init, fact);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+template <typename T>
+static int check_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && connec, "invalid mesh arguments");
+    return SODHIP_OK;
+}
+
+template <typename T>
+static int full_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                       const T *He, const T *gpvol, const T *dl, const int *inv, const int *gI,
+                       const int *gJ, const int *gK, const T *u, const T *q, const T *rho,
+                       const T *pr, const T *E, T *Rmass, T *Rmom, T *Rener, int initialize,
+                       T fact)
+{
+    int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
+    if (rc) return rc;
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && u && q && rho && pr && E &&
+                       Rmass && Rmom && Rener,
+                   "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(T)))) return rc;
+    RhsArgs<T> A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.EN = EN;
+    A.connecT = m->connecT;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.u = u;
+    A.q = q;
+    A.rho = rho;
+    A.pr = pr;
+    A.E = E;
+    A.out = (T *)ctx->scratch.ptr;
+    int tok = prof_begin(ctx, 0);
+    rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, nelem, 16));
+    prof_end(ctx, tok);
+    if (rc) return rc;
+    tok = prof_begin(ctx, 1);
+    rc = launch_gather<T, T>(ctx, m, A.out, EN, 5, Rmass, Rmom, Rmom + npoin,
+                             Rmom + 2 * (size_t)npoin, Rener, initialize, fact);
+    prof_end(ctx, tok);
+    return rc;
+}
+
+template <typename T>
+static int full_diffusion(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                          const T *He, const T *gpvol, const T *dl, const int *inv, const int *gI,
+                          const int *gJ, const int *gK, T Cp, T Pr, const T *rho_n, const T *rho,
+                          const T *u, const T *Tem, const T *mu_fluid, const T *mu_e,
+                          const T *mu_sgs, T c_rho, T c_ener, T *Rmass, T *Rmom, T *Rener,
+                          int initialize, T fact)
+{
+    int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
+    if (rc) return rc;
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && rho_n && rho && u && Tem &&
+                       mu_fluid && mu_e && Rmass && Rmom && Rener,
+                   "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(T)))) return rc;
+    RhsArgs<T> A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.EN = EN;
+    A.connecT = m->connecT;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.rho = rho;
+    A.u = u;
+    A.rho_n = rho_n;
+    A.Tem = Tem;
+    A.mu_fluid = mu_fluid;
+    A.mu_e = mu_e;
+    A.mu_sgs = mu_sgs;
+    A.Cp = Cp;
+    A.Pr = Pr;
+    A.c_rho = c_rho;
+    A.c_ener = c_ener;
+    A.out = (T *)ctx->scratch.ptr;
+    int tok = prof_begin(ctx, 0);
+    rc = launch_rhs<T>(ctx, porder, false, true, false, A, elem_grid(ctx, nelem, 16));
+    prof_end(ctx, tok);
+    if (rc) return rc;
+    tok = prof_begin(ctx, 1);
+    rc = launch_gather<T, T>(ctx, m, A.out, EN, 5, Rmass, Rmom, Rmom + npoin,
+                             Rmom + 2 * (size_t)npoin, Rener, initialize, fact);
+    prof_end(ctx, tok);
+    return rc;
+}
+
+// Fused RHS used by the stage driver (ops_stage.hip): convection + viscous
+// operator from the conserved FP64 state; writes element partials only.
+template <typename T>
+int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He,
+                       const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
+                       const int *gK, const double *rho, const double *q, const double *E,
+                       const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
+                       double mu_factor, double c_rho, double c_ener, T *out)
+{
+    RhsArgs<T> A{};
+    A.nelem = m->nelem;
+    A.npoin = m->npoin;
+    A.EN = (size_t)m->nelem * nnode_of(porder);
+    A.connecT = m->connecT;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.cr = rho;
+    A.cq = q;
+    A.cE = E;
+    A.mu_e = mu_e;
+    A.gamma = gamma;
+    A.Rgas = Rgas;
+    A.mu_factor = mu_factor;
+    A.Cp = T(Cp);
+    A.Pr = T(Pr);
+    A.c_rho = T(c_rho);
+    A.c_ener = T(c_ener);
+    A.out = out;
+    int tok = prof_begin(ctx, 0);
+    int rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 16));
+    prof_end(ctx, tok);
+    return rc;
+}
+template int rhs_fused_elements<double>(sodhip_ctx, int, const MeshCache *, const double *,
+                                        const double *, const double *, const int *, const int *,
+                                        const int *, const int *, const double *, const double *,
+                                        const double *, const double *, double, double, double,
+                                        double, double, double, double, double *);
+template int rhs_fused_elements<float>(sodhip_ctx, int, const MeshCache *, const float *,
+                                       const float *, const float *, const int *, const int *,
+                                       const int *, const int *, const double *, const double *,
+                                       const double *, const float *, double, double, double,
+                                       double, double, double, double, float *);
+
+template <typename T>
+static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                         const T *He, const T *gpvol, const T *dl, const int *inv, const int *gI,
+                         const int *gJ, const int *gK, const T *q, const T *eta, const T *u,
+                         T *Rconvec, int initialize, T fact)
+{
+    int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
+    if (rc) return rc;
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && q && eta && u && Rconvec,
+                   "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, EN * sizeof(T)))) return rc;
+    RhsArgs<T> A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.EN = EN;
+    A.connecT = m->connecT;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.u = u;
+    A.out = (T *)ctx->scratch.ptr;
+    const int grid = elem_grid(ctx, nelem, 16);
+    int tok = prof_begin(ctx, 3);
+    switch (porder) {
+    case 2: k_scalar_convec<2, T><<<grid, EG<2>::NT, 0, ctx->stream>>>(A, q, eta); break;
+    case 3: k_scalar_convec<3, T><<<grid, EG<3>::NT, 0, ctx->stream>>>(A, q, eta); break;
+    case 4: k_scalar_convec<4, T><<<grid, EG<4>::NT, 0, ctx->stream>>>(A, q, eta); break;
+    }
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return launch_gather<T, T>(ctx, m, A.out, EN, 1, Rconvec, nullptr, nullptr, nullptr, nullptr,
+                               initialize, fact);
+}
+
+} // namespace sodhip
+
+using namespace sodhip;
+
+extern "C" int sodhip_full_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                          const int *connec, const double *He,
+                                          const double *gpvol, const double *dlxigp_ip,
+                                          const int *invAtoIJK, const int *gmshAtoI,
+                                          const int *gmshAtoJ, const int *gmshAtoK,
+                                          const double *u, const double *q, const double *rho,
+                                          const double *pr, const double *E, double *Rmass,
+                                          double *Rmom, double *Rener, int initialize,
+                                          double fact)
+{
+    return full_convec<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                               gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener,
+                               initialize, fact);
+}
+
+extern "C" int sodhip_full_convec_ijk_f32(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                          const int *connec, const float *He, const float *gpvol,
+                                          const float *dlxigp_ip, const int *invAtoIJK,
+                                          const int *gmshAtoI, const int *gmshAtoJ,
+                                          const int *gmshAtoK, const float *u, const float *q,
+                                          const float *rho, const float *pr, const float *E,
+                                          float *Rmass, float *Rmom, float *Rener, int initialize,
+                                          float fact)
+{
+    return full_convec<float>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                              gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener,
+                              initialize, fact);
+}
+
+extern "C" int sodhip_full_diffusion_ijk_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, const double *He,
+    const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+    const int *gmshAtoJ, const int *gmshAtoK, double Cp, double Pr, const double *rho_n,
+    const double *rho, const double *u, const double *Tem, const double *mu_fluid,
+    const double *mu_e, const double *mu_sgs, double c_rho, double c_ener, double *Rmass,
+    double *Rmom, double *Rener, int initialize, double fact)
+{
+    return full_diffusion<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                  invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho_n, rho, u,
+                                  Tem, mu_fluid, mu_e, mu_sgs, c_rho, c_ener, Rmass, Rmom, Rener,
+                                  initialize, fact);
+}
+
+extern "C" int sodhip_full_diffusion_ijk_f32(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, const float *He,
+    const float *gpvol, const float *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+    const int *gmshAtoJ, const int *gmshAtoK, float Cp, float Pr, const float *rho_n,
+    const float *rho, const float *u, const float *Tem, const float *mu_fluid, const float *mu_e,
+    const float *mu_sgs, float c_rho, float c_ener, float *Rmass, float *Rmom, float *Rener,
+    int initialize, float fact)
+{
+    return full_diffusion<float>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                 invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho_n, rho, u,
+                                 Tem, mu_fluid, mu_e, mu_sgs, c_rho, c_ener, Rmass, Rmom, Rener,
+                                 initialize, fact);
+}
+
+extern "C" int sodhip_generic_scalar_convec_ijk_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, const double *He,
+    const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+    const int *gmshAtoJ, const int *gmshAtoK, const double *q, const double *eta,
+    const double *u, double *Rconvec, int initialize, double fact)
+{
+    return scalar_convec<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                 invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, q, eta, u, Rconvec,
+                                 initialize, fact);
+}

@@ -1,0 +1,331 @@
+// runtime.hip -- device context, OpenACC-style present table, cached
+// connectivity derivatives (element-major connectivity and node-ordered
+// incidence), launch timing and error reporting.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sodhip {
+
+static thread_local char g_err[1024] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int ensure_buf(DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.ptr) return SODHIP_OK;
+    if (b.ptr) {
+        SODHIP_TRY_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return set_error(SODHIP_ERR_ALLOC, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return SODHIP_OK;
+}
+
+int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu)
+{
+    // a multiple of 8 so that every XCD receives the same number of blocks
+    long g = (long)ctx->num_cu * blocks_per_cu;
+    long need = ((long)nelem + 7) / 8 * 8;
+    g = std::min(g, need);
+    g = std::max<long>(8, (g + 7) / 8 * 8);
+    return (int)g;
+}
+
+static void free_mesh(MeshCache *m)
+{
+    if (!m) return;
+    (void)hipFree(m->connecT);
+    (void)hipFree(m->row);
+    (void)hipFree(m->col);
+}
+
+int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+             MeshCache **out)
+{
+    for (auto &m : ctx->meshes)
+        if (m->connec_key == connec && m->porder == porder && m->nelem == nelem &&
+            m->npoin == npoin) {
+            *out = m.get();
+            return SODHIP_OK;
+        }
+    const int nn = nnode_of(porder);
+    const size_t en = (size_t)nelem * nn;
+    SODHIP_REQUIRE(en < (size_t)1 << 31, "nelem*nnode exceeds 2^31");
+    std::vector<int> h(en);
+    SODHIP_TRY_HIP(hipMemcpyAsync(h.data(), connec, en * sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> cT(en), row((size_t)npoin + 1, 0), col(en);
+    for (size_t i = 0; i < en; ++i) {
+        int v = h[i];
+        if (v < 1 || v > npoin)
+            return set_error(SODHIP_ERR_ARG, "connec entry %d outside 1..npoin=%d", v, npoin);
+    }
+    for (long e = 0; e < nelem; ++e)
+        for (int a = 0; a < nn; ++a) {
+            int n = h[(size_t)a * nelem + e] - 1;
+            cT[(size_t)e * nn + a] = n;
+            row[(size_t)n + 1]++;
+        }
+    for (long n = 0; n < npoin; ++n) row[n + 1] += row[n];
+    std::vector<int> fill(row.begin(), row.end() - 1);
+    for (long e = 0; e < nelem; ++e) // ascending (e,a): the reference's serial assembly order
+        for (int a = 0; a < nn; ++a) {
+            int n = cT[(size_t)e * nn + a];
+            col[fill[n]++] = (int)(e * nn + a);
+        }
+    auto m = std::make_unique<MeshCache>();
+    m->connec_key = connec;
+    m->porder = porder;
+    m->nelem = nelem;
+    m->npoin = npoin;
+    hipError_t e1 = hipMalloc(&m->connecT, en * sizeof(int));
+    hipError_t e2 = hipMalloc(&m->row, ((size_t)npoin + 1) * sizeof(int));
+    hipError_t e3 = hipMalloc(&m->col, en * sizeof(int));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        free_mesh(m.get());
+        return set_error(SODHIP_ERR_ALLOC, "mesh cache allocation failed");
+    }
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->connecT, cT.data(), en * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->row, row.data(), row.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->col, col.data(), en * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m.get();
+    ctx->meshes.push_back(std::move(m));
+    return SODHIP_OK;
+}
+
+int prof_begin(sodhip_ctx ctx, int kind)
+{
+    if (!ctx->prof) return -1;
+    hipEvent_t ev[2];
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->event_pool.empty()) {
+            ev[i] = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+        } else if (hipEventCreate(&ev[i]) != hipSuccess) {
+            return -1;
+        }
+    }
+    (void)hipEventRecord(ev[0], ctx->stream);
+    ctx->pending.push_back({kind, ev[0], ev[1]});
+    return (int)ctx->pending.size() - 1;
+}
+
+void prof_end(sodhip_ctx ctx, int token)
+{
+    if (token < 0) return;
+    (void)hipEventRecord(ctx->pending[token].b, ctx->stream);
+}
+
+static void prof_drain(sodhip_ctx ctx)
+{
+    if (ctx->pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ctx->slots[p.kind].total_ms += ms;
+            ctx->slots[p.kind].count += 1;
+        }
+        ctx->event_pool.push_back(p.a);
+        ctx->event_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+}
+
+} // namespace sodhip
+
+using namespace sodhip;
+
+extern "C" int sodhip_version(void) { return 10000; }
+
+extern "C" int sodhip_last_error(char *buf, int len)
+{
+    if (buf && len > 0) {
+        strncpy(buf, g_err, (size_t)len - 1);
+        buf[len - 1] = '\0';
+    }
+    return g_err[0] ? 1 : 0;
+}
+
+extern "C" int sodhip_create(int device, sodhip_ctx *out)
+{
+    SODHIP_REQUIRE(out, "null context pointer");
+    int n = 0;
+    SODHIP_TRY_HIP(hipGetDeviceCount(&n));
+    SODHIP_REQUIRE(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
+    SODHIP_TRY_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SODHIP_TRY_HIP(hipGetDeviceProperties(&prop, device));
+    auto *c = new sodhip_ctx_s();
+    c->device = device;
+    c->stream = nullptr;
+    c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    *out = c;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_destroy(sodhip_ctx ctx)
+{
+    if (!ctx) return SODHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &m : ctx->meshes) free_mesh(m.get());
+    for (auto &kv : ctx->present) (void)hipFree(kv.second.dev);
+    (void)hipFree(ctx->scratch.ptr);
+    (void)hipFree(ctx->scratch2.ptr);
+    if (ctx->hostbuf.ptr) (void)hipHostFree(ctx->hostbuf.ptr);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    delete ctx;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_set_stream(sodhip_ctx ctx, void *stream)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    ctx->stream = (hipStream_t)stream;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_synchronize(sodhip_ctx ctx)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return SODHIP_OK;
+}
+
+// ---------------------------------------------------------------- present table
+static PresentEntry *find_present(sodhip_ctx ctx, const void *host, size_t *offset)
+{
+    uintptr_t h = (uintptr_t)host;
+    auto it = ctx->present.upper_bound(h);
+    if (it == ctx->present.begin()) return nullptr;
+    --it;
+    if (h >= it->first && h < it->first + it->second.bytes) {
+        if (offset) *offset = h - it->first;
+        return &it->second;
+    }
+    return nullptr;
+}
+
+extern "C" int sodhip_enter_data(sodhip_ctx ctx, const void *host, size_t bytes, int copyin)
+{
+    SODHIP_REQUIRE(ctx && host && bytes > 0, "invalid enter_data arguments");
+    if (find_present(ctx, host, nullptr))
+        return set_error(SODHIP_ERR_ARG, "host range at %p is already present", host);
+    PresentEntry e;
+    e.bytes = bytes;
+    hipError_t rc = hipMalloc(&e.dev, bytes);
+    if (rc != hipSuccess) return set_error(SODHIP_ERR_ALLOC, "hipMalloc(%zu) failed", bytes);
+    if (copyin) {
+        SODHIP_TRY_HIP(hipMemcpyAsync(e.dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+        SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->present[(uintptr_t)host] = e;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_update_device(sodhip_ctx ctx, const void *host, size_t bytes)
+{
+    size_t off = 0;
+    PresentEntry *e = ctx ? find_present(ctx, host, &off) : nullptr;
+    if (!e) return set_error(SODHIP_ERR_NOT_PRESENT, "host address %p is not present", host);
+    SODHIP_REQUIRE(off + bytes <= e->bytes, "update_device beyond the present range");
+    SODHIP_TRY_HIP(hipMemcpyAsync((char *)e->dev + off, host, bytes, hipMemcpyHostToDevice,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_update_host(sodhip_ctx ctx, void *host, size_t bytes)
+{
+    size_t off = 0;
+    PresentEntry *e = ctx ? find_present(ctx, host, &off) : nullptr;
+    if (!e) return set_error(SODHIP_ERR_NOT_PRESENT, "host address %p is not present", host);
+    SODHIP_REQUIRE(off + bytes <= e->bytes, "update_host beyond the present range");
+    SODHIP_TRY_HIP(hipMemcpyAsync(host, (char *)e->dev + off, bytes, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_exit_data(sodhip_ctx ctx, const void *host, int copyout)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    auto it = ctx->present.find((uintptr_t)host);
+    if (it == ctx->present.end())
+        return set_error(SODHIP_ERR_NOT_PRESENT, "host address %p is not the start of a present range",
+                         host);
+    if (copyout) {
+        SODHIP_TRY_HIP(hipMemcpyAsync((void *)host, it->second.dev, it->second.bytes,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    }
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    SODHIP_TRY_HIP(hipFree(it->second.dev));
+    ctx->present.erase(it);
+    return SODHIP_OK;
+}
+
+extern "C" void *sodhip_deviceptr(sodhip_ctx ctx, const void *host)
+{
+    if (!ctx || !host) return nullptr;
+    size_t off = 0;
+    PresentEntry *e = find_present(ctx, host, &off);
+    if (!e) {
+        set_error(SODHIP_ERR_NOT_PRESENT, "host address %p is not present", host);
+        return nullptr;
+    }
+    return (char *)e->dev + off;
+}
+
+// ---------------------------------------------------------------- timing
+extern "C" int sodhip_prof_enable(sodhip_ctx ctx, int enable)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    ctx->prof = enable != 0;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_prof_get(sodhip_ctx ctx, int kind, double *total_ms, long long *count)
+{
+    SODHIP_REQUIRE(ctx && kind >= 0 && kind < kProfKinds, "invalid prof query");
+    prof_drain(ctx);
+    if (total_ms) *total_ms = ctx->slots[kind].total_ms;
+    if (count) *count = ctx->slots[kind].count;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_prof_reset(sodhip_ctx ctx)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    prof_drain(ctx);
+    for (auto &s : ctx->slots) s = ProfSlot();
+    return SODHIP_OK;
+}
