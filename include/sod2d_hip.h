@@ -201,12 +201,99 @@ int sodhip_lumped_solver_vect_f64(sodhip_ctx ctx, int npoin, int npoin_w, const 
 /* RK stage update with the lumped-mass inverse fused in (the reference has these
  * as inline loops: mod_solver.f90:40-104 then time_integ_ls.f90:323-331,350-358):
  * over working nodes, R <- R*invMl (written back, as the reference's in-place
- * lumped solve), then y <- lam1*y1 + lams*y + gam*dt*R for rho, E and q(:,1:3). */
+ * lumped solve), then y <- lam1*y1 + lams*y + gam*dt*R for rho, E and q(:,1:3).
+ * invMl == NULL: R already carries the mass inverse and is left unchanged (the
+ * first stage of rk_4_ls_main, time_integ_ls.f90:323-331). */
 int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
                                const double *invMl, double dt, double lam1, double lams,
                                double gam, const double *rho1, const double *q1,
                                const double *E1, double *rho, double *q, double *E,
                                double *Rmass, double *Rmom, double *Rener);
+
+/* ------------------------------------------------------------------------ */
+/* Nodal pieces of the time step (FP64, device pointers)                    */
+/* ------------------------------------------------------------------------ */
+
+/* updateF "Update u and EOS" (time_integ_ls.f90:574-587) over working nodes:
+ * u = q/rho, e_int, pr, aux_h = gamma/(gamma-1) pr/rho (may be NULL), Tem. */
+int sodhip_eos_stage_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                         double gamma_gas, double Rgas, const double *rho, const double *q,
+                         const double *E, double *u, double *e_int, double *pr, double *aux_h,
+                         double *Tem);
+/* rk_4_ls_main's end-of-step EOS (time_integ_ls.f90:391-407): adds csound, machno */
+int sodhip_eos_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                        double gamma_gas, double Rgas, const double *rho, const double *q,
+                        const double *E, double *u, double *e_int, double *pr, double *csound,
+                        double *machno, double *Tem);
+/* sutherland_viscosity (mod_fluid_viscosity.f90:21-41) over all npoin; mu_factor
+ * may be NULL, then the scalar mu_factor_scalar applies to every node. */
+int sodhip_sutherland_f64(sodhip_ctx ctx, int npoin, const double *Tem, const double *mu_factor,
+                          double mu_factor_scalar, double *mu_fluid);
+/* thermo_entropy (mod_entropy_viscosity.f90:411-433); eta(npoin,2), feta(npoin,3) */
+int sodhip_thermo_entropy_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                              double gamma_gas, double p_inf, double rho_inf, const double *rho,
+                              const double *pr, const double *u, double *eta, double *feta);
+/* entropy residual of rk_4_ls_main (time_integ_ls.f90:437-443): Reta(npoin,2),
+ * eta(npoin,2); auxReta = 1.5 Reta2 - 0.5 Reta1 + factor_comp (eta2-eta1)/dt,
+ * then Reta1 <- Reta2. */
+int sodhip_entropy_residual_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                double factor_comp, double dt, double *Reta, const double *eta,
+                                double *auxReta);
+
+/* ------------------------------------------------------------------------ */
+/* Fused Runge-Kutta stage                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double gamma_gas, Rgas, Cp, Pr; /* Pr: the Prt passed by updateF */
+    double mu_factor;               /* Sutherland factor (flag_mu_factor), uniform */
+    double c_rho, c_ener;           /* mod_numerical_params.f90 globals (default 1) */
+    double fact;                    /* residual sign: updateF passes -1 */
+} sodhip_stage_params;
+
+/* One stage of rk_4_ls_main (time_integ_ls.f90:339-361) with updateF
+ * (time_integ_ls.f90:489-691, flag_lps_stab = .false., no sources, one
+ * partition) fused in: at the current state (rho, q, E) it forms the primitives
+ * and Sutherland's viscosity, evaluates R = fact*(full_convec_ijk with the
+ * enthalpy in the energy slot + full_diffusion_ijk with mu_sgs = 0), applies the
+ * lumped inverse, and updates y <- lam1*y1 + lams*y + gam*dt*R over working
+ * nodes.  op_precision 8: He, gpvol, dlxigp_ip and mu_e are double; 4: they are
+ * float and the operator arithmetic runs in FP32, while the state, the gather,
+ * the mass inverse and the stage combine stay FP64.  Rmass/Rmom/Rener (all or
+ * none) receive R*invMl (the reference keeps it for the next step's stage 1). */
+int sodhip_rk_stage_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                        const int *connec, const int *lpoin_w, int op_precision, const void *He,
+                        const void *gpvol, const void *dlxigp_ip, const int *invAtoIJK,
+                        const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                        const void *mu_e, const double *invMl, const sodhip_stage_params *prm,
+                        double dt, double lam1, double lams, double gam, const double *rho1,
+                        const double *q1, const double *E1, double *rho, double *q, double *E,
+                        double *Rmass, double *Rmom, double *Rener);
+
+/* ------------------------------------------------------------------------ */
+/* Time step and diagnostics (results returned to the host; synchronising)   */
+/* ------------------------------------------------------------------------ */
+
+/* adapt_dt_cfl (mod_time_ops.f90:11-63); mu_sgs/mu_e are (nelem,nnode);
+ * mu_fluid == NULL disables the diffusive limit, mu_sgs may be NULL (zero). */
+int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                            const double *helem, const double *u, const double *csound,
+                            double cfl_conv, double cfl_diff, const double *mu_fluid,
+                            const double *mu_sgs, const double *mu_e, const double *rho,
+                            double *dt);
+/* volAvg_EK (mod_analysis.f90:10-44) */
+int sodhip_volavg_ek_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                         const double *gpvol, double rho0, const double *rho, const double *u,
+                         double *EK);
+/* visc_dissipationRate (mod_analysis.f90:46-144) */
+int sodhip_visc_dissipation_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                const int *connec, double rho0, const double *mu_fluid,
+                                const double *mu_e, const double *u, double volT,
+                                const double *gpvol, const double *He, const double *dlxigp_ip,
+                                const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                                const int *gmshAtoK, double *eps_S, double *eps_D, double *eps_T);
+/* maxMach (mod_analysis.f90:146-168) */
+int sodhip_max_mach_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                        const double *machno, double *maxmach);
 
 #ifdef __cplusplus
 }

@@ -327,3 +327,140 @@ def tgv_ic(coord, t):
                     _d(Tem), _d(csound), _d(eta), _d(machno))
     return dict(u=u, q=q, rho=rho, pr=pr, E=E, e_int=e_int, Tem=Tem, csound=csound, eta=eta,
                 machno=machno)
+
+
+# ---------------------------------------------------------------- time loop
+class TGVOracle:
+    """rk_4_ls_main (time_integ_ls.f90:202-487) with updateF (:489-691) and the
+    time loop of CFDSolverBase_evalTimeIteration (CFDSolverBase.f90:1929-2164),
+    restated on the oracle's primitives for the periodic TGV case, with
+    smart_visc_spectral as the entropy-viscosity sensor (the reference's live
+    path calls its _imex variant at the same points).
+
+    ``geo`` provides the mesh and geometry (connec_orig/connec/lpoin_w/coord,
+    He, gpvol, invMl, helem, helem_l, vol_tot) and ``ic`` the initial fields;
+    ``t`` the TGV constants (an oracle TGVParams); ``cfl`` (conv, diff)."""
+
+    def __init__(self, p, geo, ic, t, stages=5, variant=1, parity=True, dt_mode="adaptive",
+                 cfl=(0.5, 0.5), sensor=True):
+        self.p = p
+        self.T = Tables(p)
+        self.geo = geo
+        m = geo.mesh
+        self.connec, self.lpoin_w = m.connec, m.lpoin_w
+        self.npoin, self.nelem = m.npoin, m.nelem
+        nn = self.T.nnode
+        self.t = t
+        self.L, self.G = rk_ls_coeffs(stages, variant)
+        self.S = stages
+        self.parity = parity
+        self.dt_mode = dt_mode
+        self.cfl = cfl
+        self.sensor = sensor
+        c = lambda a: np.array(a, order="F", copy=True)  # noqa: E731
+        self.rho = [c(ic["rho"]), c(ic["rho"])]
+        self.q = [c(ic["q"]), c(ic["q"])]
+        self.E = [c(ic["E"]), c(ic["E"])]
+        self.u = [c(ic["u"]), c(ic["u"])]
+        self.pr = [c(ic["pr"]), c(ic["pr"])]
+        self.Tem = [c(ic["Tem"]), c(ic["Tem"])]
+        self.e_int = [c(ic["e_int"]), c(ic["e_int"])]
+        self.eta = fzeros((self.npoin, 2))
+        self.eta[:, 0] = ic["eta"]
+        self.eta[:, 1] = ic["eta"]
+        self.csound = c(ic["csound"])
+        self.machno = c(ic["machno"])
+        self.mu_e = fzeros((self.nelem, nn))
+        self.mue_l = fzeros((self.nelem, nn))
+        self.mu_sgs = fzeros((self.nelem, nn))
+        self.mu_factor = np.full(self.npoin, t.mu_factor)
+        self.mu_fluid = np.zeros(self.npoin)
+        self.Rmass, self.Rmom, self.Rener = np.zeros(self.npoin), fzeros((self.npoin, 3)), \
+            np.zeros(self.npoin)
+        self.Reta = fzeros((self.npoin, 2))
+        self.auxReta = np.zeros(self.npoin)
+        self.f_eta = fzeros((self.npoin, 3))
+        self.aux_h = np.zeros(self.npoin)
+        self.prm = visc_params(t.Rgas)
+        self.first = True
+        self.time = 0.0
+        sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
+        self.dt = self.adapt_dt()
+
+    def adapt_dt(self):
+        return adapt_dt_cfl(self.p, self.connec, self.geo.helem, self.u[1], self.csound,
+                            self.cfl[0], self.cfl[1], self.mu_fluid, self.mu_sgs, self.mu_e,
+                            self.rho[1])
+
+    def update_f(self):
+        """updateF (time_integ_ls.f90:489-691) with flag_lps_stab = .false."""
+        lw, t, g = self.lpoin_w, self.t, self.geo
+        eos_stage(lw, t.gamma_gas, t.Rgas, self.rho[1], self.q[1], self.E[1], self.u[1],
+                  self.e_int[1], self.pr[1], self.aux_h, self.Tem[1])
+        sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
+        full_convec_ijk(self.T, self.connec, g.He, g.gpvol, self.u[1], self.q[1], self.rho[1],
+                        self.pr[1], self.aux_h, self.Rmass, self.Rmom, self.Rener, True, -1.0)
+        full_diffusion_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, self.rho[1],
+                           self.rho[1], self.u[1], self.Tem[1], self.mu_fluid, self.mu_e,
+                           self.mu_sgs, self.Rmass, self.Rmom, self.Rener, False, -1.0)
+        lumped_solver_scal(lw, g.invMl, self.Rmass)
+        lumped_solver_scal(lw, g.invMl, self.Rener)
+        lumped_solver_vect(lw, g.invMl, self.Rmom)
+
+    def _scalar_convec(self, col):
+        out = np.zeros(self.npoin)
+        generic_scalar_convec_ijk(self.T, self.connec, self.geo.He, self.geo.gpvol, self.f_eta,
+                                  np.ascontiguousarray(self.eta[:, 0]), self.u[0], out)
+        lumped_solver_scal(self.lpoin_w, self.geo.invMl, out)
+        self.Reta[:, col] = out
+
+    def _sensor(self, Reta, rho, u, Tem, eta):
+        smart_visc_spectral(self.T, self.connec, self.lpoin_w, np.ascontiguousarray(Reta),
+                            self.t.gamma_gas, rho, u, Tem, np.ascontiguousarray(eta),
+                            self.geo.helem_l, self.geo.helem, self.prm, self.mu_e, self.mue_l)
+
+    def step(self):
+        for lv in (self.rho, self.q, self.E, self.u, self.pr, self.Tem, self.e_int):
+            lv[0][...] = lv[1]
+        self.eta[:, 0] = self.eta[:, 1]
+        lw = self.lpoin_w - 1
+        L, G = self.L, self.G
+        if self.first:
+            self.first = False
+            self.f_eta[lw, :] = self.u[0][lw, :] * self.eta[lw, 0][:, None]
+            self._scalar_convec(0)
+            if self.sensor:
+                self._sensor(self.Reta[:, 0], self.rho[0], self.u[0], self.Tem[0], self.eta[:, 0])
+            self.update_f()
+        if not self.parity:
+            self.update_f()
+        rk_ls_update(self.lpoin_w, 0.0, 1.0, G[1, 0] * self.dt, self.rho[0], self.q[0],
+                     self.E[0], self.rho[1], self.q[1], self.E[1], self.Rmass, self.Rmom,
+                     self.Rener)
+        for s in range(2, self.S + 1):
+            self.update_f()
+            rk_ls_update(self.lpoin_w, L[s, 0], L[s, s - 1], G[s, s - 1] * self.dt, self.rho[0],
+                         self.q[0], self.E[0], self.rho[1], self.q[1], self.E[1], self.Rmass,
+                         self.Rmom, self.Rener)
+        t = self.t
+        eos_step(self.lpoin_w, t.gamma_gas, t.Rgas, self.rho[1], self.q[1], self.E[1], self.u[1],
+                 self.e_int[1], self.pr[1], self.csound, self.machno, self.Tem[1])
+        thermo_entropy(self.lpoin_w, t.gamma_gas, t.po, t.rho0, self.rho[1], self.pr[1],
+                       self.u[0], self.eta, self.f_eta)
+        self._scalar_convec(1)
+        self.auxReta[lw] = (1.5 * self.Reta[lw, 1] - 0.5 * self.Reta[lw, 0]) + \
+            1.0 * (self.eta[lw, 1] - self.eta[lw, 0]) / self.dt
+        self.Reta[lw, 0] = self.Reta[lw, 1]
+        if self.sensor:
+            self._sensor(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[:, 1])
+        sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
+        self.time += self.dt
+        if self.dt_mode == "adaptive":
+            self.dt = self.adapt_dt()
+
+    def diagnostics(self):
+        ek = volavg_ek(self.T, self.connec, self.geo.gpvol, self.t.rho0, self.rho[1], self.u[1])
+        eS, eD, eT = visc_dissipation(self.T, self.connec, self.t.rho0, self.mu_fluid, self.mu_e,
+                                      self.u[1], self.geo.vol_tot, self.geo.gpvol, self.geo.He)
+        mm = max_mach(self.lpoin_w, self.machno)
+        return dict(time=self.time, EK=ek, eps_S=eS, eps_D=eD, eps_T=eT, maxmach=mm)

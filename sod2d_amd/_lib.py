@@ -68,6 +68,18 @@ _SIGS = {
     "sodhip_lumped_solver_scal_f64": (I, [P, I, I, P, P, P]),
     "sodhip_lumped_solver_vect_f64": (I, [P, I, I, P, P, P]),
     "sodhip_rk_stage_update_f64": (I, [P, I, I, P, P, D, D, D, D] + [P] * 9),
+    "sodhip_eos_stage_f64": (I, [P, I, I, P, D, D] + [P] * 8),
+    "sodhip_eos_step_f64": (I, [P, I, I, P, D, D] + [P] * 9),
+    "sodhip_sutherland_f64": (I, [P, I, P, P, D, P]),
+    "sodhip_thermo_entropy_f64": (I, [P, I, I, P, D, D, D] + [P] * 5),
+    "sodhip_entropy_residual_f64": (I, [P, I, I, P, D, D, P, P, P]),
+    "sodhip_rk_stage_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
+                            + [C.POINTER(StageParams), D, D, D, D] + [P] * 9),
+    "sodhip_adapt_dt_cfl_f64": (I, [P, I, I, I, P, P, P, P, D, D, P, P, P, P, C.POINTER(D)]),
+    "sodhip_volavg_ek_f64": (I, [P, I, I, I, P, P, D, P, P, C.POINTER(D)]),
+    "sodhip_visc_dissipation_f64": (I, [P, I, I, I, P, D, P, P, P, D] + [P] * 7
+                                    + [C.POINTER(D)] * 3),
+    "sodhip_max_mach_f64": (I, [P, I, I, P, P, C.POINTER(D)]),
 }
 
 _lib = None

@@ -36,19 +36,29 @@ __global__ __launch_bounds__(256) void k_rk_update(int npoin, int npoin_w,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
-    const double m = invMl[n];
-    const double rm = Rmass[n] * m;
-    Rmass[n] = rm;
-    rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * rm;
-    const double re = Rener[n] * m;
-    Rener[n] = re;
-    E[n] = lam1 * E1[n] + lams * E[n] + gdt * re;
+    if (invMl) {
+        const double m = invMl[n];
+        const double rm = Rmass[n] * m;
+        Rmass[n] = rm;
+        rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * rm;
+        const double re = Rener[n] * m;
+        Rener[n] = re;
+        E[n] = lam1 * E1[n] + lams * E[n] + gdt * re;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const size_t k = n + (size_t)npoin * d;
-        const double rq = Rmom[k] * m;
-        Rmom[k] = rq;
-        q[k] = lam1 * q1[k] + lams * q[k] + gdt * rq;
+        for (int d = 0; d < 3; ++d) {
+            const size_t k = n + (size_t)npoin * d;
+            const double rq = Rmom[k] * m;
+            Rmom[k] = rq;
+            q[k] = lam1 * q1[k] + lams * q[k] + gdt * rq;
+        }
+    } else { // R already carries the mass inverse (the residual kept from updateF)
+        rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * Rmass[n];
+        E[n] = lam1 * E1[n] + lams * E[n] + gdt * Rener[n];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t k = n + (size_t)npoin * d;
+            q[k] = lam1 * q1[k] + lams * q[k] + gdt * Rmom[k];
+        }
     }
 }
 
@@ -90,17 +100,207 @@ extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w
                                           double *rho, double *q, double *E, double *Rmass,
                                           double *Rmom, double *Rener)
 {
-    SODHIP_REQUIRE(ctx && lpoin_w && invMl && rho1 && q1 && E1 && rho && q && E && Rmass && Rmom &&
-                       Rener,
+    SODHIP_REQUIRE(ctx && lpoin_w && rho1 && q1 && E1 && rho && q && E && Rmass && Rmom && Rener,
                    "null argument");
     SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
     if (npoin_w == 0) return SODHIP_OK;
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
-    int tok = prof_begin(ctx, 1);
+    int tok = prof_begin(ctx, 3);
     k_rk_update<<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(
         npoin, npoin_w, lpoin_w, invMl, lam1, lams, gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom,
         Rener);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+// ---------------------------------------------------------------- EOS and friends
+namespace sodhip {
+
+// updateF "Update u and EOS" (time_integ_ls.f90:574-587)
+__global__ __launch_bounds__(256) void k_eos_stage(int npoin, int npoin_w,
+                                                   const int *__restrict__ lpoin_w, double g,
+                                                   double Rgas, const double *__restrict__ rho,
+                                                   const double *__restrict__ q,
+                                                   const double *__restrict__ E,
+                                                   double *__restrict__ u, double *__restrict__ e_int,
+                                                   double *__restrict__ pr,
+                                                   double *__restrict__ aux_h,
+                                                   double *__restrict__ Tem)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const double r = rho[n];
+    double umag = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ud = q[n + (size_t)npoin * d] / r;
+        u[n + (size_t)npoin * d] = ud;
+        umag += ud * ud;
+    }
+    const double ei = E[n] / r - 0.5 * umag;
+    const double p = r * (g - 1.0) * ei;
+    e_int[n] = ei;
+    pr[n] = p;
+    if (aux_h) aux_h[n] = (g / (g - 1.0)) * p / r;
+    Tem[n] = p / (r * Rgas);
+}
+
+// rk_4_ls_main end-of-step EOS (time_integ_ls.f90:391-407)
+__global__ __launch_bounds__(256) void k_eos_step(int npoin, int npoin_w,
+                                                  const int *__restrict__ lpoin_w, double g,
+                                                  double Rgas, const double *__restrict__ rho,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ E,
+                                                  double *__restrict__ u, double *__restrict__ e_int,
+                                                  double *__restrict__ pr,
+                                                  double *__restrict__ csound,
+                                                  double *__restrict__ machno,
+                                                  double *__restrict__ Tem)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const double r = rho[n];
+    double umag = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ud = q[n + (size_t)npoin * d] / r;
+        u[n + (size_t)npoin * d] = ud;
+        umag += ud * ud;
+    }
+    const double ei = E[n] / r - 0.5 * umag;
+    const double p = r * (g - 1.0) * ei;
+    e_int[n] = ei;
+    pr[n] = p;
+    const double c = sqrt(g * p / r);
+    csound[n] = c;
+    machno[n] = sqrt(umag) / c;
+    Tem[n] = p / (r * Rgas);
+}
+
+// sutherland_viscosity (mod_fluid_viscosity.f90:21-41), over all npoin
+__global__ __launch_bounds__(256) void k_sutherland(int npoin, const double *__restrict__ Tem,
+                                                    const double *__restrict__ mu_factor,
+                                                    double mu_factor_scalar,
+                                                    double *__restrict__ mu_fluid)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npoin) return;
+    const double T = Tem[n];
+    const double f = mu_factor ? mu_factor[n] : mu_factor_scalar;
+    mu_fluid[n] = f * 0.000001458 * pow(T, 1.5) / (T + 110.4);
+}
+
+// thermo_entropy (mod_entropy_viscosity.f90:411-433); eta is (npoin,2)
+__global__ __launch_bounds__(256) void k_thermo_entropy(int npoin, int npoin_w,
+                                                        const int *__restrict__ lpoin_w, double g,
+                                                        double p_inf, double rho_inf,
+                                                        const double *__restrict__ rho,
+                                                        const double *__restrict__ pr,
+                                                        const double *__restrict__ u,
+                                                        double *__restrict__ eta,
+                                                        double *__restrict__ feta)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const double e1 = eta[n + (size_t)npoin];
+    eta[n] = e1;
+    const double r = rho[n];
+    eta[n + (size_t)npoin] =
+        (r / (g - 1.0)) * log(fmax(pr[n] / p_inf, 2.220446049250313e-16) / pow(r / rho_inf, g));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) feta[n + (size_t)npoin * d] = u[n + (size_t)npoin * d] * e1;
+}
+
+// entropy residual of rk_4_ls_main (time_integ_ls.f90:437-443); Reta and eta are (npoin,2)
+__global__ __launch_bounds__(256) void k_entropy_residual(int npoin, int npoin_w,
+                                                          const int *__restrict__ lpoin_w,
+                                                          double factor_comp, double dt,
+                                                          double *__restrict__ Reta,
+                                                          const double *__restrict__ eta,
+                                                          double *__restrict__ auxReta)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const double r1 = Reta[n], r2 = Reta[n + (size_t)npoin];
+    auxReta[n] = (1.5 * r2 - 0.5 * r1) + factor_comp * (eta[n + (size_t)npoin] - eta[n]) / dt;
+    Reta[n] = r2;
+}
+
+} // namespace sodhip
+
+#define SODHIP_NODE_LAUNCH(kind, count, kernel, ...)                                        \
+    do {                                                                                   \
+        if ((count) > 0) {                                                                 \
+            int _tok = prof_begin(ctx, kind);                                              \
+            kernel<<<((count) + 255) / 256, 256, 0, ctx->stream>>>(__VA_ARGS__);           \
+            prof_end(ctx, _tok);                                                           \
+            SODHIP_TRY_HIP(hipGetLastError());                                             \
+        }                                                                                  \
+    } while (0)
+
+extern "C" int sodhip_eos_stage_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                    double gamma_gas, double Rgas, const double *rho,
+                                    const double *q, const double *E, double *u, double *e_int,
+                                    double *pr, double *aux_h, double *Tem)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && q && E && u && e_int && pr && Tem, "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_eos_stage, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, rho, q, E,
+                       u, e_int, pr, aux_h, Tem);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_eos_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                   double gamma_gas, double Rgas, const double *rho,
+                                   const double *q, const double *E, double *u, double *e_int,
+                                   double *pr, double *csound, double *machno, double *Tem)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && q && E && u && e_int && pr && csound && machno && Tem,
+                   "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_eos_step, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, rho, q, E,
+                       u, e_int, pr, csound, machno, Tem);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_sutherland_f64(sodhip_ctx ctx, int npoin, const double *Tem,
+                                     const double *mu_factor, double mu_factor_scalar,
+                                     double *mu_fluid)
+{
+    SODHIP_REQUIRE(ctx && Tem && mu_fluid, "null argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin, k_sutherland, npoin, Tem, mu_factor, mu_factor_scalar, mu_fluid);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_thermo_entropy_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                         const int *lpoin_w, double gamma_gas, double p_inf,
+                                         double rho_inf, const double *rho, const double *pr,
+                                         const double *u, double *eta, double *feta)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && pr && u && eta && feta, "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_thermo_entropy, npoin, npoin_w, lpoin_w, gamma_gas, p_inf,
+                       rho_inf, rho, pr, u, eta, feta);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_entropy_residual_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                           const int *lpoin_w, double factor_comp, double dt,
+                                           double *Reta, const double *eta, double *auxReta)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && Reta && eta && auxReta, "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_entropy_residual, npoin, npoin_w, lpoin_w, factor_comp, dt,
+                       Reta, eta, auxReta);
     return SODHIP_OK;
 }

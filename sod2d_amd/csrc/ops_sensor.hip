@@ -1,2 +1,258 @@
-// ops_sensor.hip -- entropy-viscosity sensor (placeholder translation unit).
+// ops_sensor.hip -- entropy-viscosity sensor smart_visc_spectral
+// (mod_entropy_viscosity.f90:14-111) with the intra-element 3x3x3 filter of
+// init_filters (mod_filters.f90:15-55).
+//
+// The entropy normalisation (mean of eta over working nodes in FP64, then the
+// maximum deviation) is a deterministic two-level reduction: every block of
+// the first kernel sums a fixed slice in a fixed tree order; the second kernel
+// re-reduces the partial sums in the same fixed order in every block (all
+// blocks obtain the identical mean) and forms partial maxima; the element
+// kernel folds the partial maxima (max is order independent).  No host
+// synchronisation is needed between the three launches.
+#include <algorithm>
+
 #include "common.hpp"
+
+namespace sodhip {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+template <typename T> __device__ __forceinline__ T block_sum(T v, T *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T> __device__ __forceinline__ T block_max(T v, T *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// partial sums of eta over working nodes: block b owns a contiguous slice
+__global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int *__restrict__ lpoin_w,
+                                                         const double *__restrict__ eta,
+                                                         double *__restrict__ part)
+{
+    __shared__ double sh[kRedThreads];
+    const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    double s = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += eta[lpoin_w[i] - 1];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// every block: mean = (sum of partial sums)/npoin_w in a fixed order, then its
+// slice's max |eta - mean|
+__global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
+                                                            const int *__restrict__ lpoin_w,
+                                                            const double *__restrict__ eta,
+                                                            const double *__restrict__ part,
+                                                            int nparts, double *__restrict__ pmax)
+{
+    __shared__ double sh[kRedThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    const double mean = block_sum(s, sh) / (double)npoin_w;
+    const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    double m = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+        m = fmax(m, fabs(eta[lpoin_w[i] - 1] - mean));
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) pmax[blockIdx.x] = m;
+}
+
+template <int P> struct SG {
+    static constexpr int N1 = P + 1;
+    static constexpr int NN = N1 * N1 * N1;
+    static constexpr int NT = (NN + 63) / 64 * 64;
+};
+
+struct SensorArgs {
+    int nelem, npoin;
+    const int *connecT;
+    const double *Reta, *rho, *u, *Tem, *helem, *helem_k;
+    const int *inv;
+    double gamma, Rgas, ce, cmax, cglob, fact_low_mach;
+    const double *pmax; // partial maxima (nparts) or nullptr when not normalising
+    int nparts;
+    double *mu_e, *mue_l;
+};
+
+// sod2d 1-D index s (1-based) -> geometric position along the line (1-based)
+__device__ __forceinline__ int geo_of(int s, int N1) { return s == 1 ? 1 : (s == 2 ? N1 : s - 1); }
+
+// convertIJK(0:p+2): geometric position (with one ghost each side) -> sod2d
+// index, reflecting 0 -> 3 and p+2 -> p (mod_filters.f90:44-50)
+__device__ __forceinline__ int convert_ijk(int g, int P)
+{
+    if (g == 0) return 3;
+    if (g == 1) return 1;
+    if (g == P + 2) return P;
+    if (g == P + 1) return 2;
+    return g + 1; // 2..P -> 3..P+1
+}
+
+template <int P>
+__global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
+{
+    constexpr int N1 = SG<P>::N1, NN = SG<P>::NN, NT = SG<P>::NT;
+    __shared__ double sm[NT];
+    __shared__ double red[NT];
+    __shared__ double s_norm;
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    if (lane == 0) {
+        double nrm = 1.0;
+        if (A.pmax) {
+            nrm = 0.0;
+            for (int i = 0; i < A.nparts; ++i) nrm = fmax(nrm, A.pmax[i]);
+        }
+        s_norm = nrm;
+    }
+    // the lane's sod2d (I,J,K): position of a in invAtoIJK (a permutation)
+    int I = 1, J = 1, K = 1;
+    for (int k = 0; k < N1; ++k)
+        for (int j = 0; j < N1; ++j)
+            for (int i = 0; i < N1; ++i)
+                if (A.inv[i + N1 * (j + N1 * k)] - 1 == a) {
+                    I = i + 1;
+                    J = j + 1;
+                    K = k + 1;
+                }
+    // the 27 stencil source nodes of the lane's output point
+    const int ii = geo_of(I, N1), jj = geo_of(J, N1), kk = geo_of(K, N1);
+    int src[27];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const int ci = convert_ijk(ii + l - 1, P) - 1, cj = convert_ijk(jj + m - 1, P) - 1,
+                          ck = convert_ijk(kk + n - 1, P) - 1;
+                src[9 * l + 3 * m + n] = min(max(A.inv[ci + N1 * (cj + N1 * ck)] - 1, 0), NN - 1);
+            }
+    __syncthreads();
+    const double norm = s_norm;
+    const double w[3] = {0.25, 0.5, 0.25};
+    for (int e = blockIdx.x; e < A.nelem; e += gridDim.x) {
+        const int n = act ? A.connecT[(size_t)e * NN + a] : 0;
+        const double u0 = A.u[n], u1 = A.u[n + A.npoin], u2 = A.u[n + 2 * (size_t)A.npoin];
+        const double aux2 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * A.Tem[n]);
+        const double rho = A.rho[n];
+        double be = act ? (rho * A.helem_k[e]) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
+        red[lane] = be;
+        __syncthreads();
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (lane < s) red[lane] = fmax(red[lane], red[lane + s]);
+            __syncthreads();
+        }
+        const double betae = red[0];
+        const double R1 = rho * fabs(A.Reta[n]) / norm;
+        const double h = act ? A.helem[e + (size_t)A.nelem * a] : 0.0;
+        const double Ve = A.ce * R1 * (h * h);
+        const double ml = A.cglob * fmin(Ve, betae);
+        if (act) {
+            sm[a] = ml;
+            A.mue_l[e + (size_t)A.nelem * a] = ml;
+        }
+        __syncthreads();
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) acc += w[l] * w[m] * w[nn] * sm[src[9 * l + 3 * m + nn]];
+        if (act) A.mu_e[e + (size_t)A.nelem * a] = acc;
+        __syncthreads();
+    }
+}
+
+} // namespace sodhip
+
+using namespace sodhip;
+
+extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                              int npoin_w, const int *connec, const int *lpoin_w,
+                                              const double *Reta, double gamma_gas,
+                                              const double *rho, const double *u,
+                                              const double *Tem, const double *eta,
+                                              const double *helem, const double *helem_k,
+                                              const int *invAtoIJK, const sodhip_visc_params *prm,
+                                              double *mu_e, double *mue_l)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && Reta && rho && u && Tem && eta && helem && helem_k &&
+                       invAtoIJK && mu_e && mue_l,
+                   "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    if ((rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double)))) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    double *pmax = part + kRedBlocks;
+    const bool normalise = prm->flag_normalise_entropy == 1;
+    int tok = prof_begin(ctx, 2);
+    if (normalise) {
+        k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+        k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+                                                                  kRedBlocks, pmax);
+    }
+    SensorArgs A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.connecT = m->connecT;
+    A.Reta = Reta;
+    A.rho = rho;
+    A.u = u;
+    A.Tem = Tem;
+    A.helem = helem;
+    A.helem_k = helem_k;
+    A.inv = invAtoIJK;
+    A.gamma = gamma_gas;
+    A.Rgas = prm->Rgas;
+    A.ce = prm->ce;
+    A.cmax = prm->cmax;
+    A.cglob = prm->cglob;
+    A.fact_low_mach = prm->flag_drop_c_in_envit ? 0.0 : 1.0;
+    A.pmax = normalise ? pmax : nullptr;
+    A.nparts = kRedBlocks;
+    A.mu_e = mu_e;
+    A.mue_l = mue_l;
+    const int grid = std::min(nelem, ctx->num_cu * 16);
+    switch (porder) {
+    case 2: k_smart_visc<2><<<grid, SG<2>::NT, 0, ctx->stream>>>(A); break;
+    case 3: k_smart_visc<3><<<grid, SG<3>::NT, 0, ctx->stream>>>(A); break;
+    case 4: k_smart_visc<4><<<grid, SG<4>::NT, 0, ctx->stream>>>(A); break;
+    }
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}

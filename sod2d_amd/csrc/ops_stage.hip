@@ -1,2 +1,436 @@
-// ops_stage.hip -- fused RK stage (placeholder translation unit).
+// ops_stage.hip -- the fused Runge-Kutta stage (updateF + lumped solve + stage
+// combine), the CFL time step and the Taylor-Green diagnostics.
+//
+// Fused stage = two launches:
+//   1. k_rhs<CONV,DIFF,CONS> (ops_element.hip): per element, the primitives of
+//      updateF's EOS block and Sutherland's law are formed from the conserved
+//      FP64 state in registers, then the split-form convection and the viscous
+//      operator are evaluated together (they share grad(rho) and grad(u)) and
+//      written as per element-node partials;
+//   2. k_gather_update: per working node, the partials are gathered in the
+//      reference's serial (element, node) order, scaled by fact and by the
+//      lumped inverse, and the Shu-Osher combine y = lam1*y1 + lams*y + gam*dt*R
+//      is applied -- the RHS never round-trips through HBM as an assembled array
+//      unless the caller asks for it (the reference keeps it for the next step's
+//      first stage).
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
+
+namespace sodhip {
+
+template <typename T>
+int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He,
+                       const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
+                       const int *gK, const double *rho, const double *q, const double *E,
+                       const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
+                       double mu_factor, double c_rho, double c_ener, T *out);
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_gather_update(
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ row,
+    const int *__restrict__ col, const TS *__restrict__ scr, size_t EN,
+    const double *__restrict__ invMl, double fact, double lam1, double lams, double gdt,
+    const double *__restrict__ rho1, const double *__restrict__ q1, const double *__restrict__ E1,
+    double *__restrict__ rho, double *__restrict__ q, double *__restrict__ E,
+    double *__restrict__ Rmass, double *__restrict__ Rmom, double *__restrict__ Rener)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const int j0 = row[n], j1 = row[n + 1];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = j0; j < j1; ++j) {
+        const int c = col[j];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) acc[f] += fact * (double)scr[(size_t)f * EN + c];
+    }
+    const double m = invMl[n];
+    const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
+    const double rm = acc[0] * m, r0 = acc[1] * m, r1 = acc[2] * m, r2 = acc[3] * m,
+                 re = acc[4] * m;
+    rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * rm;
+    q[n] = lam1 * q1[n] + lams * q[n] + gdt * r0;
+    q[n1] = lam1 * q1[n1] + lams * q[n1] + gdt * r1;
+    q[n2] = lam1 * q1[n2] + lams * q[n2] + gdt * r2;
+    E[n] = lam1 * E1[n] + lams * E[n] + gdt * re;
+    if (Rmass) {
+        Rmass[n] = rm;
+        Rmom[n] = r0;
+        Rmom[n1] = r1;
+        Rmom[n2] = r2;
+        Rener[n] = re;
+    }
+}
+
+template <typename T>
+static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                    const int *connec, const int *lpoin_w, const T *He, const T *gpvol,
+                    const T *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                    const T *mu_e, const double *invMl, const sodhip_stage_params *prm, double dt,
+                    double lam1, double lams, double gam, const double *rho1, const double *q1,
+                    const double *E1, double *rho, double *q, double *E, double *Rmass,
+                    double *Rmom, double *Rener)
+{
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(T)))) return rc;
+    T *scr = (T *)ctx->scratch.ptr;
+    rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
+                               prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
+                               prm->c_rho, prm->c_ener, scr);
+    if (rc) return rc;
+    SODHIP_TRY_HIP(hipGetLastError());
+    int tok = prof_begin(ctx, 1);
+    k_gather_update<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(
+        npoin, npoin_w, lpoin_w, m->row, m->col, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
+        rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+// ---------------------------------------------------------------- CFL time step
+// adapt_dt_cfl (mod_time_ops.f90:11-63): per element, then a min over blocks.
+__global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder,
+                                                const int *__restrict__ connecT,
+                                                const double *__restrict__ helem,
+                                                const double *__restrict__ u,
+                                                const double *__restrict__ csound, int npoin,
+                                                double cfl_conv, double cfl_diff,
+                                                const double *__restrict__ mu_fluid,
+                                                const double *__restrict__ mu_sgs,
+                                                const double *__restrict__ mu_e,
+                                                const double *__restrict__ rho,
+                                                double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = 1.0e14;
+    if (e < nelem) {
+        double L3 = 0.0, maxmu = 0.0;
+        for (int a = 0; a < nnode; ++a) {
+            const int n = connecT[(size_t)e * nnode + a];
+            double um = fabs(u[n]);
+            um = fmax(um, fabs(u[n + npoin]));
+            um = fmax(um, fabs(u[n + 2 * (size_t)npoin]));
+            L3 = fmax(L3, um + csound[n]);
+            if (mu_fluid) {
+                const double sg = mu_sgs ? mu_sgs[e + (size_t)nelem * a] : 0.0;
+                maxmu = fmax(maxmu, (sg + mu_fluid[n] + mu_e[e + (size_t)nelem * a]) / rho[n]);
+            }
+        }
+        const double hh = helem[e] / (double)(2 * porder + 1);
+        dtl = cfl_conv * hh / L3;
+        if (mu_fluid) dtl = fmin(dtl, cfl_diff * (hh * hh) / maxmu);
+    }
+    sh[threadIdx.x] = dtl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// ---------------------------------------------------------------- diagnostics
+// volAvg_EK (mod_analysis.f90:10-44): per element sum_g gpvol 1/2 rho |u|^2 in
+// FP64 (the shape functions are the identity at the GLL points), then fixed-order
+// block sums; the host adds the block partials in order.
+__global__ __launch_bounds__(256) void k_ek(int nelem, int nnode, const int *__restrict__ connecT,
+                                            const double *__restrict__ gpvol,
+                                            const double *__restrict__ rho,
+                                            const double *__restrict__ u, int npoin,
+                                            double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double r1 = 0.0;
+    if (e < nelem) {
+        for (int a = 0; a < nnode; ++a) {
+            const int n = connecT[(size_t)e * nnode + a];
+            const double u0 = u[n], u1 = u[n + npoin], u2 = u[n + 2 * (size_t)npoin];
+            r1 += gpvol[(size_t)e * nnode + a] * 0.5 * rho[n] * (u0 * u0 + u1 * u1 + u2 * u2);
+        }
+    }
+    sh[threadIdx.x] = r1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// visc_dissipationRate (mod_analysis.f90:46-144): per quadrature point the
+// velocity gradient (line derivatives, as the operators), enstrophy
+// (mu_f + mu_e)|curl u|^2 and dilatation mu_f (div u)^2, weighted by gpvol;
+// element sums by a fixed tree, written per element.
+template <int P>
+__global__ __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) void k_dissipation(
+    int nelem, int npoin, const int *__restrict__ connecT, const double *__restrict__ He,
+    const double *__restrict__ gpvol, const double *__restrict__ dl, const int *__restrict__ inv,
+    const int *__restrict__ gI, const int *__restrict__ gJ, const int *__restrict__ gK,
+    const double *__restrict__ u, const double *__restrict__ mu_fluid,
+    const double *__restrict__ mu_e, double *__restrict__ elS, double *__restrict__ elD)
+{
+    constexpr int N1 = P + 1, NN = N1 * N1 * N1, NT = (NN + 63) / 64 * 64;
+    __shared__ double su[3 * NT];
+    __shared__ double rs[NT], rd[NT];
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    const int own[3] = {min(max(gI[a] - 1, 0), P), min(max(gJ[a] - 1, 0), P),
+                        min(max(gK[a] - 1, 0), P)};
+    for (int e = blockIdx.x; e < nelem; e += gridDim.x) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? connecT[ea] : 0;
+        if (act)
+            for (int d = 0; d < 3; ++d) su[d * NT + a] = u[n + (size_t)npoin * d];
+        __syncthreads();
+        double du[3][3]; // du[i][k] = d u_i / d xi_k
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < 3; ++i) du[i][k] = 0.0;
+            for (int m = 0; m < N1; ++m) {
+                int idx[3] = {own[0], own[1], own[2]};
+                idx[k] = m;
+                const int b = min(max(inv[idx[0] + N1 * (idx[1] + N1 * idx[2])] - 1, 0), NN - 1);
+                const double c = dl[a + NN * (k + 3 * m)];
+                for (int i = 0; i < 3; ++i) du[i][k] += c * su[i * NT + b];
+            }
+        }
+        double g[3][3]; // g[j][i] = d u_i / d x_j
+        const double *hp = He + 9 * ea;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i)
+                g[j][i] = act ? hp[j + 0] * du[i][0] + hp[j + 3] * du[i][1] + hp[j + 6] * du[i][2] : 0.0;
+        const double div = g[0][0] + g[1][1] + g[2][2];
+        const double w0 = g[1][2] - g[2][1], w1 = g[2][0] - g[0][2], w2 = g[0][1] - g[1][0];
+        const double curl2 = w0 * w0 + w1 * w1 + w2 * w2;
+        const double vol = act ? gpvol[ea] : 0.0;
+        const double mf = act ? mu_fluid[n] : 0.0;
+        const double me = act ? mu_e[e + (size_t)nelem * a] : 0.0;
+        rs[lane] = vol * (mf + me) * curl2;
+        rd[lane] = vol * mf * div * div;
+        __syncthreads();
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (lane < s) {
+                rs[lane] += rs[lane + s];
+                rd[lane] += rd[lane + s];
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            elS[e] = rs[0];
+            elD[e] = rd[0];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_blocks(int n, const double *__restrict__ v,
+                                                    double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    sh[threadIdx.x] = i < n ? v[i] : 0.0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_max_mach(int npoin_w, const int *__restrict__ lpoin_w,
+                                                  const double *__restrict__ machno,
+                                                  double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    sh[threadIdx.x] = i < npoin_w ? machno[lpoin_w[i] - 1] : 0.0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+static int fetch_parts(sodhip_ctx ctx, const double *dev, int n, std::vector<double> &h)
+{
+    h.resize(n);
+    SODHIP_TRY_HIP(hipMemcpyAsync(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    return SODHIP_OK;
+}
+
+} // namespace sodhip
+
+using namespace sodhip;
+
+extern "C" int sodhip_rk_stage_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                   const int *connec, const int *lpoin_w, int op_precision,
+                                   const void *He, const void *gpvol, const void *dlxigp_ip,
+                                   const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                                   const int *gmshAtoK, const void *mu_e, const double *invMl,
+                                   const sodhip_stage_params *prm, double dt, double lam1,
+                                   double lams, double gam, const double *rho1, const double *q1,
+                                   const double *E1, double *rho, double *q, double *E,
+                                   double *Rmass, double *Rmom, double *Rener)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && He && gpvol && dlxigp_ip && invAtoIJK && gmshAtoI &&
+                       gmshAtoJ && gmshAtoK && mu_e && invMl && rho1 && q1 && E1 && rho && q && E,
+                   "null array argument");
+    SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    if (op_precision == 8)
+        return rk_stage<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                                (const double *)He, (const double *)gpvol,
+                                (const double *)dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
+                                (const double *)mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1,
+                                E1, rho, q, E, Rmass, Rmom, Rener);
+    if (op_precision == 4)
+        return rk_stage<float>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                               (const float *)He, (const float *)gpvol, (const float *)dlxigp_ip,
+                               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e, invMl,
+                               prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom,
+                               Rener);
+    return set_error(SODHIP_ERR_ARG, "op_precision must be 8 or 4 (got %d)", op_precision);
+}
+
+extern "C" int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                       const int *connec, const double *helem, const double *u,
+                                       const double *csound, double cfl_conv, double cfl_diff,
+                                       const double *mu_fluid, const double *mu_sgs,
+                                       const double *mu_e, const double *rho, double *dt)
+{
+    SODHIP_REQUIRE(ctx && connec && helem && u && csound && dt, "null argument");
+    SODHIP_REQUIRE(!mu_fluid || (mu_e && rho), "the diffusive limit needs mu_e and rho");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    const int nb = (nelem + 255) / 256;
+    if ((rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double)))) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    int tok = prof_begin(ctx, 3);
+    k_dt_cfl<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), porder, m->connecT, helem, u,
+                                          csound, npoin, cfl_conv, cfl_diff, mu_fluid, mu_sgs,
+                                          mu_e, rho, part);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    std::vector<double> h;
+    if ((rc = fetch_parts(ctx, part, nb, h))) return rc;
+    double v = 1.0e14;
+    for (double x : h) v = std::min(v, x);
+    *dt = v;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_volavg_ek_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                    const int *connec, const double *gpvol, double rho0,
+                                    const double *rho, const double *u, double *EK)
+{
+    SODHIP_REQUIRE(ctx && connec && gpvol && rho && u && EK, "null argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    const int nb = (nelem + 255) / 256;
+    if ((rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double)))) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    k_ek<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), m->connecT, gpvol, rho, u, npoin,
+                                      part);
+    SODHIP_TRY_HIP(hipGetLastError());
+    std::vector<double> h;
+    if ((rc = fetch_parts(ctx, part, nb, h))) return rc;
+    double s = 0.0;
+    for (double x : h) s += x;
+    // the reference normalises by (2*3.14159)^3 whatever the box (mod_analysis.f90:37)
+    *EK = s / (rho0 * std::pow(2.0 * 3.14159, 3.0));
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_visc_dissipation_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                           const int *connec, double rho0,
+                                           const double *mu_fluid, const double *mu_e,
+                                           const double *u, double volT, const double *gpvol,
+                                           const double *He, const double *dlxigp_ip,
+                                           const int *invAtoIJK, const int *gmshAtoI,
+                                           const int *gmshAtoJ, const int *gmshAtoK,
+                                           double *eps_S, double *eps_D, double *eps_T)
+{
+    SODHIP_REQUIRE(ctx && connec && mu_fluid && mu_e && u && gpvol && He && dlxigp_ip &&
+                       invAtoIJK && gmshAtoI && gmshAtoJ && gmshAtoK && eps_S && eps_D && eps_T,
+                   "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    const int nb = (nelem + 255) / 256;
+    if ((rc = ensure_buf(ctx->scratch2, (2 * (size_t)nelem + 2 * nb) * sizeof(double)))) return rc;
+    double *elS = (double *)ctx->scratch2.ptr, *elD = elS + nelem, *pS = elD + nelem, *pD = pS + nb;
+    const int grid = std::min(nelem, ctx->num_cu * 16);
+    switch (porder) {
+    case 2:
+        k_dissipation<2><<<grid, 64, 0, ctx->stream>>>(nelem, npoin, m->connecT, He, gpvol,
+                                                      dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                                                      gmshAtoK, u, mu_fluid, mu_e, elS, elD);
+        break;
+    case 3:
+        k_dissipation<3><<<grid, 64, 0, ctx->stream>>>(nelem, npoin, m->connecT, He, gpvol,
+                                                      dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                                                      gmshAtoK, u, mu_fluid, mu_e, elS, elD);
+        break;
+    case 4:
+        k_dissipation<4><<<grid, 128, 0, ctx->stream>>>(nelem, npoin, m->connecT, He, gpvol,
+                                                       dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                                                       gmshAtoK, u, mu_fluid, mu_e, elS, elD);
+        break;
+    }
+    k_sum_blocks<<<nb, 256, 0, ctx->stream>>>(nelem, elS, pS);
+    k_sum_blocks<<<nb, 256, 0, ctx->stream>>>(nelem, elD, pD);
+    SODHIP_TRY_HIP(hipGetLastError());
+    std::vector<double> h;
+    if ((rc = fetch_parts(ctx, pS, 2 * nb, h))) return rc;
+    double sS = 0.0, sD = 0.0;
+    for (int i = 0; i < nb; ++i) sS += h[i];
+    for (int i = 0; i < nb; ++i) sD += h[nb + i];
+    const double alpha = 1.0 / (rho0 * volT);
+    sS *= alpha;
+    sD = (double)(4.0f / 3.0f) * sD * alpha; // default-real literal (mod_analysis.f90:139)
+    *eps_S = sS;
+    *eps_D = sD;
+    *eps_T = sS + sD;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_max_mach_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                   const double *machno, double *maxmach)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && machno && maxmach, "null argument");
+    SODHIP_REQUIRE(npoin_w > 0 && npoin_w <= npoin, "npoin_w outside 1..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    const int nb = (npoin_w + 255) / 256;
+    int rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    k_max_mach<<<nb, 256, 0, ctx->stream>>>(npoin_w, lpoin_w, machno, part);
+    SODHIP_TRY_HIP(hipGetLastError());
+    std::vector<double> h;
+    if ((rc = fetch_parts(ctx, part, nb, h))) return rc;
+    double v = 0.0;
+    for (double x : h) v = std::max(v, x);
+    *maxmach = v;
+    return SODHIP_OK;
+}

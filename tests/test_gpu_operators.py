@@ -188,3 +188,36 @@ def test_lumped_and_stage_update(oracle):
     assert relerr(dv["q"].cpu().numpy(), oq.ravel(order="F")) < 1e-15
     assert relerr(dv["E"].cpu().numpy(), oE) < 1e-15
     assert relerr(dv["Rq"].cpu().numpy(), oRq.ravel(order="F")) < 1e-15
+
+
+@pytest.mark.parametrize("p,normalise", [(3, 1), (3, 0), (2, 1), (4, 1)])
+def test_smart_visc_spectral_f64(oracle, p, normalise):
+    from sod2d_amd import device as D
+    g = make_case(p, 3, periodic=(p != 4), jitter=0.0 if p != 4 else 0.05)
+    m, T = g.mesh, g.tables
+    r = np.random.default_rng(13 + p)
+    n, ne, nn = m.npoin, m.nelem, T.nnode
+    rho = r.uniform(0.9, 1.1, n)
+    u = np.asfortranarray(r.normal(0, 0.3, (n, 3)))
+    Tem = r.uniform(0.9, 1.1, n)
+    eta = r.normal(0, 1.0, n)
+    # Reta spanning both branches of min(Ve, beta_e)
+    Reta = r.normal(0, 1.0, n) * 10.0 ** r.uniform(-2, 4, n)
+    gamma, Rgas = 1.4, 287.0
+    prm_o = oracle.visc_params(Rgas, flag_normalise_entropy=normalise)
+    mu_e = fz((ne, nn))
+    mu_e, mue_l = oracle.smart_visc_spectral(oracle.Tables(p), m.connec, m.lpoin_w, Reta, gamma, rho,
+                                             u, Tem, eta, g.helem_l, g.helem, prm_o, mu_e)
+    dmu = torch.zeros(ne * nn, dtype=torch.float64, device="cuda")
+    dml = torch.zeros(ne * nn, dtype=torch.float64, device="cuda")
+    prm = D.visc_params(Rgas, flag_normalise_entropy=normalise)
+    D.smart_visc_spectral(ne, n, m.npoin_w, _dev(m.connec), _dev(m.lpoin_w), _dev(Reta), None,
+                          None, None, None, None, gamma, _dev(rho), _dev(u), None, _dev(Tem),
+                          _dev(eta), _dev(g.helem_l), _dev(g.helem), None, dmu,
+                          _dev(T.invAtoIJK), None, None, None, dml, prm)
+    torch.cuda.synchronize()
+    assert relerr(dml.cpu().numpy(), mue_l.ravel(order="F")) < 1e-13
+    assert relerr(dmu.cpu().numpy(), mu_e.ravel(order="F")) < 1e-13
+    # both limiter branches are exercised
+    lim = mue_l.ravel(order="F")
+    assert 0 < np.mean(lim == lim.max()) < 1

@@ -1,0 +1,60 @@
+"""Parity of the device time loop (rk_4_ls_main sequence with the fused stage)
+with the oracle's restatement, step by step, on small periodic TGV meshes.
+
+Bars (from the FP64 target): conserved fields rho, rho*u, E within 1e-12
+normwise relative per step; the sensor output mu_e within 1e-10 (it carries a
+min/max limiter, so rounding can flip a branch on isolated points only through
+ties); dt and the diagnostics EK and eps within 1e-10.  The FP32-operator mode
+(FP64 state, gather, mass inverse and combine) is checked against the FP64
+oracle with per-field bounds stated below."""
+import numpy as np
+import pytest
+
+from helpers import make_case, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(oracle, p, n, stages=5, parity=True, op_precision=8, dt_mode="adaptive", M=0.1):
+    from sod2d_amd.solver import TGVParams, TGVRun
+    g = make_case(p, n, periodic=True, lengths=(2 * np.pi,) * 3)
+    tp = TGVParams(M=M)
+    to = oracle.tgv_params(M=M)
+    ic = oracle.tgv_ic(g.mesh.coord, to)
+    dev = TGVRun(p, geometry=g, params=tp, initial=ic, stages=stages, parity=parity,
+                 op_precision=op_precision, dt_mode=dt_mode)
+    orc = oracle.TGVOracle(p, g, ic, to, stages=stages, parity=parity, dt_mode=dt_mode,
+                           cfl=(tp.cfl_conv, tp.cfl_diff))
+    return dev, orc
+
+
+@pytest.mark.parametrize("p,n,stages,parity", [(3, 4, 5, True), (3, 3, 4, False), (2, 4, 3, True)])
+def test_time_loop_fp64(oracle, p, n, stages, parity):
+    dev, orc = _setup(oracle, p, n, stages, parity)
+    assert abs(dev.dt - orc.dt) <= 1e-12 * orc.dt
+    for step in range(6):
+        dev.step()
+        orc.step()
+        s = dev.state()
+        w = orc.lpoin_w - 1
+        assert relerr(s["rho"][w], orc.rho[1][w]) < 1e-12, step
+        assert relerr(s["q"][w], orc.q[1][w]) < 1e-12, step
+        assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
+        assert relerr(s["mu_e"], orc.mu_e) < 1e-10, step
+        assert abs(dev.dt - orc.dt) <= 1e-10 * orc.dt, step
+    dd, do = dev.diagnostics(), orc.diagnostics()
+    for k in ("EK", "eps_S", "eps_D", "eps_T", "maxmach"):
+        assert abs(dd[k] - do[k]) <= 1e-10 * abs(do[k]) + 1e-300, (k, dd[k], do[k])
+
+
+def test_time_loop_fp32_operators(oracle):
+    """FP32 operator arithmetic with FP64 state: per-field error vs the FP64
+    oracle after 6 steps at Ma=0.1 (bounds: rho 2e-7, rho*u 1e-5, E 2e-7)."""
+    dev, orc = _setup(oracle, 3, 4, op_precision=4)
+    for _ in range(6):
+        dev.step()
+        orc.step()
+    s = dev.state()
+    w = orc.lpoin_w - 1
+    e = {k: relerr(s[k][w], getattr(orc, k)[1][w]) for k in ("rho", "q", "E")}
+    assert e["rho"] < 2e-7 and e["q"] < 1e-5 and e["E"] < 2e-7, e
