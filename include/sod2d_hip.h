@@ -248,6 +248,8 @@ typedef struct {
     double mu_factor;               /* Sutherland factor (flag_mu_factor), uniform */
     double c_rho, c_ener;           /* mod_numerical_params.f90 globals (default 1) */
     double fact;                    /* residual sign: updateF passes -1 */
+    double p_ref;                   /* uniform pressure removed before the operator
+                                       arithmetic (only grad p enters); 0 = none */
 } sodhip_stage_params;
 
 /* One stage of rk_4_ls_main (time_integ_ls.f90:339-361) with updateF

@@ -31,7 +31,7 @@ class ViscParams(C.Structure):
 class StageParams(C.Structure):
     """sodhip_stage_params: physical constants of the fused RK stage."""
     _fields_ = [("gamma_gas", D), ("Rgas", D), ("Cp", D), ("Pr", D), ("mu_factor", D),
-                ("c_rho", D), ("c_ener", D), ("fact", D)]
+                ("c_rho", D), ("c_ener", D), ("fact", D), ("p_ref", D)]
 
 
 # name -> (restype, argtypes)

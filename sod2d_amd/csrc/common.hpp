@@ -41,18 +41,18 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// Node-ordered incidence (CSR) of a connectivity: for every node n (0-based),
-// entries row[n]..row[n+1]-1 of col hold e*nnode+a for each (e,a) with
-// connec(e,a) == n+1, in ascending (e,a) order -- the order in which the
-// reference's serial element loop adds element contributions.
+// Node-ordered incidence of a connectivity: node n (0-based) owns the slots
+// row[n]..row[n+1]-1, one per (e,a) with connec(e,a) == n+1, in ascending (e,a)
+// order -- the order in which the reference's serial element loop adds element
+// contributions.  slot[e*nnode+a] is the slot of (e,a): element kernels write
+// their per element-node residuals there, so that the per-node gather reads
+// contiguous memory.
 struct MeshCache {
     const int *connec_key = nullptr; // device pointer of the caller's connec
     int porder = 0, nelem = 0, npoin = 0;
     int *connecT = nullptr; // (nelem, nnode) element-major, 0-based node ids
     int *row = nullptr;     // npoin+1
-    int *col = nullptr;     // nelem*nnode
-    // per-element-node index of the node's CSR row start is not needed: the
-    // gather kernel walks rows.
+    int *slot = nullptr;    // nelem*nnode
 };
 
 struct PresentEntry {

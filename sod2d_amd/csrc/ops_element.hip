@@ -40,8 +40,8 @@ template <int P> struct EG {
 // Per-lane line tables: the P other nodes of the lane's three GLL lines.
 template <int P, typename T> struct LineTab {
     int off[3][P];
-    T ds[3], d[3][P];   // d/dxi_k at the lane's point: own and other coefficients
-    T dsT[3], dT[3][P]; // transposed coefficients for the weak divergence
+    int own[3];       // the lane's position along each line (0-based)
+    T ds[3], d[3][P]; // d/dxi_k at the lane's point: own and other coefficients
 };
 
 template <int P, typename T>
@@ -58,8 +58,8 @@ __device__ __forceinline__ void build_linetab(int a, const T *__restrict__ dl,
                         min(max(gK[a] - 1, 0), P)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+        t.own[k] = own[k];
         t.ds[k] = dl[a + NN * (k + 3 * own[k])];
-        t.dsT[k] = dl[a + NN * (k + 3 * own[k])];
 #pragma unroll
         for (int o = 0; o < P; ++o) {
             const int m = o + (o >= own[k] ? 1 : 0);
@@ -68,7 +68,6 @@ __device__ __forceinline__ void build_linetab(int a, const T *__restrict__ dl,
             const int b = min(max(inv[idx[0] + N1 * (idx[1] + N1 * idx[2])] - 1, 0), NN - 1);
             t.off[k][o] = b;
             t.d[k][o] = dl[a + NN * (k + 3 * m)];
-            t.dT[k][o] = dl[b + NN * (k + 3 * own[k])];
         }
     }
 }
@@ -100,8 +99,9 @@ template <typename T> struct RhsArgs {
     T Cp, Pr, c_rho, c_ener;
     // conserved inputs (fused stage): FP64 state
     const double *cr, *cq, *cE;
-    double gamma, Rgas, mu_factor;
-    T *out; // SoA [5][EN]: mass, mom1, mom2, mom3, energy
+    double gamma, Rgas, mu_factor, p_ref;
+    const int *slot; // node-ordered output slot of each (element, node)
+    T *out;          // SoA [5][EN]: mass, mom1, mom2, mom3, energy, in slot order
 };
 
 // Sutherland's law (mod_fluid_viscosity.f90:21-41)
@@ -112,260 +112,393 @@ template <typename T> __device__ __forceinline__ T sutherland(T mu_factor, T Tk)
 
 // CONV: split-form convection (full_convec_ijk); DIFF: viscous operator
 // (full_diffusion_ijk); CONS: inputs are the conserved FP64 state (the fused
-// stage computes the primitives of updateF's EOS block in-kernel) instead of the
-// primitive arrays.
+// stage forms the primitives of updateF's EOS block and Sutherland's viscosity
+// in-kernel) instead of the primitive arrays.
+//
+// Per element (one lane per node / quadrature point), in phases that keep the
+// live register set small (values cross phases through LDS):
+//   stage   nodal fields -> LDS;
+//   conv    velocity lines: d/dxi u_i, the contravariant line weights
+//           W_k(m) = D_k(m) sum_j He_g(j,k) u_j(m), div(u_i u), div u; then the
+//           density, pressure, H, rho*H and momentum lines against W;
+//   diff 1  density, temperature and velocity lines -> grad rho, grad T, grad u
+//           -> viscous fluxes mapped to contravariant components
+//           G_k(F) = gpvol sum_d He(d,k) F_d, staged in LDS;
+//   diff 2  weak divergence sum_k sum_{b on k-line} dl(b,k,I_k(a)) G_k(F)(b)
+//           with the transposed coefficients kept per lane in LDS;
+//   out     residuals written to the node-ordered slots of the incidence table.
 template <int P, typename T, bool CONV, bool DIFF, bool CONS>
-__global__ __launch_bounds__(EG<P>::NT) void k_rhs(RhsArgs<T> A)
+__global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
 {
-    constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
-    // pass-1 fields
+    constexpr int N1 = EG<P>::N1, NN = EG<P>::NN, NT = EG<P>::NT;
     constexpr int F_RHO = 0, F_U = 1, F_T = 4;
     constexpr int F_C = DIFF ? 5 : 4; // first convection-only field
     constexpr int F_P = F_C, F_H = F_C + 1, F_RH = F_C + 2, F_Q = F_C + 3;
     constexpr int NF1 = CONV ? F_C + 6 : F_C;
     constexpr int NF2 = DIFF ? 18 : 0;
-    __shared__ T s1[NF1 * NT];
-    __shared__ T s2[(NF2 > 0 ? NF2 : 1) * NT];
+    // the viscous fluxes (pass 2) reuse the nodal-field area: one LDS region
+    // of max(NF1, NF2) fields keeps the block small enough for several blocks
+    // per CU
+    __shared__ T s1[(NF1 > NF2 ? NF1 : NF2) * NT];
+    T *const s2 = s1;
 
     const int lane = threadIdx.x;
     const bool act = lane < NN;
     const int a = act ? lane : 0;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    // line derivative d/dxi_k of LDS field f at the lane's point
+    auto dxi = [&](int f, int k, T own) {
+        T v = lt.ds[k] * own;
+#pragma unroll
+        for (int o = 0; o < P; ++o) v += lt.d[k][o] * s1[f * NT + lt.off[k][o]];
+        return v;
+    };
+
+    // Inputs of one element at the lane's node; the next element's are loaded
+    // while the current one is computed (software pipelining of the global
+    // loads, whose latency would otherwise be exposed at low occupancy).
+    struct In {
+        double cr, cq[3], cE;                // conserved state (CONS)
+        T rho, u[3], q[3], pr, E, Tem, rhon, muf; // primitive inputs (drop-in)
+        T mue, musgs;
+        int slot;
+    };
+    auto load = [&](In &x, int e, int n) {
+        const size_t ea = (size_t)e * NN + a;
+        if constexpr (CONS) {
+            x.cr = A.cr[n];
+            x.cq[0] = A.cq[n];
+            x.cq[1] = A.cq[n + A.npoin];
+            x.cq[2] = A.cq[n + 2 * (size_t)A.npoin];
+            x.cE = A.cE[n];
+        } else {
+            x.rho = A.rho[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) x.u[d] = A.u[n + (size_t)A.npoin * d];
+            if constexpr (CONV) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x.q[d] = A.q[n + (size_t)A.npoin * d];
+                x.pr = A.pr[n];
+                x.E = A.E[n];
+            }
+            if constexpr (DIFF) {
+                x.Tem = A.Tem[n];
+                x.rhon = A.rho_n[n];
+                x.muf = A.mu_fluid[n];
+                x.musgs = A.mu_sgs ? A.mu_sgs[e + (size_t)A.nelem * a] : T(0);
+            }
+        }
+        if constexpr (DIFF) x.mue = A.mu_e[e + (size_t)A.nelem * a];
+        x.slot = A.slot[ea];
+    };
 
     const ElemRange er = elem_range(A.nelem);
+    if (er.begin >= er.end) return;
+    // element indices past the block's range are clamped to its last element:
+    // the extra loads are harmless and keep the load stream unconditional
+    const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
+    In cur, nxt;
+    load(cur, er.begin, A.connecT[(size_t)er.begin * NN + a]);
+    int n_nx = A.connecT[(size_t)min(er.begin + er.stride, elast) * NN + a];
     for (int e = er.begin; e < er.end; e += er.stride) {
-        const size_t ea = (size_t)e * NN + a;
-        const int n = act ? A.connecT[ea] : 0;
-        // ---------------- own node values
-        T rho, u[3], Tk = T(0), muf = T(0), rhon = T(0), mue = T(0), musgs = T(0);
-        T p = T(0), H = T(0), rhoH = T(0), q[3] = {T(0), T(0), T(0)};
-        if constexpr (CONS) {
-            const double r = A.cr[n];
-            const double qq[3] = {A.cq[n], A.cq[n + A.npoin], A.cq[n + 2 * (size_t)A.npoin]};
-            const double EE = A.cE[n];
-            double ud[3], umag = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                ud[d] = qq[d] / r;
-                umag += ud[d] * ud[d];
-            }
-            const double eint = EE / r - 0.5 * umag;
-            const double pd = r * (A.gamma - 1.0) * eint;
-            const double hd = (A.gamma / (A.gamma - 1.0)) * pd / r;
-            const double Td = pd / (r * A.Rgas);
-            rho = T(r);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                u[d] = T(ud[d]);
-                q[d] = T(qq[d]);
-            }
-            p = T(pd);
-            const T kin = T(0.5 * umag);
-            H = T(hd) + kin;
-            rhoH = rho * H;
-            Tk = T(Td);
-            rhon = rho;
-            muf = T(sutherland<double>(A.mu_factor, Td));
-            mue = A.mu_e[e + (size_t)A.nelem * a];
-        } else {
-            rho = A.rho[n];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) u[d] = A.u[n + (size_t)A.npoin * d];
-            if constexpr (CONV) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) q[d] = A.q[n + (size_t)A.npoin * d];
-                p = A.pr[n];
-                const T kin = u[0] * u[0] * T(0.5) + u[1] * u[1] * T(0.5) + u[2] * u[2] * T(0.5);
-                H = A.E[n] + kin;
-                rhoH = rho * H;
-            }
-            if constexpr (DIFF) {
-                Tk = A.Tem[n];
-                rhon = A.rho_n[n];
-                muf = A.mu_fluid[n];
-                mue = A.mu_e[e + (size_t)A.nelem * a];
-                musgs = A.mu_sgs ? A.mu_sgs[e + (size_t)A.nelem * a] : T(0);
-            }
-        }
-        // ---------------- stage nodal fields in LDS
-        if (act) {
-            s1[F_RHO * NT + a] = rho;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s1[(F_U + d) * NT + a] = u[d];
-            if constexpr (DIFF) s1[F_T * NT + a] = Tk;
-            if constexpr (CONV) {
-                s1[F_P * NT + a] = p;
-                s1[F_H * NT + a] = H;
-                s1[F_RH * NT + a] = rhoH;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) s1[(F_Q + d) * NT + a] = q[d];
-            }
-        }
-        // element geometry at the lane's point
-        T He[3][3];
-        const T *hp = A.He + 9 * ea;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) He[i][j] = act ? hp[i + 3 * j] : T(0);
-        const T vol = act ? A.gpvol[ea] : T(0);
-        __syncthreads();
-
-        // contravariant components at the point: Ut_k = sum_j He(j,k) u_j
-        T Ut[3], Qt[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
-            Qt[k] = He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2];
-        }
-        // convection accumulators
-        T urho = 0, gp[3] = {0, 0, 0}, qH = 0, urhoH = 0, qgu[3] = {0, 0, 0}, ugq[3] = {0, 0, 0};
-        T divQ = 0, divFq[3] = {0, 0, 0}, divFuu[3] = {0, 0, 0}, divRHu = 0, divHu = 0, divUw = 0;
-        // diffusion accumulators: physical gradients
-        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const T c0 = lt.ds[k];
-            T dr = c0 * rho, du[3], dT = c0 * Tk, dp = c0 * p, dH = c0 * H, dRH = c0 * rhoH, dq[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                du[d] = c0 * u[d];
-                dq[d] = c0 * q[d];
-            }
-            if constexpr (CONV) {
-                const T Wo = c0 * Ut[k];
+        const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
+        load(nxt, e2, n_nx);
+        const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
+        T mue = T(0), musgs = T(0), muf = T(0), rhon = T(0);
+        if constexpr (DIFF) mue = cur.mue;
+        // ---------------- stage own node values
+        {
+            T rho, u[3], Tk = T(0), p = T(0), H = T(0), rhoH = T(0), q[3] = {T(0), T(0), T(0)};
+            if constexpr (CONS) {
+                const double r = cur.cr;
+                const double ri = 1.0 / r;
+                double ud[3], umag = 0.0;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    divFq[d] += Wo * q[d];
-                    divFuu[d] += Wo * u[d];
+                    ud[d] = cur.cq[d] * ri;
+                    umag += ud[d] * ud[d];
                 }
-                divRHu += Wo * rhoH;
-                divHu += Wo * H;
-                divUw += Wo;
-            }
+                const double eint = cur.cE * ri - 0.5 * umag;
+                const double pd = r * (A.gamma - 1.0) * eint;
+                const double hd = (A.gamma / (A.gamma - 1.0)) * pd * ri;
+                const double Td = pd * ri / A.Rgas;
+                rho = T(r);
 #pragma unroll
-            for (int o = 0; o < P; ++o) {
-                const int b = lt.off[k][o];
-                const T c = lt.d[k][o];
-                const T rb = s1[F_RHO * NT + b];
-                T ub[3];
+                for (int d = 0; d < 3; ++d) {
+                    u[d] = T(ud[d]);
+                    q[d] = T(cur.cq[d]);
+                }
+                // only grad p enters: a uniform reference pressure is removed
+                // before the (possibly FP32) operator arithmetic
+                p = T(pd - A.p_ref);
+                H = T(hd + 0.5 * umag);
+                rhoH = T(r * (hd + 0.5 * umag));
+                Tk = T(Td);
+                rhon = rho;
+                muf = T(sutherland<double>(A.mu_factor, Td));
+            } else {
+                rho = cur.rho;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) ub[d] = s1[(F_U + d) * NT + b];
-                dr += c * rb;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) du[d] += c * ub[d];
-                if constexpr (DIFF) dT += c * s1[F_T * NT + b];
+                for (int d = 0; d < 3; ++d) u[d] = cur.u[d];
                 if constexpr (CONV) {
-                    const T pb = s1[F_P * NT + b], Hb = s1[F_H * NT + b], RHb = s1[F_RH * NT + b];
-                    T qb[3];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) qb[d] = s1[(F_Q + d) * NT + b];
-                    dp += c * pb;
-                    dH += c * Hb;
-                    dRH += c * RHb;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) dq[d] += c * qb[d];
-                    const T W = c * (He[0][k] * ub[0] + He[1][k] * ub[1] + He[2][k] * ub[2]);
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        divFq[d] += W * qb[d];
-                        divFuu[d] += W * ub[d];
-                    }
-                    divRHu += W * RHb;
-                    divHu += W * Hb;
-                    divUw += W;
+                    for (int d = 0; d < 3; ++d) q[d] = cur.q[d];
+                    p = cur.pr;
+                    const T kin = u[0] * u[0] * T(0.5) + u[1] * u[1] * T(0.5) + u[2] * u[2] * T(0.5);
+                    H = cur.E + kin;
+                    rhoH = rho * H;
+                }
+                if constexpr (DIFF) {
+                    Tk = cur.Tem;
+                    rhon = cur.rhon;
+                    muf = cur.muf;
+                    musgs = cur.musgs;
                 }
             }
-            if constexpr (CONV) {
-                urho += Ut[k] * dr;
-                qH += Qt[k] * dH;
-                urhoH += Ut[k] * dRH;
+            if (act) {
+                s1[F_RHO * NT + a] = rho;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    gp[i] += He[i][k] * dp;
-                    qgu[i] += Qt[k] * du[i];
-                    ugq[i] += Ut[k] * dq[i];
-                }
-                divQ += He[0][k] * dq[0] + He[1][k] * dq[1] + He[2][k] * dq[2];
-            }
-            if constexpr (DIFF) {
+                for (int d = 0; d < 3; ++d) s1[(F_U + d) * NT + a] = u[d];
+                if constexpr (DIFF) s1[F_T * NT + a] = Tk;
+                if constexpr (CONV) {
+                    s1[F_P * NT + a] = p;
+                    s1[F_H * NT + a] = H;
+                    s1[F_RH * NT + a] = rhoH;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    gR[i] += He[i][k] * dr;
-                    gT[i] += He[i][k] * dT;
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) gU[i][j] += He[j][k] * du[i];
+                    for (int d = 0; d < 3; ++d) s1[(F_Q + d) * NT + a] = q[d];
                 }
             }
         }
+        T He[3][3];
+        {
+            const T *hp = A.He + 9 * ((size_t)e * NN + a);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
+        }
+        const T vol = A.gpvol[(size_t)e * NN + a];
+        __syncthreads();
 
         T Rm = 0, Rq[3] = {0, 0, 0}, Re = 0;
+        T gR[3] = {0, 0, 0};                           // DIFF: grad rho
+        T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}; // DIFF: gU[i][j] = d u_i / d x_j
+        const T rho = s1[F_RHO * NT + a];
+        T u[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u[i] = s1[(F_U + i) * NT + a];
+        // ---------------- velocity lines: d/dxi u_i (-> grad u) and, for the
+        // convection, the contravariant line weights W and div(u_i u), div u
+        T Wo[3], Wx[3][P];
+        T divUw = 0;
+        {
+            T q[3] = {0, 0, 0}, Qt[3] = {0, 0, 0}, Ut[3] = {0, 0, 0}, divFuu[3] = {0, 0, 0};
+            if constexpr (CONV) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) q[i] = s1[(F_Q + i) * NT + a];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
+                    Qt[k] = He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T c0 = lt.ds[k];
+                T du[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) du[i] = c0 * u[i];
+                if constexpr (CONV) {
+                    Wo[k] = c0 * Ut[k];
+                    divUw += Wo[k];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) divFuu[i] += Wo[k] * u[i];
+                }
+#pragma unroll
+                for (int o = 0; o < P; ++o) {
+                    const int b = lt.off[k][o];
+                    const T c = lt.d[k][o];
+                    T ub[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        ub[i] = s1[(F_U + i) * NT + b];
+                        du[i] += c * ub[i];
+                    }
+                    if constexpr (CONV) {
+                        const T w = c * (He[0][k] * ub[0] + He[1][k] * ub[1] + He[2][k] * ub[2]);
+                        Wx[k][o] = w;
+                        divUw += w;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) divFuu[i] += w * ub[i];
+                    }
+                }
+                if constexpr (CONV) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Rq[i] += Qt[k] * du[i];
+                }
+                if constexpr (DIFF) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) gU[i][j] += He[j][k] * du[i];
+                }
+            }
+            if constexpr (CONV) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Rq[i] = T(0.25) * (Rq[i] + rho * divFuu[i]);
+            }
+        }
+        // ---------------- scalar and momentum lines, one field per (rolled)
+        // iteration: line derivative d (and, for the convection, the W-weighted
+        // line sum `div` = div(s u)) folded at once into the residuals
+        constexpr int NLF = CONV ? 7 : 1; // rho[, p, H, rhoH, q1, q2, q3]
+        T divQ = 0;
+#pragma unroll 1
+        for (int fi = 0; fi < NLF; ++fi) {
+            const int f = fi == 0 ? F_RHO : (fi == 1 ? F_P : (fi == 2 ? F_H : (fi == 3 ? F_RH : F_Q + fi - 4)));
+            const T *sf = s1 + f * NT;
+            const T own = sf[a];
+            T d[3], div = 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                T v = lt.ds[k] * own, w = T(0);
+                if constexpr (CONV) w = Wo[k] * own;
+#pragma unroll
+                for (int o = 0; o < P; ++o) {
+                    const T x = sf[lt.off[k][o]];
+                    v += lt.d[k][o] * x;
+                    if constexpr (CONV) w += Wx[k][o] * x;
+                }
+                d[k] = v;
+                div += w;
+            }
+            if (fi == 0) {
+                if constexpr (DIFF) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) gR[i] = He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                }
+                if constexpr (CONV) {
+                    T Ut[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
+                    const T urho = Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2];
+                    Rm += T(0.5) * urho;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * u[i] * urho;
+                    Re += T(0.25) * s1[F_H * NT + a] * urho;
+                }
+            } else if constexpr (CONV) {
+                if (fi == 1) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Rq[i] += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                } else if (fi == 2) { // H: q.grad H + rho div(H u)
+                    T qt = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        qt += (He[0][k] * s1[F_Q * NT + a] + He[1][k] * s1[(F_Q + 1) * NT + a] +
+                               He[2][k] * s1[(F_Q + 2) * NT + a]) * d[k];
+                    Re += T(0.25) * (qt + rho * div);
+                } else { // rho H, q_i: u.grad s + div(s u)
+                    T ut = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        ut += (He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2]) * d[k];
+                    const T c = T(0.25) * (ut + div);
+                    if (fi == 3) {
+                        Re += c;
+                    } else {
+                        const int i = fi - 4;
+                        const T dq = He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                        divQ += dq;
+                        if (i == 0) Rq[0] += c;
+                        else if (i == 1) Rq[1] += c;
+                        else Rq[2] += c;
+                    }
+                }
+            }
+        }
         if constexpr (CONV) {
-            Rm = T(0.5) * (divQ + rho * divUw) + T(0.5) * urho;
+            const T H = s1[F_H * NT + a], rhoH = s1[F_RH * NT + a];
+            Rm += T(0.5) * (divQ + rho * divUw);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                Rq[i] = T(0.25) * (divFq[i] + u[i] * divQ + q[i] * divUw + rho * divFuu[i]) + gp[i] +
-                        T(0.25) * (u[i] * urho + qgu[i] + ugq[i]);
-            Re = T(0.25) * (divRHu + H * divQ + rhoH * divUw + rho * divHu) +
-                 T(0.25) * (qH + urhoH + H * urho);
+                Rq[i] += T(0.25) * (u[i] * divQ + s1[(F_Q + i) * NT + a] * divUw);
+            Re += T(0.25) * (H * divQ + rhoH * divUw);
             Rm *= vol;
             Re *= vol;
 #pragma unroll
             for (int i = 0; i < 3; ++i) Rq[i] *= vol;
         }
 
+        // ---------------- viscous operator
         if constexpr (DIFF) {
-            // pass 1: viscous fluxes at the point, mapped to contravariant
-            // components G_k(F) = gpvol * sum_d He(d,k) F_d
-            const T mufg = muf + rhon * musgs;
-            const T mut = mufg + mue;
-            const T divU = gU[0][0] + gU[1][1] + gU[2][2];
-            const T tt = mufg * T(2.0 / 3.0) * divU;
-            T tau[3][3];
+            T G[18]; // contravariant viscous fluxes G_k(F) = gpvol sum_d He(d,k) F_d
+            {
+                const T mufg = muf + rhon * musgs;
+                const T mut = mufg + mue;
+                const T divU = gU[0][0] + gU[1][1] + gU[2][2];
+                const T tt = mufg * T(2.0 / 3.0) * divU;
+                T tau[3][3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
+                    for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
 #pragma unroll
-            for (int i = 0; i < 3; ++i) tau[i][i] -= tt;
-            T tu[3];
+                for (int i = 0; i < 3; ++i) tau[i][i] -= tt;
+                T dT[3];
+                const T Tk = s1[F_T * NT + a];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) tu[i] = tau[i][0] * u[0] + tau[i][1] * u[1] + tau[i][2] * u[2];
-            T vHe[3][3];
+                for (int k = 0; k < 3; ++k) dT[k] = dxi(F_T, k, Tk);
+                T gt[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int i = 0; i < 3; ++i) gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) vHe[i][j] = vol * He[i][j];
-            T G[18];
+                for (int k = 0; k < 3; ++k) {
+                    const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
+                    T tuk = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                G[0 + k] = vHe[0][k] * gR[0] + vHe[1][k] * gR[1] + vHe[2][k] * gR[2];
-                G[3 + k] = vHe[0][k] * gT[0] + vHe[1][k] * gT[1] + vHe[2][k] * gT[2];
-                G[6 + k] = vHe[0][k] * tu[0] + vHe[1][k] * tu[1] + vHe[2][k] * tu[2];
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-                    G[9 + 3 * r + k] = vHe[0][k] * tau[r][0] + vHe[1][k] * tau[r][1] + vHe[2][k] * tau[r][2];
+                    for (int r = 0; r < 3; ++r) {
+                        const T g = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
+                        G[9 + 3 * r + k] = g;
+                        tuk += g * u[r]; // tau is symmetric: contravariant (tau u)_k
+                    }
+                    G[6 + k] = tuk;
+                    G[0 + k] = h0 * gR[0] + h1 * gR[1] + h2 * gR[2];
+                    G[3 + k] = h0 * gt[0] + h1 * gt[1] + h2 * gt[2];
+                }
             }
+            __syncthreads(); // every lane is done with the nodal fields
             if (act) {
 #pragma unroll
                 for (int f = 0; f < 18; ++f) s2[f * NT + a] = G[f];
             }
             __syncthreads();
-            // pass 2: weak divergence WD(F)(a) = sum_k sum_{b on k-line} dl(b,k,I_k(a)) G_k(F)(b)
-            T W6[6];
+            // pass 2: weak divergence with the transposed coefficients
+            // dl(b,k,I_k(a)) of the lane's lines (cache-resident table)
+            T cT[3][P];
 #pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int o = 0; o < P; ++o) cT[k][o] = A.dl[lt.off[k][o] + NN * (k + 3 * lt.own[k])];
+            T W6[6];
+#pragma unroll 1
             for (int fl = 0; fl < 6; ++fl) {
                 T acc = 0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const int f = 3 * fl + k;
-                    acc += lt.dsT[k] * G[f];
+                    const T *sf = s2 + (3 * fl + k) * NT;
+                    T v = lt.ds[k] * sf[a];
 #pragma unroll
-                    for (int o = 0; o < P; ++o) acc += lt.dT[k][o] * s2[f * NT + lt.off[k][o]];
+                    for (int o = 0; o < P; ++o) v += cT[k][o] * sf[lt.off[k][o]];
+                    acc += v;
                 }
-                W6[fl] = acc;
+                if (fl == 0) W6[0] = acc;
+                else if (fl == 1) W6[1] = acc;
+                else if (fl == 2) W6[2] = acc;
+                else if (fl == 3) W6[3] = acc;
+                else if (fl == 4) W6[4] = acc;
+                else W6[5] = acc;
             }
             const T nu_e = A.c_rho * mue / rhon;
             const T kappa = muf * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr +
@@ -377,13 +510,16 @@ __global__ __launch_bounds__(EG<P>::NT) void k_rhs(RhsArgs<T> A)
         }
 
         if (act) {
+            const size_t s = (size_t)cur.slot;
             T *o = A.out;
-            o[ea] = Rm;
-            o[A.EN + ea] = Rq[0];
-            o[2 * A.EN + ea] = Rq[1];
-            o[3 * A.EN + ea] = Rq[2];
-            o[4 * A.EN + ea] = Re;
+            o[s] = Rm;
+            o[A.EN + s] = Rq[0];
+            o[2 * A.EN + s] = Rq[1];
+            o[3 * A.EN + s] = Rq[2];
+            o[4 * A.EN + s] = Re;
         }
+        cur = nxt;
+        n_nx = n_nx2;
         __syncthreads();
     }
 }
@@ -401,42 +537,48 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
     const int a = act ? lane : 0;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
-    const ElemRange er = elem_range(A.nelem);
-    for (int e = er.begin; e < er.end; e += er.stride) {
+    struct In {
+        T et, u[3], fv[3], He[9], vol;
+        int slot;
+    };
+    auto load = [&](In &x, int e) {
         const size_t ea = (size_t)e * NN + a;
         const int n = act ? A.connecT[ea] : 0;
-        const T et = eta[n];
-        T u[3], fv[3];
+        x.et = eta[n];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            u[d] = A.u[n + (size_t)A.npoin * d];
-            fv[d] = f[n + (size_t)A.npoin * d];
+            x.u[d] = A.u[n + (size_t)A.npoin * d];
+            x.fv[d] = f[n + (size_t)A.npoin * d];
         }
-        if (act) {
-            s[a] = et;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                s[(1 + d) * NT + a] = u[d];
-                s[(4 + d) * NT + a] = fv[d];
-            }
-        }
-        T He[3][3];
         const T *hp = A.He + 9 * ea;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 9; ++i) x.He[i] = hp[i];
+        x.vol = A.gpvol[ea];
+        x.slot = A.slot[ea];
+    };
+    const ElemRange er = elem_range(A.nelem);
+    In cur, nxt;
+    if (er.begin < er.end) load(cur, er.begin);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        if (e + er.stride < er.end) load(nxt, e + er.stride);
+        if (act) {
+            s[a] = cur.et;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) He[i][j] = act ? hp[i + 3 * j] : T(0);
-        const T vol = act ? A.gpvol[ea] : T(0);
+            for (int d = 0; d < 3; ++d) {
+                s[(1 + d) * NT + a] = cur.u[d];
+                s[(4 + d) * NT + a] = cur.fv[d];
+            }
+        }
         __syncthreads();
         T gE[3] = {0, 0, 0}, divU = 0, divF = 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const T c0 = lt.ds[k];
-            T de = c0 * et, du[3], df[3];
+            T de = c0 * cur.et, du[3], df[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                du[d] = c0 * u[d];
-                df[d] = c0 * fv[d];
+                du[d] = c0 * cur.u[d];
+                df[d] = c0 * cur.fv[d];
             }
 #pragma unroll
             for (int o = 0; o < P; ++o) {
@@ -451,23 +593,25 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                gE[i] += He[i][k] * de;
-                divU += He[i][k] * du[i];
-                divF += He[i][k] * df[i];
+                const T h = cur.He[i + 3 * k]; // He(i,k)
+                gE[i] += h * de;
+                divU += h * du[i];
+                divF += h * df[i];
             }
         }
-        T r = T(0.5) * (divF + et * divU);
+        T r = T(0.5) * (divF + cur.et * divU);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) r += T(0.5) * (u[i] * gE[i]);
-        if (act) A.out[ea] = vol * r;
+        for (int i = 0; i < 3; ++i) r += T(0.5) * (cur.u[i] * gE[i]);
+        if (act) A.out[cur.slot] = cur.vol * r;
+        cur = nxt;
         __syncthreads();
     }
 }
 
-// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(col_j).
+// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(j), j over
+// the node's slots (contiguous; the element kernels write in slot order).
 template <typename TS, typename TR>
 __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict__ row,
-                                                const int *__restrict__ col,
                                                 const TS *__restrict__ scr, size_t EN, int nf,
                                                 TR *__restrict__ R0, TR *__restrict__ R1,
                                                 TR *__restrict__ R2, TR *__restrict__ R3,
@@ -480,7 +624,7 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
     for (int f = 0; f < nf; ++f) {
         TR acc = init ? TR(0) : R[f][n];
         const TS *s = scr + (size_t)f * EN;
-        for (int j = j0; j < j1; ++j) acc += fact * TR(s[col[j]]);
+        for (int j = j0; j < j1; ++j) acc += fact * TR(s[j]);
         R[f][n] = acc;
     }
 }
@@ -519,8 +663,8 @@ static int launch_gather(sodhip_ctx ctx, const MeshCache *m, const TS *scr, size
 {
     const int bs = 256;
     const int grid = (m->npoin + bs - 1) / bs;
-    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, m->col, scr, EN, nf, R0, R1,
-                                                   R2, R3, R4, init, fact);
+    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, scr, EN, nf, R0, R1, R2, R3,
+                                                   R4, init, fact);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
 }
@@ -556,6 +700,7 @@ static int full_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const i
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
+    A.slot = m->slot;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -603,6 +748,7 @@ static int full_diffusion(sodhip_ctx ctx, int porder, int nelem, int npoin, cons
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
+    A.slot = m->slot;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -640,13 +786,14 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
                        const int *gK, const double *rho, const double *q, const double *E,
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
-                       double mu_factor, double c_rho, double c_ener, T *out)
+                       double mu_factor, double c_rho, double c_ener, double p_ref, T *out)
 {
     RhsArgs<T> A{};
     A.nelem = m->nelem;
     A.npoin = m->npoin;
     A.EN = (size_t)m->nelem * nnode_of(porder);
     A.connecT = m->connecT;
+    A.slot = m->slot;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -661,6 +808,7 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     A.gamma = gamma;
     A.Rgas = Rgas;
     A.mu_factor = mu_factor;
+    A.p_ref = p_ref;
     A.Cp = T(Cp);
     A.Pr = T(Pr);
     A.c_rho = T(c_rho);
@@ -675,12 +823,12 @@ template int rhs_fused_elements<double>(sodhip_ctx, int, const MeshCache *, cons
                                         const double *, const double *, const int *, const int *,
                                         const int *, const int *, const double *, const double *,
                                         const double *, const double *, double, double, double,
-                                        double, double, double, double, double *);
+                                        double, double, double, double, double, double *);
 template int rhs_fused_elements<float>(sodhip_ctx, int, const MeshCache *, const float *,
                                        const float *, const float *, const int *, const int *,
                                        const int *, const int *, const double *, const double *,
                                        const double *, const float *, double, double, double,
-                                       double, double, double, double, float *);
+                                       double, double, double, double, double, float *);
 
 template <typename T>
 static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
@@ -702,6 +850,7 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
+    A.slot = m->slot;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;

@@ -156,23 +156,48 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
     __syncthreads();
     const double norm = s_norm;
     const double w[3] = {0.25, 0.5, 0.25};
-    for (int e = blockIdx.x; e < A.nelem; e += gridDim.x) {
+    // XCD-aware element ranges: blocks of one XCD (blockIdx % 8) walk one
+    // contiguous eighth of the elements, so the element-fastest mu_e / mue_l /
+    // helem lines are filled by one L2
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7, rk = blockIdx.x >> 3;
+    const int chunk = (A.nelem + 7) >> 3;
+    const int eb = xcd * chunk, ee = min(A.nelem, eb + chunk);
+    struct In {
+        double u0, u1, u2, T, rho, Reta, h, hk;
+    };
+    auto load = [&](In &x, int e) {
         const int n = act ? A.connecT[(size_t)e * NN + a] : 0;
-        const double u0 = A.u[n], u1 = A.u[n + A.npoin], u2 = A.u[n + 2 * (size_t)A.npoin];
-        const double aux2 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-        const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * A.Tem[n]);
-        const double rho = A.rho[n];
-        double be = act ? (rho * A.helem_k[e]) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
-        red[lane] = be;
-        __syncthreads();
-        for (int s = NT / 2; s > 0; s >>= 1) {
-            if (lane < s) red[lane] = fmax(red[lane], red[lane + s]);
+        x.u0 = A.u[n];
+        x.u1 = A.u[n + A.npoin];
+        x.u2 = A.u[n + 2 * (size_t)A.npoin];
+        x.T = A.Tem[n];
+        x.rho = A.rho[n];
+        x.Reta = A.Reta[n];
+        x.h = act ? A.helem[e + (size_t)A.nelem * a] : 0.0;
+        x.hk = A.helem_k[e];
+    };
+    In cur, nxt;
+    if (eb + rk < ee) load(cur, eb + rk);
+    for (int e = eb + rk; e < ee; e += per) {
+        if (e + per < ee) load(nxt, e + per);
+        const double aux2 = sqrt(cur.u0 * cur.u0 + cur.u1 * cur.u1 + cur.u2 * cur.u2);
+        const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * cur.T);
+        double be = act ? (cur.rho * cur.hk) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
+        if constexpr (NT == 64) {
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) be = fmax(be, __shfl_xor(be, s, 64));
+        } else {
+            red[lane] = be;
             __syncthreads();
+            for (int s = NT / 2; s > 0; s >>= 1) {
+                if (lane < s) red[lane] = fmax(red[lane], red[lane + s]);
+                __syncthreads();
+            }
+            be = red[0];
         }
-        const double betae = red[0];
-        const double R1 = rho * fabs(A.Reta[n]) / norm;
-        const double h = act ? A.helem[e + (size_t)A.nelem * a] : 0.0;
-        const double Ve = A.ce * R1 * (h * h);
+        const double betae = be;
+        const double R1 = cur.rho * fabs(cur.Reta) / norm;
+        const double Ve = A.ce * R1 * (cur.h * cur.h);
         const double ml = A.cglob * fmin(Ve, betae);
         if (act) {
             sm[a] = ml;
@@ -187,6 +212,7 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
 #pragma unroll
                 for (int nn = 0; nn < 3; ++nn) acc += w[l] * w[m] * w[nn] * sm[src[9 * l + 3 * m + nn]];
         if (act) A.mu_e[e + (size_t)A.nelem * a] = acc;
+        cur = nxt;
         __syncthreads();
     }
 }
@@ -246,7 +272,7 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     A.nparts = kRedBlocks;
     A.mu_e = mu_e;
     A.mue_l = mue_l;
-    const int grid = std::min(nelem, ctx->num_cu * 16);
+    const int grid = elem_grid(ctx, nelem, 16);
     switch (porder) {
     case 2: k_smart_visc<2><<<grid, SG<2>::NT, 0, ctx->stream>>>(A); break;
     case 3: k_smart_visc<3><<<grid, SG<3>::NT, 0, ctx->stream>>>(A); break;

@@ -25,12 +25,12 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
                        const int *gK, const double *rho, const double *q, const double *E,
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
-                       double mu_factor, double c_rho, double c_ener, T *out);
+                       double mu_factor, double c_rho, double c_ener, double p_ref, T *out);
 
 template <typename TS>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ row,
-    const int *__restrict__ col, const TS *__restrict__ scr, size_t EN,
+    const TS *__restrict__ scr, size_t EN,
     const double *__restrict__ invMl, double fact, double lam1, double lams, double gdt,
     const double *__restrict__ rho1, const double *__restrict__ q1, const double *__restrict__ E1,
     double *__restrict__ rho, double *__restrict__ q, double *__restrict__ E,
@@ -42,9 +42,8 @@ __global__ __launch_bounds__(256) void k_gather_update(
     const int j0 = row[n], j1 = row[n + 1];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = j0; j < j1; ++j) {
-        const int c = col[j];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) acc[f] += fact * (double)scr[(size_t)f * EN + c];
+        for (int f = 0; f < 5; ++f) acc[f] += fact * (double)scr[(size_t)f * EN + j];
     }
     const double m = invMl[n];
     const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
@@ -81,12 +80,12 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     T *scr = (T *)ctx->scratch.ptr;
     rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
                                prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
-                               prm->c_rho, prm->c_ener, scr);
+                               prm->c_rho, prm->c_ener, prm->p_ref, scr);
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
     int tok = prof_begin(ctx, 1);
     k_gather_update<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(
-        npoin, npoin_w, lpoin_w, m->row, m->col, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
+        npoin, npoin_w, lpoin_w, m->row, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());

@@ -55,7 +55,7 @@ static void free_mesh(MeshCache *m)
     if (!m) return;
     (void)hipFree(m->connecT);
     (void)hipFree(m->row);
-    (void)hipFree(m->col);
+    (void)hipFree(m->slot);
 }
 
 int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
@@ -74,7 +74,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     SODHIP_TRY_HIP(hipMemcpyAsync(h.data(), connec, en * sizeof(int), hipMemcpyDeviceToHost,
                                   ctx->stream));
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<int> cT(en), row((size_t)npoin + 1, 0), col(en);
+    std::vector<int> cT(en), row((size_t)npoin + 1, 0), slot(en);
     for (size_t i = 0; i < en; ++i) {
         int v = h[i];
         if (v < 1 || v > npoin)
@@ -91,7 +91,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     for (long e = 0; e < nelem; ++e) // ascending (e,a): the reference's serial assembly order
         for (int a = 0; a < nn; ++a) {
             int n = cT[(size_t)e * nn + a];
-            col[fill[n]++] = (int)(e * nn + a);
+            slot[(size_t)e * nn + a] = fill[n]++;
         }
     auto m = std::make_unique<MeshCache>();
     m->connec_key = connec;
@@ -100,7 +100,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     m->npoin = npoin;
     hipError_t e1 = hipMalloc(&m->connecT, en * sizeof(int));
     hipError_t e2 = hipMalloc(&m->row, ((size_t)npoin + 1) * sizeof(int));
-    hipError_t e3 = hipMalloc(&m->col, en * sizeof(int));
+    hipError_t e3 = hipMalloc(&m->slot, en * sizeof(int));
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
         free_mesh(m.get());
         return set_error(SODHIP_ERR_ALLOC, "mesh cache allocation failed");
@@ -109,7 +109,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
                                   ctx->stream));
     SODHIP_TRY_HIP(hipMemcpyAsync(m->row, row.data(), row.size() * sizeof(int),
                                   hipMemcpyHostToDevice, ctx->stream));
-    SODHIP_TRY_HIP(hipMemcpyAsync(m->col, col.data(), en * sizeof(int), hipMemcpyHostToDevice,
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->slot, slot.data(), en * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream));
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
     *out = m.get();

@@ -155,8 +155,11 @@ class TGVRun:
         self.first = True
         self.time = 0.0
         self.istep = 0
+        # FP32 operators: the uniform reference pressure is removed before the
+        # operator arithmetic (only grad p enters the split form)
         self.sp = StageParams(self.t.gamma_gas, self.t.Rgas, self.t.Cp, self.t.Prt,
-                              self.t.mu_factor, 1.0, 1.0, -1.0)
+                              self.t.mu_factor, 1.0, 1.0, -1.0,
+                              self.t.po if op_precision == 4 else 0.0)
         self.vp = ViscParams(0.1, 0.5, 1.0, self.t.Rgas, 1, 0)
         self.ctx.sync_stream()
         # evalInitialViscosity + evalInitialDt (CFDSolverBase.f90:1384-1460)
