@@ -103,6 +103,9 @@ def main(argv=None):
     ap.add_argument("--clean", action="store_true",
                     help="fresh RHS at every stage instead of the reference's sequencing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run 3 plain FP64 copy kernels (y = 1.0 * x) of "
+                         "MIB MiB each: the reference launches for calibrating HBM byte counters")
     a = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -136,6 +139,14 @@ def main(argv=None):
     prof = {k: run.ctx.prof_get(k) for k in range(4)}
     run.ctx.prof_enable(False)
     diag = run.diagnostics()
+    if a.copy_probe > 0:
+        n = a.copy_probe * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
 
     t_step = (t1 - t0) / a.steps
     N, EN = run.npoin_w, run.nelem * run.nnode

@@ -24,7 +24,8 @@ SOURCES = ["runtime.hip", "ops_element.hip", "ops_node.hip", "ops_sensor.hip", "
            "setup.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
-# Packed FP32 VALU forms are disabled on this target.
+# Packed FP32 VALU forms (v_pk_*_f32) are disabled: all FP32 work is issued as
+# scalar-per-lane VALU operations.
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-munsafe-fp-atomics"]
 

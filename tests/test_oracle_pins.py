@@ -1,0 +1,617 @@
+"""Pins of the oracle -- the CPU restatement the device path is checked
+against -- by checks that do not restate its formulas, chosen so that a
+dropped term, a wrong sign or index, or a transposed operand in it fails one:
+
+* the reference's own modules, where they compile from their own sources with
+  flang (oracle/_ref, `make -C oracle ref`): GLL rule (mod_maths.f90:29-39),
+  Lagrange basis (mod_maths.f90:243-323), shape functions and quadrature
+  points (quadrature_rules.f90:51-78, elem_hex.f90:22-43 as driven by
+  CFDSolverBase.f90:1530-1576), the Jacobian (jacobian_oper.f90:7-140),
+  Sutherland's law (mod_fluid_viscosity.f90:21-41) and the sensor's filter
+  tables (mod_filters.f90:15-55).  Skipped when those are not built;
+* closed forms: GLL nodes and weights, quadrature exactness, the Kronecker
+  property of the basis (as unitt_hex64_test.f90:40-55), exactness of the
+  derivative matrix on polynomials, gmsh's hexahedron node layout, affine
+  Jacobians, the volume of the box;
+* operator invariants: uniform states, periodic conservation, the partition of
+  unity of the weak divergence, and convergence of full_convec_ijk and
+  full_diffusion_ijk to the analytic divergences of smooth fields;
+* smart_visc_spectral on states whose result is known in closed form (zero
+  residual, the upper limiter, the impulse response of the filter with the
+  reference's table written out by hand);
+* the RK tables against the stability polynomials of their order.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+import sympy as sp
+
+from helpers import relerr
+from sod2d_amd import case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def ref_lib(p):
+    path = os.path.join(ROOT, "oracle", "_ref", f"p{p}", "libsod2d_ref.so")
+    if not os.path.exists(path):
+        pytest.skip("reference modules not built here (make -C oracle ref)")
+    L = C.CDLL(path)
+    assert L.ref_porder() == p
+    return L
+
+
+def oracle_geometry(O, p, n, periodic, lengths=None, jitter=0.0):
+    """Mesh from the box generator, all geometry from the oracle."""
+    m = case.box_mesh(p, n, periodic=periodic, lengths=lengths, jitter=jitter)
+    T = O.Tables(p)
+    gpvol, He = O.elem_jacobian(p, m.connec_orig, m.coord, T.dNgp, T.wgp)
+    Ml = O.lumped_mass(p, m.connec, gpvol, m.npoin)
+    invMl = O.inv_ml(Ml, m.lpoin_w)
+    return m, T, gpvol, He, Ml, invMl
+
+
+# ============================================================== reference modules
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_ref_gll_and_lagrange(oracle, p):
+    L = ref_lib(p)
+    xi, w = np.zeros(p + 1), np.zeros(p + 1)
+    L.ref_gll(_p(xi), _p(w))
+    oxi, ow = oracle.gll(p)
+    assert np.abs(oxi - xi).max() <= 2e-16 and np.abs(ow - w).max() <= 2e-16
+    L.ref_lagrange.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+    for x in np.random.default_rng(p).uniform(-1, 1, 9).tolist() + xi.tolist():
+        l, dl = np.zeros(p + 1), np.zeros(p + 1)
+        L.ref_lagrange(_p(xi), x, _p(l), _p(dl))
+        assert np.abs(oracle.lagrange(p, xi, x) - l).max() <= 1e-15
+        assert np.abs(oracle.lagrange_deriv(p, xi, x) - dl).max() <= 1e-14 * (p + 1) ** 2
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_ref_shape_functions(oracle, p):
+    L = ref_lib(p)
+    T = oracle.Tables(p)
+    ng = T.nnode
+    xgp, wgp = np.zeros((ng, 3), order="F"), np.zeros(ng)
+    Ngp, dNgp = np.zeros((ng, ng), order="F"), np.zeros((3, ng, ng), order="F")
+    dl = np.zeros((ng, 3, p + 1), order="F")
+    L.ref_shape(_p(np.ascontiguousarray(T.a2ijk, dtype=np.int32)), _p(xgp), _p(wgp), _p(Ngp),
+                _p(dNgp), _p(dl))
+    assert np.abs(T.xgp - xgp).max() <= 2e-16
+    assert np.abs(T.wgp - wgp).max() <= 2e-16
+    assert np.abs(T.Ngp - Ngp).max() <= 1e-15
+    assert np.abs(T.dNgp - dNgp).max() <= 1e-14 * (p + 1) ** 2
+    assert np.abs(T.dlxigp_ip - dl).max() <= 1e-14 * (p + 1) ** 2
+
+
+@pytest.mark.parametrize("p,n", [(2, (2, 3, 2)), (3, (2, 2, 3)), (4, (2, 1, 2))])
+def test_ref_elem_jacobian(oracle, p, n):
+    L = ref_lib(p)
+    T = oracle.Tables(p)
+    m = case.box_mesh(p, n, periodic=False, lengths=(1.0, 2.0, 0.5), jitter=0.1)
+    ne, ng = m.nelem, T.nnode
+    gpvol, He = np.zeros((1, ng, ne), order="F"), np.zeros((3, 3, ng, ne), order="F")
+    L.ref_elem_jacobian.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
+    L.ref_elem_jacobian(ne, m.npoin, _p(m.connec_orig), _p(m.coord), _p(T.dNgp), _p(T.wgp),
+                        _p(gpvol), _p(He))
+    og, oH = oracle.elem_jacobian(p, m.connec_orig, m.coord, T.dNgp, T.wgp)
+    assert relerr(og, gpvol) < 1e-15
+    assert relerr(oH, He) < 1e-14
+
+
+def test_ref_sutherland(oracle):
+    L = ref_lib(3)
+    r = np.random.default_rng(3)
+    T = r.uniform(50.0, 3000.0, 257)
+    mf = r.uniform(0.5, 2.0e4, 257)
+    mu = np.zeros(257)
+    L.ref_sutherland.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ref_sutherland(257, _p(T), _p(mf), _p(mu))
+    om = np.zeros(257)
+    oracle.sutherland(T, mf, om)
+    assert np.abs(om - mu).max() <= 2e-16 * np.abs(mu).max()
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_ref_filter_tables(oracle, p):
+    L = ref_lib(p)
+    cijk = np.zeros(p + 3, np.int32)
+    al, am, an = np.zeros(3), np.zeros(3), np.zeros(3)
+    L.ref_filters(_p(cijk), _p(al), _p(am), _p(an))
+    assert (oracle.convert_ijk(p) == cijk).all()
+    # the oracle's sensor hard-codes the weights (1/4, 1/2, 1/4) in all directions
+    for wts in (al, am, an):
+        assert (wts == np.array([0.25, 0.5, 0.25])).all()
+
+
+# ============================================================== closed forms: basis
+def test_gll_closed_forms(oracle):
+    xi, w = oracle.gll(2)
+    assert np.abs(xi - [-1, 0, 1]).max() < 1e-15
+    assert np.abs(w - [1 / 3, 4 / 3, 1 / 3]).max() < 1e-15
+    xi, w = oracle.gll(3)
+    s = 1 / math.sqrt(5)
+    assert np.abs(xi - [-1, -s, s, 1]).max() < 1e-15
+    assert np.abs(w - [1 / 6, 5 / 6, 5 / 6, 1 / 6]).max() < 1e-15
+    xi, w = oracle.gll(4)
+    s = math.sqrt(3 / 7)
+    assert np.abs(xi - [-1, -s, 0, s, 1]).max() < 1e-15
+    assert np.abs(w - [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10]).max() < 1e-15
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
+def test_gll_exactness_and_basis(oracle, p):
+    xi, w = oracle.gll(p)
+    # GLL with p+1 points integrates polynomials of degree <= 2p-1 exactly
+    for k in range(2 * p):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs((w * xi ** k).sum() - exact) < 1e-14
+    # basis i belongs to node xi[lo[i]], lo = the 1-D order [-1, +1, interior]
+    # of mod_maths.f90:264-268; Kronecker property and exactness of the
+    # derivative on polynomials of degree <= p
+    lo = [0, p] + list(range(1, p))
+    xl = xi[lo]
+    for b in range(p + 1):
+        l = oracle.lagrange(p, xi, xl[b])
+        assert np.abs(l - np.eye(p + 1)[b]).max() < 1e-15
+    for x in np.linspace(-1, 1, 7):
+        assert abs(oracle.lagrange(p, xi, x).sum() - 1.0) < 1e-14
+        dl = oracle.lagrange_deriv(p, xi, x)
+        assert abs(dl.sum()) < 1e-13
+        for k in range(1, p + 1):
+            assert abs((dl * xl ** k).sum() - k * x ** (k - 1)) < 1e-12
+
+
+# gmsh hexahedron layout: corners, then the 12 edges in this order, each run
+# from its first to its second corner (elem_hex.f90:17), then the 6 faces in
+# the order of elem_hex.f90:18, then the interior as a hexahedron of order p-2.
+GMSH_CORNERS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1),
+                (0, 1, 1)]
+GMSH_EDGES = [(1, 2), (1, 4), (1, 5), (2, 3), (2, 6), (3, 4), (3, 7), (4, 8), (5, 6), (5, 8),
+              (6, 7), (7, 8)]
+GMSH_FACES = [(1, 4, 3, 2), (1, 2, 6, 5), (1, 5, 8, 4), (2, 3, 7, 6), (3, 4, 8, 7), (5, 6, 7, 8)]
+
+
+def _check_gmsh_table(t, p, off=0):
+    if p == 0:
+        assert tuple(t[0]) == (off, off, off)
+        return
+    c = [tuple(off + p * np.array(v)) for v in GMSH_CORNERS]
+    for r in range(8):
+        assert tuple(t[r]) == c[r]
+    r = 8
+    for a, b in GMSH_EDGES:
+        A, B = np.array(c[a - 1]), np.array(c[b - 1])
+        for i in range(1, p):
+            assert tuple(t[r]) == tuple(A + (B - A) * i // p), (r, a, b)
+            r += 1
+    for f in GMSH_FACES:
+        V = [np.array(c[v - 1]) for v in f]
+        nf = (p - 1) ** 2
+        pts = [tuple(t[r + q]) for q in range(nf)]
+        # every face node lies inside the face, and the first (p>2: four) nodes
+        # sit next to the face's corners in the face's vertex order
+        for q in pts:
+            lam = np.linalg.lstsq(np.stack([V[1] - V[0], V[3] - V[0]], 1).astype(float),
+                                  np.array(q, float) - V[0], rcond=None)[0]
+            assert np.allclose(np.stack([V[1] - V[0], V[3] - V[0]], 1) @ lam + V[0], q)
+            assert (lam > 0).all() and (lam < 1).all()
+        if p > 2:
+            for m in range(4):
+                d = [np.abs(np.array(pts[m]) - V[k]).sum() for k in range(4)]
+                assert int(np.argmin(d)) == m
+        r += nf
+    if p >= 2:
+        _check_gmsh_table(t[r:], p - 2, off + 1)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5])
+def test_gmsh_hex_layout(oracle, p):
+    t = np.asarray(oracle.gmsh_hex_table(p))
+    n1 = p + 1
+    assert len({tuple(x) for x in t}) == n1 ** 3 and t.min() == 0 and t.max() == p
+    _check_gmsh_table(t, p)
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_index_maps_and_shape(oracle, p):
+    T = oracle.Tables(p)
+    n1, nn = p + 1, T.nnode
+    assert sorted(T.a2ijk.tolist()) == list(range(1, nn + 1))
+    # invAtoIJK and gmshAtoI/J/K are inverse maps of each other
+    for a in range(1, nn + 1):
+        assert T.invAtoIJK[T.gI[a - 1] - 1, T.gJ[a - 1] - 1, T.gK[a - 1] - 1] == a
+    # nodes are the quadrature points: N_a(x_b) = delta_ab (unitt_hex64_test.f90:40-55)
+    assert np.abs(T.Ngp - np.eye(nn)).max() < 1e-14
+    # quadrature points of one 1-D index share a coordinate, ordered as the
+    # reference's 1-D order [1, p+1, 2..p] of the GLL roots
+    xi, w1 = oracle.gll(p)
+    lo = [0, p] + list(range(1, p))
+    for a in range(nn):
+        i, j, k = T.gI[a] - 1, T.gJ[a] - 1, T.gK[a] - 1
+        assert np.abs(T.xgp[a] - [xi[lo[i]], xi[lo[j]], xi[lo[k]]]).max() == 0.0
+        assert abs(T.wgp[a] - w1[lo[i]] * w1[lo[j]] * w1[lo[k]]) < 1e-16
+    # derivative of the interpolant of a trilinear-times-degree-p field is exact
+    f = lambda x: (x[:, 0] ** p) * (1 + x[:, 1]) - 3 * x[:, 2] ** (p - 1) * x[:, 0]  # noqa: E731
+    df = lambda x: np.stack([p * x[:, 0] ** (p - 1) * (1 + x[:, 1]) - 3 * x[:, 2] ** (p - 1),  # noqa
+                             x[:, 0] ** p, -3 * (p - 1) * x[:, 2] ** (p - 2) * x[:, 0]], 1)
+    fv = f(T.xgp)
+    grad = np.einsum("dag,a->gd", T.dNgp, fv)
+    assert np.abs(grad - df(T.xgp)).max() < 1e-12
+
+
+# ============================================================== closed forms: geometry
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_affine_jacobian(oracle, p):
+    n, Lx = (2, 3, 4), (1.0, 2.0, 3.0)
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, n, False, lengths=Lx)
+    h = np.array(Lx) / np.array(n)
+    for d in range(3):
+        for e in range(3):
+            expect = 2.0 / h[d] if d == e else 0.0
+            assert np.abs(He[d, e] - expect).max() < 1e-12
+    detJ = np.prod(h / 2)
+    assert np.abs(gpvol[0] - T.wgp[:, None] * detJ).max() < 1e-14
+    vol = np.prod(Lx)
+    assert abs(gpvol.sum() - vol) < 1e-12 * vol
+    assert abs(Ml.sum() - vol) < 1e-12 * vol and (Ml > 0).all()
+
+
+def test_jittered_jacobian_volume(oracle):
+    # the jitter of the generator keeps boundary nodes on the box faces, so the
+    # integral of det J over the mesh is the box volume up to quadrature error
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, 4, (3, 2, 2), False,
+                                                 lengths=(1.0, 1.0, 1.0), jitter=0.05)
+    assert (gpvol > 0).all()
+    assert abs(gpvol.sum() - 1.0) < 1e-3
+
+
+# ============================================================== operator invariants
+def _fields(m, rng):
+    npoin = m.npoin
+    rho = rng.uniform(0.9, 1.1, npoin)
+    u = np.asfortranarray(rng.normal(0, 0.3, (npoin, 3)))
+    pr = rng.uniform(0.9, 1.1, npoin) / 1.4
+    return rho, u, pr
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_convec_uniform_state_is_zero(oracle, p):
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, 2, False, lengths=(1, 1, 1),
+                                                 jitter=0.1)
+    npoin = m.npoin
+    u = np.asfortranarray(np.tile([0.3, -0.2, 0.1], (npoin, 1)))
+    rho, pr = np.full(npoin, 1.2), np.full(npoin, 0.7)
+    q = np.asfortranarray(rho[:, None] * u)
+    h = 1.4 / 0.4 * pr / rho
+    Rm, Rq, Re = oracle.full_convec_ijk(T, m.connec, He, gpvol, u, q, rho, pr, h)
+    for R in (Rm, Rq, Re):
+        assert np.abs(R).max() < 1e-13
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_convec_periodic_conservation(oracle, p):
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, 3, True)
+    rho, u, pr = _fields(m, np.random.default_rng(p))
+    q = np.asfortranarray(rho[:, None] * u)
+    h = 1.4 / 0.4 * pr / rho
+    Rm, Rq, Re = oracle.full_convec_ijk(T, m.connec, He, gpvol, u, q, rho, pr, h)
+    w = m.lpoin_w - 1
+    for R in (Rm, Rq[:, 0], Rq[:, 1], Rq[:, 2], Re):
+        assert abs(R[w].sum()) < 1e-13 * np.abs(R[w]).sum()
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_diffusion_partition_of_unity(oracle, p):
+    """The weak divergence sum_a int grad(N_a).F vanishes when summed over the
+    nodes of any mesh (sum_a N_a = 1), whatever the fluxes.  The momentum
+    residual is such a divergence.  The energy and mass residuals multiply
+    their heat / density-diffusion fluxes by kappa_e and nu_e taken at the
+    output node, not at the quadrature point (elem_diffu.f90:169-192), so they
+    sum to zero only where those coefficients are uniform."""
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, (2, 3, 2), False,
+                                                 lengths=(1, 2, 1), jitter=0.1)
+    r = np.random.default_rng(5)
+    rho, u, pr = _fields(m, r)
+    Tem = pr / rho
+    nn = T.nnode
+    mu_f = r.uniform(1e-4, 2e-4, m.npoin)
+    mu_e = np.asfortranarray(r.uniform(0, 1e-3, (m.nelem, nn)))
+    mu_sgs = np.asfortranarray(r.uniform(0, 1e-4, (m.nelem, nn)))
+    Rm, Rq, Re = oracle.full_diffusion_ijk(T, m.connec, He, gpvol, 1004.0, 0.71, rho, rho, u, Tem,
+                                           mu_f, mu_e, mu_sgs)
+    for R in (Rq[:, 0], Rq[:, 1], Rq[:, 2]):
+        assert abs(R.sum()) < 1e-13 * np.abs(R).sum()
+    assert abs(Re.sum()) > 1e-3 * np.abs(Re).sum()
+    assert abs(Rm.sum()) > 1e-3 * np.abs(Rm).sum()
+    # uniform coefficients: every residual sums to zero
+    one = np.ones(m.npoin)
+    z = np.zeros((m.nelem, nn), order="F")
+    Rm, Rq, Re = oracle.full_diffusion_ijk(T, m.connec, He, gpvol, 1004.0, 0.71, one, rho, u, Tem,
+                                           np.full(m.npoin, 1.5e-4), np.full_like(z, 3e-4), z)
+    for R in (Rm, Rq[:, 0], Rq[:, 1], Rq[:, 2], Re):
+        assert abs(R.sum()) < 1e-13 * np.abs(R).sum()
+    # a uniform state has no diffusive flux
+    uu = np.asfortranarray(np.tile([0.3, -0.2, 0.1], (m.npoin, 1)))
+    one = np.ones(m.npoin)
+    Rm, Rq, Re = oracle.full_diffusion_ijk(T, m.connec, He, gpvol, 1004.0, 0.71, one, one, uu,
+                                           one, mu_f, mu_e, mu_sgs)
+    for R in (Rm, Rq, Re):
+        assert np.abs(R).max() < 1e-15
+
+
+# ---- convergence to the analytic operators on smooth periodic fields
+X, Y, Z = sp.symbols("x y z")
+S_RHO = 1 + sp.Rational(1, 10) * sp.sin(X + 2 * Y) * sp.cos(Z)
+S_U = [sp.Rational(1, 2) * sp.sin(X) * sp.cos(Y) * sp.cos(Z),
+       -sp.Rational(3, 10) * sp.cos(X) * sp.sin(Y) + sp.Rational(1, 10) * sp.cos(2 * Z),
+       sp.Rational(1, 5) * sp.sin(Y + Z)]
+S_P = sp.Rational(7, 10) + sp.Rational(1, 10) * sp.cos(X) * sp.cos(2 * Z)
+GAMMA = sp.Rational(7, 5)
+
+
+def _lam(expr):
+    f = sp.lambdify((X, Y, Z), expr, "numpy")
+    return lambda c: np.broadcast_to(f(c[:, 0], c[:, 1], c[:, 2]), (c.shape[0],)).astype(float)
+
+
+def _convec_exact():
+    div = lambda F: sum(sp.diff(F[i], v) for i, v in enumerate((X, Y, Z)))  # noqa: E731
+    k = sum(ui ** 2 for ui in S_U) / 2
+    H = GAMMA / (GAMMA - 1) * S_P / S_RHO + k
+    mass = div([S_RHO * ui for ui in S_U])
+    mom = [div([S_RHO * S_U[i] * uj for uj in S_U]) + sp.diff(S_P, v)
+           for i, v in enumerate((X, Y, Z))]
+    ener = div([S_RHO * H * ui for ui in S_U])
+    return _lam(mass), [_lam(e) for e in mom], _lam(ener)
+
+
+def _diffu_exact(mu, Cp, Pr):
+    V = (X, Y, Z)
+    gu = [[sp.diff(S_U[i], V[j]) for j in range(3)] for i in range(3)]
+    divu = sum(gu[i][i] for i in range(3))
+    tau = [[mu * (gu[i][j] + gu[j][i]) - (sp.Rational(2, 3) * mu * divu if i == j else 0)
+            for j in range(3)] for i in range(3)]
+    Tem = S_P / S_RHO
+    mom = [sum(sp.diff(tau[i][j], V[j]) for j in range(3)) for i in range(3)]
+    kap = mu * Cp / Pr
+    ener = sum(sp.diff(sum(tau[i][j] * S_U[j] for j in range(3)) + kap * sp.diff(Tem, V[i]), V[i])
+               for i in range(3))
+    return [_lam(e) for e in mom], _lam(ener), _lam(Tem)
+
+
+def _smooth_state(m):
+    c = m.coord
+    rho = _lam(S_RHO)(c)
+    u = np.asfortranarray(np.stack([_lam(e)(c) for e in S_U], 1))
+    pr = _lam(S_P)(c)
+    return rho, u, pr
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_convec_converges_to_divergence(oracle, p):
+    ex_mass, ex_mom, ex_ener = _convec_exact()
+    errs = []
+    for n in (8, 16):
+        m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, n, True)
+        rho, u, pr = _smooth_state(m)
+        q = np.asfortranarray(rho[:, None] * u)
+        h = 1.4 / 0.4 * pr / rho
+        Rm, Rq, Re = oracle.full_convec_ijk(T, m.connec, He, gpvol, u, q, rho, pr, h,
+                                            initialize=True, fact=-1.0)
+        w = m.lpoin_w - 1
+        c = m.coord[w]
+        e = [relerr(Rm[w] * invMl[w], -ex_mass(c)), relerr(Re[w] * invMl[w], -ex_ener(c))]
+        e += [relerr(Rq[w, d] * invMl[w], -ex_mom[d](c)) for d in range(3)]
+        errs.append(np.array(e))
+    # the discrete residual approaches minus the analytic divergence at order p
+    assert (errs[1] < {2: 6e-2, 3: 5e-3}[p]).all(), errs
+    assert (errs[0] / errs[1] > 2 ** p * 0.75).all(), errs
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_diffusion_converges_to_divergence(oracle, p):
+    mu, Cp, Pr = 0.05, 2.0, 0.71
+    ex_mom, ex_ener, ex_T = _diffu_exact(mu, Cp, Pr)
+    errs = []
+    for n in (8, 16):
+        m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, n, True)
+        rho, u, pr = _smooth_state(m)
+        Tem = ex_T(m.coord)
+        z = np.zeros((m.nelem, T.nnode), order="F")
+        Rm, Rq, Re = oracle.full_diffusion_ijk(T, m.connec, He, gpvol, Cp, Pr, rho, rho, u, Tem,
+                                               np.full(m.npoin, mu), z, z, initialize=True,
+                                               fact=-1.0)
+        w = m.lpoin_w - 1
+        c = m.coord[w]
+        assert np.abs(Rm).max() == 0.0
+        e = [relerr(Re[w] * invMl[w], ex_ener(c))]
+        e += [relerr(Rq[w, d] * invMl[w], ex_mom[d](c)) for d in range(3)]
+        errs.append(np.array(e))
+    # the weak viscous divergence converges at order p on these smooth fields
+    assert (errs[1] < {2: 5e-2, 3: 5e-3}[p]).all(), errs
+    assert (errs[0] / errs[1] > 2 ** p * 0.75).all(), errs
+
+
+def test_scalar_convec_converges(oracle):
+    p = 3
+    errs = []
+    eta_s = sp.cos(X) * sp.sin(Y + Z) + 2
+    div = sum(sp.diff(S_U[i] * eta_s, v) for i, v in enumerate((X, Y, Z)))
+    ex = _lam(div)
+    for n in (8, 16):
+        m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, n, True)
+        rho, u, pr = _smooth_state(m)
+        eta = _lam(eta_s)(m.coord)
+        fe = np.asfortranarray(u * eta[:, None])
+        R = oracle.generic_scalar_convec_ijk(T, m.connec, He, gpvol, fe, eta, u)
+        w = m.lpoin_w - 1
+        errs.append(relerr(R[w] * invMl[w], ex(m.coord[w])))
+    assert errs[1] < 2e-3 and errs[0] / errs[1] > 2 ** p * 0.75, errs
+
+
+# ============================================================== sensor
+def _sensor_case(oracle, p, Reta, rho=1.0, uvec=(0.3, 0.4, 0.0), Tem=2.0, normalise=0):
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, 1, False, lengths=(1, 1, 1))
+    npoin, nn = m.npoin, T.nnode
+    helem = oracle.char_length(p, m.connec_orig, m.coord)
+    helem_l = oracle.char_length_spectral(p, m.connec_orig, m.coord)
+    u = np.asfortranarray(np.tile(uvec, (npoin, 1)))
+    prm = oracle.visc_params(287.0, flag_normalise_entropy=normalise)
+    mu_e = np.zeros((m.nelem, nn), order="F")
+    oracle.smart_visc_spectral(T, m.connec, m.lpoin_w, Reta, 1.4, np.full(npoin, rho), u,
+                               np.full(npoin, Tem), np.zeros(npoin), helem_l, helem, prm, mu_e)
+    return m, T, mu_e, helem, helem_l, prm
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_sensor_zero_and_limiter(oracle, p):
+    m, T, mu_e, *_ = _sensor_case(oracle, p, np.zeros((p + 1) ** 3))
+    assert (mu_e == 0).all()
+    # a huge residual saturates at the limiter beta_e = rho h (cmax/p)(|u| + c);
+    # the filter weights sum to one, so the filtered field is that constant
+    rho, Tem = 1.3, 2.0
+    m, T, mu_e, helem, helem_l, prm = _sensor_case(oracle, p, np.full((p + 1) ** 3, 1e30), rho=rho,
+                                                   Tem=Tem)
+    beta = rho * helem[0] * (0.5 / p) * (0.5 + math.sqrt(287.0 * 1.4 * Tem))
+    assert np.abs(mu_e - beta).max() < 1e-15 * beta
+
+
+def test_sensor_filter_impulse_p3(oracle):
+    """One node with a small residual: the output is the filter's impulse
+    response.  The 1-D filter matrix below is written out by hand from the
+    reference's table convertIJK(0:5) = [3,1,3,4,2,3] (mod_filters.f90:44-50,
+    checked against the compiled module above): rows are output table indices
+    1..4 (1-D order [-1, +1, -1/sqrt5, +1/sqrt5]), columns source indices."""
+    p = 3
+    W = np.zeros((4, 4))
+    W[0, [0, 2]] = [0.5, 0.5]             # index 1: ghost mirrored to index 3
+    W[2, [0, 2, 3]] = [0.25, 0.5, 0.25]   # index 3: neighbours 1 and 4
+    W[3, [2, 3, 1]] = [0.25, 0.5, 0.25]   # index 4: neighbours 3 and 2
+    W[1, [3, 1, 2]] = [0.25, 0.5, 0.25]   # index 2: ghost taken from index 3
+    T = oracle.Tables(p)
+    for src in [(0, 0, 0), (2, 3, 1), (1, 1, 2), (3, 2, 0)]:
+        a_src = T.invAtoIJK[src] - 1
+        m0 = case.box_mesh(p, 1, periodic=False, lengths=(1, 1, 1))
+        Reta = np.zeros(m0.npoin)
+        node = m0.connec[0, a_src] - 1
+        Reta[node] = 1e-9
+        m, T, mu_e, helem, helem_l, prm = _sensor_case(oracle, p, Reta)
+        spike = prm.ce * 1.0 * 1e-9 * helem_l[0, a_src] ** 2
+        for a in range(T.nnode):
+            i, j, k = T.gI[a] - 1, T.gJ[a] - 1, T.gK[a] - 1
+            expect = spike * W[i, src[0]] * W[j, src[1]] * W[k, src[2]]
+            assert abs(mu_e[0, a] - expect) <= 1e-15 * spike, (src, a)
+
+
+def test_sensor_normalisation(oracle):
+    """flag_normalise_entropy divides the residual by max |eta - mean(eta)| over
+    the working nodes: with eta = 1 at one node and 0 elsewhere that is
+    1 - 1/npoin_w."""
+    p = 2
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, 1, False, lengths=(1, 1, 1))
+    npoin, nn = m.npoin, T.nnode
+    helem = oracle.char_length(p, m.connec_orig, m.coord)
+    helem_l = oracle.char_length_spectral(p, m.connec_orig, m.coord)
+    u = np.asfortranarray(np.tile([0.3, 0.4, 0.0], (npoin, 1)))
+    eta = np.zeros(npoin)
+    eta[5] = 1.0
+    Reta = np.full(npoin, 1e-9)
+    out = []
+    for flag in (0, 1):
+        prm = oracle.visc_params(287.0, flag_normalise_entropy=flag)
+        mu_e = np.zeros((m.nelem, nn), order="F")
+        oracle.smart_visc_spectral(T, m.connec, m.lpoin_w, Reta, 1.4, np.ones(npoin), u,
+                                   np.full(npoin, 2.0), eta, helem_l, helem, prm, mu_e)
+        out.append(mu_e)
+    scale = 1.0 / (1.0 - 1.0 / m.npoin_w)
+    assert np.abs(out[1] - scale * out[0]).max() <= 1e-14 * np.abs(out[1]).max()
+
+
+# ============================================================== dt and entropy
+def test_adapt_dt_uniform_state(oracle):
+    p = 3
+    m, T, gpvol, He, Ml, invMl = oracle_geometry(oracle, p, 2, False, lengths=(1, 1, 1))
+    npoin, nn = m.npoin, T.nnode
+    helem = oracle.char_length(p, m.connec_orig, m.coord)
+    u = np.asfortranarray(np.tile([0.3, -0.7, 0.1], (npoin, 1)))
+    c, rho, mu = 1.5, 1.2, 1e-3
+    z = np.zeros((m.nelem, nn), order="F")
+    dt = oracle.adapt_dt_cfl(p, m.connec, helem, u, np.full(npoin, c), 0.5, 0.4,
+                             np.full(npoin, mu), z, z, np.full(npoin, rho))
+    hh = helem[0] / (2 * p + 1)
+    # convective limit on max_i |u_i| + c, diffusive limit on (mu / rho)
+    assert abs(dt - min(0.5 * hh / (0.7 + c), 0.4 * hh * hh / (mu / rho))) < 1e-16
+
+
+def test_thermo_entropy_isentrope(oracle):
+    n = 11
+    r = np.random.default_rng(2)
+    rho = r.uniform(0.5, 2.0, n)
+    p_inf, rho_inf, g = 0.8, 1.1, 1.4
+    lw = np.arange(1, n + 1, dtype=np.int32)
+    u = np.asfortranarray(r.normal(0, 1, (n, 3)))
+    for fac, shift in ((1.0, 0.0), (2.0, math.log(2.0))):
+        pr = fac * p_inf * (rho / rho_inf) ** g
+        eta = np.zeros((n, 2), order="F")
+        feta = np.zeros((n, 3), order="F")
+        oracle.thermo_entropy(lw, g, p_inf, rho_inf, rho, pr, u, eta, feta)
+        assert np.abs(eta[:, 1] - rho / (g - 1) * shift).max() < 1e-14
+        # the flux is formed with the previous level's entropy (slot 1 before the call)
+        assert np.abs(feta).max() == 0.0
+
+
+# ============================================================== Runge-Kutta tables
+def _amplification(oracle, L, G, S, zs):
+    out = []
+    for z in zs:
+        lw = np.array([1], np.int32)
+        y0 = [np.ones(1), np.ones((1, 3), order="F"), np.ones(1)]
+        y = [a.copy(order="F") for a in y0]
+        R = lambda y: [z * y[0], np.asfortranarray(z * y[1]), z * y[2]]  # noqa: E731
+        oracle.rk_ls_update(lw, 0.0, 1.0, G[1, 0], *y0, *y, *R(y0))
+        for s in range(2, S + 1):
+            r = R(y)
+            oracle.rk_ls_update(lw, L[s, 0], L[s, s - 1], G[s, s - 1], *y0, *y, *r)
+        out.append(y[0][0])
+        assert y[1][0, 1] == y[0][0] and y[2][0] == y[0][0]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("stages,variant,order,tol", [(3, 1, 3, 1e-15), (4, 1, 3, 1e-15),
+                                                      (5, 1, 3, 2e-14), (5, 2, 3, 5e-7),
+                                                      (5, 3, 3, 5e-7), (5, 4, 3, 5e-7)])
+def test_rk_ls_stability_polynomial(oracle, stages, variant, order, tol):
+    try:
+        L, G = oracle.rk_ls_coeffs(stages, variant)
+    except ValueError:
+        pytest.skip("scheme not provided")
+    S = stages
+    # consistency: each stage is a convex combination of y0 and the previous stage
+    for s in range(2, S + 1):
+        assert abs(L[s, 0] + L[s, s - 1] - 1.0) < 1e-15
+    zs = np.linspace(-1.0, 1.0, S + 1)
+    V = np.vander(zs, S + 1, increasing=True)
+    coef = np.linalg.solve(V, _amplification(oracle, L, G, S, zs))
+    taylor = [1.0 / math.factorial(k) for k in range(order + 1)]
+    assert np.abs(coef[:order + 1] - taylor).max() < tol, coef
+
+
+def test_rk4_classical(oracle):
+    a, b, c = oracle.rk4_coeffs()
+    # the classical tableau: y' = lambda y gives 1 + z + z^2/2 + z^3/6 + z^4/24
+    for z in (-0.5, 0.1, 0.7):
+        k, s = [], 0.0
+        for i in range(4):
+            yi = 1.0 + (a[i] * z * k[-1] if i else 0.0)
+            k.append(yi)
+            s += b[i] * z * yi
+        assert abs(1.0 + s - sum(z ** j / math.factorial(j) for j in range(5))) < 1e-15
+    assert np.allclose(c, a)
