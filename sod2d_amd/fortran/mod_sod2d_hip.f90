@@ -1,0 +1,464 @@
+! mod_sod2d_hip.f90 -- ISO_C_BINDING layer over libsod2d_hip (include/sod2d_hip.h).
+!
+! The subroutines full_convec_ijk, full_diffusion_ijk, generic_scalar_convec_ijk
+! (elem_convec.f90:227-594, elem_diffu.f90:12-223), smart_visc_spectral
+! (mod_entropy_viscosity.f90:14-111) and lumped_solver_scal_opt /
+! lumped_solver_vect_opt (mod_solver.f90:40-104) keep the reference's dummy
+! argument lists (names, order, types, kinds, shapes, intents and optional
+! arguments), so a caller switches to the device path by using this module in
+! place of elem_convec / elem_diffu / mod_entropy_viscosity / mod_solver.
+! rk_stage_update is the stage combine of rk_4_ls_main
+! (time_integ_ls.f90:323-361), which the reference writes inline, with the
+! lumped-mass inverse fused in.
+!
+! Arrays are host arrays that have been mapped with sod2d_hip_enter_data (the
+! equivalent of OpenACC's enter data): each subroutine looks up their device
+! mirrors and runs on them; results stay on the device until
+! sod2d_hip_update_host.  A missing mapping or a failed launch stops the
+! program with the library's message, as the reference stops on errors.
+!
+! The element order is fixed at compile time (-D__PORDER__, as the reference).
+module mod_sod2d_hip
+   use iso_c_binding
+   implicit none
+   private
+
+   integer(4), parameter :: rp = 8
+   integer(4), parameter :: ndime = 3
+   integer(4), parameter :: porder = __PORDER__
+   integer(4), parameter :: nnode = (porder+1)**3
+   integer(4), parameter :: ngaus = nnode
+
+   public :: sod2d_hip_init, sod2d_hip_finalize, sod2d_hip_synchronize
+   public :: sod2d_hip_enter_data, sod2d_hip_update_device, sod2d_hip_update_host
+   public :: sod2d_hip_exit_data, sod2d_hip_set_visc_params
+   public :: full_convec_ijk, full_diffusion_ijk, generic_scalar_convec_ijk
+   public :: smart_visc_spectral, lumped_solver_scal_opt, lumped_solver_vect_opt
+   public :: rk_stage_update
+
+   type, bind(C) :: visc_params_t
+      real(c_double) :: ce, cmax, cglob, Rgas
+      integer(c_int) :: flag_normalise_entropy, flag_drop_c_in_envit
+   end type visc_params_t
+
+   type(c_ptr), save :: ctx = c_null_ptr
+   ! module globals the reference's sensor reads from mod_numerical_params,
+   ! with the reference's defaults (mod_numerical_params.f90:29-41)
+   type(visc_params_t), save :: vprm = visc_params_t(0.1d0, 0.5d0, 1.0d0, 1.0d0, 1, 0)
+   ! the reference's optional `fact` keeps the value of the last call that
+   ! passed it (a local with an initialiser is SAVE'd)
+   real(rp), save :: fact_convec = 1.0_rp, fact_diffu = 1.0_rp, fact_scalar = 1.0_rp
+   ! likewise fact_low_mach of smart_visc_spectral, once dropped, stays dropped
+   logical, save :: low_mach_dropped = .false.
+
+   interface sod2d_hip_enter_data
+      module procedure enter_r8, enter_i4
+   end interface
+   interface sod2d_hip_update_device
+      module procedure upd_dev_r8, upd_dev_i4
+   end interface
+   interface sod2d_hip_update_host
+      module procedure upd_host_r8, upd_host_i4
+   end interface
+   interface sod2d_hip_exit_data
+      module procedure exit_r8, exit_i4
+   end interface
+
+   interface
+      integer(c_int) function c_create(device, c) bind(C, name='sodhip_create')
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr) :: c
+      end function
+      integer(c_int) function c_destroy(c) bind(C, name='sodhip_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      integer(c_int) function c_synchronize(c) bind(C, name='sodhip_synchronize')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c
+      end function
+      integer(c_int) function c_enter_data(c, host, bytes, copyin) bind(C, name='sodhip_enter_data')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: c, host
+         integer(c_size_t), value :: bytes
+         integer(c_int), value :: copyin
+      end function
+      integer(c_int) function c_update_device(c, host, bytes) bind(C, name='sodhip_update_device')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: c, host
+         integer(c_size_t), value :: bytes
+      end function
+      integer(c_int) function c_update_host(c, host, bytes) bind(C, name='sodhip_update_host')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: c, host
+         integer(c_size_t), value :: bytes
+      end function
+      integer(c_int) function c_exit_data(c, host, copyout) bind(C, name='sodhip_exit_data')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, host
+         integer(c_int), value :: copyout
+      end function
+      type(c_ptr) function c_deviceptr(c, host) bind(C, name='sodhip_deviceptr')
+         import :: c_ptr
+         type(c_ptr), value :: c
+         type(*), dimension(*), intent(in) :: host
+      end function
+      integer(c_int) function c_last_error(buf, n) bind(C, name='sodhip_last_error')
+         import :: c_int, c_char
+         character(kind=c_char), dimension(*) :: buf
+         integer(c_int), value :: n
+      end function
+      integer(c_int) function c_full_convec(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
+            gJ, gK, u, q, rho, pr, E, Rmass, Rmom, Rener, init, fact) &
+            bind(C, name='sodhip_full_convec_ijk_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, u, q, rho, pr, E
+         type(c_ptr), value :: Rmass, Rmom, Rener
+         integer(c_int), value :: p, nelem, npoin, init
+         real(c_double), value :: fact
+      end function
+      integer(c_int) function c_full_diffusion(c, p, nelem, npoin, connec, He, gpvol, dl, inv, &
+            gI, gJ, gK, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs, c_rho, c_ener, &
+            Rmass, Rmom, Rener, init, fact) bind(C, name='sodhip_full_diffusion_ijk_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, rho_n, rho, u, Tem
+         type(c_ptr), value :: mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener
+         integer(c_int), value :: p, nelem, npoin, init
+         real(c_double), value :: Cp, Pr, c_rho, c_ener, fact
+      end function
+      integer(c_int) function c_scalar_convec(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
+            gJ, gK, q, eta, u, Rconvec, init, fact) &
+            bind(C, name='sodhip_generic_scalar_convec_ijk_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, q, eta, u, Rconvec
+         integer(c_int), value :: p, nelem, npoin, init
+         real(c_double), value :: fact
+      end function
+      integer(c_int) function c_smart_visc(c, p, nelem, npoin, npoin_w, connec, lpoin_w, Reta, &
+            gamma_gas, rho, u, Tem, eta, helem, helem_k, inv, prm, mu_e, mue_l) &
+            bind(C, name='sodhip_smart_visc_spectral_f64')
+         import :: c_int, c_ptr, c_double, visc_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, Reta, rho, u, Tem, eta, helem, helem_k, inv
+         type(c_ptr), value :: mu_e, mue_l
+         integer(c_int), value :: p, nelem, npoin, npoin_w
+         real(c_double), value :: gamma_gas
+         type(visc_params_t), intent(in) :: prm
+      end function
+      integer(c_int) function c_lumped_scal(c, npoin, npoin_w, lpoin_w, invMl, R) &
+            bind(C, name='sodhip_lumped_solver_scal_f64')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, lpoin_w, invMl, R
+         integer(c_int), value :: npoin, npoin_w
+      end function
+      integer(c_int) function c_lumped_vect(c, npoin, npoin_w, lpoin_w, invMl, R) &
+            bind(C, name='sodhip_lumped_solver_vect_f64')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, lpoin_w, invMl, R
+         integer(c_int), value :: npoin, npoin_w
+      end function
+      integer(c_int) function c_rk_update(c, npoin, npoin_w, lpoin_w, invMl, dt, lam1, lams, gam, &
+            rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener) &
+            bind(C, name='sodhip_rk_stage_update_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, lpoin_w, invMl, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener
+         integer(c_int), value :: npoin, npoin_w
+         real(c_double), value :: dt, lam1, lams, gam
+      end function
+   end interface
+
+contains
+
+   ! ------------------------------------------------------------ context
+   subroutine sod2d_hip_init(device)
+      integer(4), intent(in), optional :: device
+      integer(c_int) :: dev
+      dev = 0
+      if (present(device)) dev = device
+      if (c_associated(ctx)) return
+      call check(c_create(dev, ctx), 'sodhip_create')
+   end subroutine sod2d_hip_init
+
+   subroutine sod2d_hip_finalize()
+      if (.not. c_associated(ctx)) return
+      call check(c_destroy(ctx), 'sodhip_destroy')
+      ctx = c_null_ptr
+   end subroutine sod2d_hip_finalize
+
+   subroutine sod2d_hip_synchronize()
+      call check(c_synchronize(ctx), 'sodhip_synchronize')
+   end subroutine sod2d_hip_synchronize
+
+   ! Module globals of mod_numerical_params read by smart_visc_spectral:
+   ! ce, cmax, cglob, nscbc_Rgas_inf, flag_normalise_entropy, flag_drop_c_in_envit
+   subroutine sod2d_hip_set_visc_params(ce, cmax, cglob, Rgas, flag_normalise_entropy, &
+         flag_drop_c_in_envit)
+      real(rp), intent(in) :: ce, cmax, cglob, Rgas
+      integer(4), intent(in) :: flag_normalise_entropy
+      logical, intent(in) :: flag_drop_c_in_envit
+      vprm%ce = ce
+      vprm%cmax = cmax
+      vprm%cglob = cglob
+      vprm%Rgas = Rgas
+      vprm%flag_normalise_entropy = flag_normalise_entropy
+      if (flag_drop_c_in_envit) low_mach_dropped = .true.
+   end subroutine sod2d_hip_set_visc_params
+
+   ! ------------------------------------------------------------ present table
+   subroutine enter_r8(a, copyin)
+      real(c_double), dimension(..), intent(in), target :: a
+      logical, intent(in), optional :: copyin
+      integer(c_int) :: ci
+      ci = 1
+      if (present(copyin)) then
+         if (.not. copyin) ci = 0
+      end if
+      call check(c_enter_data(ctx, c_loc(a), size(a, kind=c_size_t) * 8_c_size_t, ci), &
+                 'sodhip_enter_data')
+   end subroutine enter_r8
+
+   subroutine enter_i4(a, copyin)
+      integer(c_int), dimension(..), intent(in), target :: a
+      logical, intent(in), optional :: copyin
+      integer(c_int) :: ci
+      ci = 1
+      if (present(copyin)) then
+         if (.not. copyin) ci = 0
+      end if
+      call check(c_enter_data(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t, ci), &
+                 'sodhip_enter_data')
+   end subroutine enter_i4
+
+   subroutine upd_dev_r8(a)
+      real(c_double), dimension(..), intent(in), target :: a
+      call check(c_update_device(ctx, c_loc(a), size(a, kind=c_size_t) * 8_c_size_t), &
+                 'sodhip_update_device')
+   end subroutine upd_dev_r8
+
+   subroutine upd_dev_i4(a)
+      integer(c_int), dimension(..), intent(in), target :: a
+      call check(c_update_device(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t), &
+                 'sodhip_update_device')
+   end subroutine upd_dev_i4
+
+   subroutine upd_host_r8(a)
+      real(c_double), dimension(..), intent(inout), target :: a
+      call check(c_update_host(ctx, c_loc(a), size(a, kind=c_size_t) * 8_c_size_t), &
+                 'sodhip_update_host')
+   end subroutine upd_host_r8
+
+   subroutine upd_host_i4(a)
+      integer(c_int), dimension(..), intent(inout), target :: a
+      call check(c_update_host(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t), &
+                 'sodhip_update_host')
+   end subroutine upd_host_i4
+
+   subroutine exit_r8(a, copyout)
+      real(c_double), dimension(..), intent(inout), target :: a
+      logical, intent(in), optional :: copyout
+      integer(c_int) :: co
+      co = 0
+      if (present(copyout)) then
+         if (copyout) co = 1
+      end if
+      call check(c_exit_data(ctx, c_loc(a), co), 'sodhip_exit_data')
+   end subroutine exit_r8
+
+   subroutine exit_i4(a, copyout)
+      integer(c_int), dimension(..), intent(inout), target :: a
+      logical, intent(in), optional :: copyout
+      integer(c_int) :: co
+      co = 0
+      if (present(copyout)) then
+         if (copyout) co = 1
+      end if
+      call check(c_exit_data(ctx, c_loc(a), co), 'sodhip_exit_data')
+   end subroutine exit_i4
+
+   ! ------------------------------------------------------------ helpers
+   subroutine check(rc, what)
+      integer(c_int), intent(in) :: rc
+      character(*), intent(in) :: what
+      character(kind=c_char) :: buf(512)
+      integer(c_int) :: n, i
+      character(len=512) :: msg
+      if (rc == 0) return
+      n = c_last_error(buf, 512)
+      msg = ' '
+      do i = 1, 512
+         if (buf(i) == c_null_char) exit
+         msg(i:i) = buf(i)
+      end do
+      write(*,'(a,a,a,i0,a,a)') 'mod_sod2d_hip: ', what, ' failed (', rc, '): ', trim(msg)
+      error stop 1
+   end subroutine check
+
+   type(c_ptr) function dev(host, what)
+      type(*), dimension(*), intent(in) :: host
+      character(*), intent(in) :: what
+      if (.not. c_associated(ctx)) then
+         write(*,'(a)') 'mod_sod2d_hip: sod2d_hip_init was not called'
+         error stop 1
+      end if
+      dev = c_deviceptr(ctx, host)
+      if (.not. c_associated(dev)) then
+         write(*,'(a,a,a)') 'mod_sod2d_hip: argument ', what, ' is not mapped to the device'
+         error stop 1
+      end if
+   end function dev
+
+   integer(c_int) function init_flag(initialze)
+      logical, optional, intent(in) :: initialze
+      init_flag = 1
+      if (present(initialze)) then
+         if (.not. initialze) init_flag = 0
+      end if
+   end function init_flag
+
+   ! ------------------------------------------------------------ operators
+   ! elem_convec.f90:227-470.  Ngp, dNgp and xgp are part of the reference's
+   ! interface and are not read.
+   subroutine full_convec_ijk(nelem,npoin,connec,Ngp,dNgp,He,gpvol,dlxigp_ip,xgp,invAtoIJK, &
+         gmshAtoI,gmshAtoJ,gmshAtoK,u,q,rho,pr,E,Rmass,Rmom,Rener,initialze,fact)
+      integer(4), intent(in)  :: nelem, npoin
+      integer(4), intent(in)  :: connec(nelem,nnode)
+      real(rp),    intent(in)  :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
+      real(rp),    intent(in)  :: He(ndime,ndime,ngaus,nelem),xgp(ngaus,ndime)
+      real(rp),    intent(in)  :: dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),    intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),    intent(in)  :: q(npoin,ndime), u(npoin,ndime), rho(npoin),pr(npoin), E(npoin)
+      real(rp),    intent(inout) :: Rmass(npoin)
+      real(rp),    intent(inout) :: Rmom(npoin,ndime)
+      real(rp),    intent(inout) :: Rener(npoin)
+      logical, optional, intent(in)    :: initialze
+      real(rp), optional, intent(in)  :: fact
+      if (present(fact)) fact_convec = fact
+      call check(c_full_convec(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(He,'He'), &
+         dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), &
+         dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), &
+         dev(u,'u'), dev(q,'q'), dev(rho,'rho'), dev(pr,'pr'), dev(E,'E'), dev(Rmass,'Rmass'), &
+         dev(Rmom,'Rmom'), dev(Rener,'Rener'), init_flag(initialze), fact_convec), &
+         'full_convec_ijk')
+   end subroutine full_convec_ijk
+
+   ! elem_diffu.f90:12-223.  Ngp and Ml are part of the reference's interface
+   ! and are not read; c_rho and c_ener take the reference's defaults (1).
+   subroutine full_diffusion_ijk(nelem,npoin,connec,Ngp,He,gpvol,dlxigp_ip,invAtoIJK,gmshAtoI, &
+         gmshAtoJ,gmshAtoK,Cp,Pr,rho_n,rho,u,Tem,mu_fluid,mu_e,mu_sgs,Ml,Rmass,Rmom,Rener, &
+         initialze,fact)
+      integer(4), intent(in)  :: nelem, npoin
+      integer(4), intent(in)  :: connec(nelem,nnode)
+      real(rp),   intent(in)  :: Ngp(ngaus,nnode)
+      real(rp),   intent(in)  :: He(ndime,ndime,ngaus,nelem),dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)  :: Cp,Pr,rho_n(npoin),rho(npoin),u(npoin,ndime), Tem(npoin)
+      real(rp),   intent(in)  :: mu_e(nelem,ngaus), mu_sgs(nelem,ngaus),Ml(npoin)
+      real(rp),   intent(in)  :: mu_fluid(npoin)
+      real(rp),   intent(inout) :: Rmass(npoin)
+      real(rp),   intent(inout) :: Rmom(npoin,ndime)
+      real(rp),   intent(inout) :: Rener(npoin)
+      logical, optional, intent(in)    :: initialze
+      real(rp), optional, intent(in)  :: fact
+      if (present(fact)) fact_diffu = fact
+      call check(c_full_diffusion(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(He,'He'), &
+         dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), &
+         dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), Cp, Pr, &
+         dev(rho_n,'rho_n'), dev(rho,'rho'), dev(u,'u'), dev(Tem,'Tem'), &
+         dev(mu_fluid,'mu_fluid'), dev(mu_e,'mu_e'), dev(mu_sgs,'mu_sgs'), 1.0_rp, 1.0_rp, &
+         dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener'), init_flag(initialze), &
+         fact_diffu), 'full_diffusion_ijk')
+   end subroutine full_diffusion_ijk
+
+   ! elem_convec.f90:472-594
+   subroutine generic_scalar_convec_ijk(nelem,npoin,connec,Ngp,dNgp,He,gpvol,dlxigp_ip,xgp, &
+         invAtoIJK,gmshAtoI,gmshAtoJ,gmshAtoK,q,eta,u,Rconvec,initialze,fact)
+      integer(4), intent(in)  :: nelem, npoin
+      integer(4), intent(in)  :: connec(nelem,nnode)
+      real(rp),    intent(in)  :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
+      real(rp),    intent(in)  :: He(ndime,ndime,ngaus,nelem),xgp(ngaus,ndime)
+      real(rp),    intent(in)  :: dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),    intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),    intent(in)  :: q(npoin,ndime)
+      real(rp),    intent(in)  :: eta(npoin)
+      real(rp),    intent(in)  :: u(npoin,ndime)
+      real(rp),    intent(inout) :: Rconvec(npoin)
+      logical, optional, intent(in)    :: initialze
+      real(rp), optional, intent(in)  :: fact
+      if (present(fact)) fact_scalar = fact
+      call check(c_scalar_convec(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(He,'He'), &
+         dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), &
+         dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), &
+         dev(q,'q'), dev(eta,'eta'), dev(u,'u'), dev(Rconvec,'Rconvec'), init_flag(initialze), &
+         fact_scalar), 'generic_scalar_convec_ijk')
+   end subroutine generic_scalar_convec_ijk
+
+   ! mod_entropy_viscosity.f90:14-111.  Ngp, coord, dNgp, gpvol, wgp, csound,
+   ! Ml and gmshAtoI/J/K are part of the reference's interface and are not read.
+   subroutine smart_visc_spectral(nelem,npoin,npoin_w,connec,lpoin_w,Reta,Ngp,coord,dNgp,gpvol, &
+         wgp,gamma_gas,rho,u,csound,Tem,eta,helem,helem_k,Ml,mu_e,invAtoIJK,gmshAtoI,gmshAtoJ, &
+         gmshAtoK,mue_l)
+      integer(4), intent(in)  :: nelem, npoin,npoin_w, connec(nelem,nnode),lpoin_w(npoin_w)
+      real(rp),   intent(in)  :: Reta(npoin), Ngp(ngaus,nnode),gamma_gas
+      real(rp),   intent(in)  :: rho(npoin), u(npoin,ndime),csound(npoin), Tem(npoin), eta(npoin)
+      real(rp),   intent(in)  :: helem(nelem,nnode),helem_k(nelem),Ml(npoin)
+      real(rp),   intent(out) :: mu_e(nelem,ngaus)
+      real(rp),   intent(in)  :: coord(npoin,ndime), dNgp(ndime,nnode,ngaus), wgp(ngaus)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode),gmshAtoK(nnode)
+      real(rp),intent(inout)  :: mue_l(nelem,nnode)
+      type(visc_params_t) :: prm
+      prm = vprm
+      prm%flag_drop_c_in_envit = merge(1, 0, low_mach_dropped)
+      call check(c_smart_visc(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+         dev(lpoin_w,'lpoin_w'), dev(Reta,'Reta'), gamma_gas, dev(rho,'rho'), dev(u,'u'), &
+         dev(Tem,'Tem'), dev(eta,'eta'), dev(helem,'helem'), dev(helem_k,'helem_k'), &
+         dev(invAtoIJK,'invAtoIJK'), prm, dev(mu_e,'mu_e'), dev(mue_l,'mue_l')), &
+         'smart_visc_spectral')
+   end subroutine smart_visc_spectral
+
+   ! mod_solver.f90:40-59
+   subroutine lumped_solver_scal_opt(npoin,npoin_w,lpoin_w,invMl,R)
+      integer(4), intent(in)    :: npoin, npoin_w, lpoin_w(npoin_w)
+      real(rp),    intent(in)    :: invMl(npoin)
+      real(rp),    intent(inout) :: R(npoin)
+      call check(c_lumped_scal(ctx, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), dev(invMl,'invMl'), &
+         dev(R,'R')), 'lumped_solver_scal_opt')
+   end subroutine lumped_solver_scal_opt
+
+   ! mod_solver.f90:81-104
+   subroutine lumped_solver_vect_opt(npoin,npoin_w,lpoin_w,invMl,R)
+      integer(4), intent(in)    :: npoin, npoin_w, lpoin_w(npoin_w)
+      real(rp),    intent(in)    :: invMl(npoin)
+      real(rp),    intent(inout) :: R(npoin,ndime)
+      call check(c_lumped_vect(ctx, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), dev(invMl,'invMl'), &
+         dev(R,'R')), 'lumped_solver_vect_opt')
+   end subroutine lumped_solver_vect_opt
+
+   ! Stage combine of rk_4_ls_main (time_integ_ls.f90:323-361) over working
+   ! nodes with the lumped solve fused in: R <- R*invMl (kept, as the in-place
+   ! lumped solve of the reference), then y <- lam1*y1 + lams*y + gam*dt*R for
+   ! rho, E and q.  With invMl absent R already carries the mass inverse.
+   subroutine rk_stage_update(npoin,npoin_w,lpoin_w,dt,lam1,lams,gam,rho1,q1,E1,rho,q,E, &
+         Rmass,Rmom,Rener,invMl)
+      integer(4), intent(in)    :: npoin, npoin_w, lpoin_w(npoin_w)
+      real(rp),   intent(in)    :: dt, lam1, lams, gam
+      real(rp),   intent(in)    :: rho1(npoin), q1(npoin,ndime), E1(npoin)
+      real(rp),   intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(rp),   intent(inout) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp),   intent(in), optional :: invMl(npoin)
+      type(c_ptr) :: pinv
+      pinv = c_null_ptr
+      if (present(invMl)) pinv = dev(invMl,'invMl')
+      call check(c_rk_update(ctx, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), pinv, dt, lam1, lams, &
+         gam, dev(rho1,'rho1'), dev(q1,'q1'), dev(E1,'E1'), dev(rho,'rho'), dev(q,'q'), &
+         dev(E,'E'), dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener')), 'rk_stage_update')
+   end subroutine rk_stage_update
+
+end module mod_sod2d_hip
