@@ -44,15 +44,18 @@ struct DevBuf {
 // Node-ordered incidence of a connectivity: node n (0-based) owns the slots
 // row[n]..row[n+1]-1, one per (e,a) with connec(e,a) == n+1, in ascending (e,a)
 // order -- the order in which the reference's serial element loop adds element
-// contributions.  slot[e*nnode+a] is the slot of (e,a): element kernels write
-// their per element-node residuals there, so that the per-node gather reads
-// contiguous memory.
+// contributions.  Element kernels write the residual of (e,a) to the scratch
+// position e*nnode + perm[a] (whole elements, so the writes are coalesced;
+// perm groups an element's nodes by their (k, j, i) line indices so that the
+// four cache lines of an element hold one k-plane each); src[slot] is that
+// position, read by the per-node gathers in slot order.
 struct MeshCache {
     const int *connec_key = nullptr; // device pointer of the caller's connec
     int porder = 0, nelem = 0, npoin = 0;
     int *connecT = nullptr; // (nelem, nnode) element-major, 0-based node ids
     int *row = nullptr;     // npoin+1
-    int *slot = nullptr;    // nelem*nnode
+    int *src = nullptr;     // nelem*nnode, scratch position of each slot
+    int *perm = nullptr;    // nnode, element-local scratch position of node a
 };
 
 struct PresentEntry {

@@ -12,9 +12,10 @@
 //  * blocks walk the elements in a grid-stride loop whose block -> element map
 //    gives each XCD (blockIdx % 8) one contiguous eighth of the elements, so
 //    nodes shared by neighbouring elements are re-read from the same L2;
-//  * element residuals go to a per element-node scratch array (SoA); a second
-//    kernel gathers them per node through the node-ordered incidence table in
-//    the reference's serial (element, node) order -- deterministic, no atomics.
+//  * element residuals go to a per element-node scratch array (SoA, whole
+//    elements: coalesced writes); a second kernel gathers them per node through
+//    the node-ordered incidence table in the reference's serial (element, node)
+//    order -- deterministic, no atomics.
 //
 // Arithmetic restatement: the split forms are evaluated through identities that
 // hold for arbitrary nodal inputs (linearity of the discrete derivative and
@@ -26,8 +27,17 @@
 //  * flux divergences sum_j d_j(s u_j) are evaluated from the line values of s
 //    times the contravariant velocity sum_j He_g(j,k) u_j(m) at the line node m.
 #include <cstdio>
+#include <type_traits>
 
 #include "common.hpp"
+#include "device.hpp"
+
+#ifndef SODHIP_SPLIT_WAVES
+#define SODHIP_SPLIT_WAVES 2
+#endif
+#ifndef SODHIP_SPLIT_RHS
+#define SODHIP_SPLIT_RHS 1
+#endif
 
 namespace sodhip {
 
@@ -40,8 +50,9 @@ template <int P> struct EG {
 // Per-lane line tables: the P other nodes of the lane's three GLL lines.
 template <int P, typename T> struct LineTab {
     int off[3][P];
-    int own[3];       // the lane's position along each line (0-based)
     T ds[3], d[3][P]; // d/dxi_k at the lane's point: own and other coefficients
+    T t[3][P];        // transposed: d/dxi_k of the lane's basis function at the
+                      // other nodes (weak divergence)
 };
 
 template <int P, typename T>
@@ -58,7 +69,6 @@ __device__ __forceinline__ void build_linetab(int a, const T *__restrict__ dl,
                         min(max(gK[a] - 1, 0), P)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        t.own[k] = own[k];
         t.ds[k] = dl[a + NN * (k + 3 * own[k])];
 #pragma unroll
         for (int o = 0; o < P; ++o) {
@@ -68,6 +78,31 @@ __device__ __forceinline__ void build_linetab(int a, const T *__restrict__ dl,
             const int b = min(max(inv[idx[0] + N1 * (idx[1] + N1 * idx[2])] - 1, 0), NN - 1);
             t.off[k][o] = b;
             t.d[k][o] = dl[a + NN * (k + 3 * m)];
+            t.t[k][o] = dl[b + NN * (k + 3 * own[k])];
+        }
+    }
+}
+
+// Copies the geometry block of one element (HEB bytes of He, then GVB bytes of
+// gpvol) HBM -> LDS with the direct-to-LDS loads of gfx950: one wave
+// instruction moves a row of 64 lanes x W bytes; the NW waves of the
+// workgroup take alternate rows.  Lanes past the end re-read the last W bytes.
+template <int W, int HEB, int GVB, int NW>
+__device__ __forceinline__ void geo_copy(const unsigned char *he, const unsigned char *gv,
+                                         unsigned char *geo, int wl, int wv)
+{
+    constexpr int R = 64 * W, ROWS = (HEB + GVB + R - 1) / R;
+#pragma unroll
+    for (int r0 = 0; r0 < ROWS; r0 += NW) {
+        const int r = r0 + wv;
+        if (r < ROWS) {
+            const int byte = r * R + wl * W;
+            const unsigned char *src = byte < HEB ? he + byte : gv + min(byte - HEB, GVB - W);
+            auto *dst = (__attribute__((address_space(3))) void *)(geo + r * R);
+            if constexpr (W == 16)
+                __builtin_amdgcn_global_load_lds((const void *)src, dst, 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds((const void *)src, dst, 4, 0, 0);
         }
     }
 }
@@ -100,8 +135,9 @@ template <typename T> struct RhsArgs {
     // conserved inputs (fused stage): FP64 state
     const double *cr, *cq, *cE;
     double gamma, Rgas, mu_factor, p_ref;
-    const int *slot; // node-ordered output slot of each (element, node)
-    T *out;          // SoA [5][EN]: mass, mom1, mom2, mom3, energy, in slot order
+    const int *perm; // element-local scratch position of each node
+    T *out;          // SoA [5][EN]: mass, mom1, mom2, mom3, energy; (e,a) at e*NN + perm[a]
+    int geo16;       // He and gpvol blocks of every element are 16-byte aligned
 };
 
 // Sutherland's law (mod_fluid_viscosity.f90:21-41)
@@ -126,7 +162,7 @@ template <typename T> __device__ __forceinline__ T sutherland(T mu_factor, T Tk)
 //           G_k(F) = gpvol sum_d He(d,k) F_d, staged in LDS;
 //   diff 2  weak divergence sum_k sum_{b on k-line} dl(b,k,I_k(a)) G_k(F)(b)
 //           with the transposed coefficients kept per lane in LDS;
-//   out     residuals written to the node-ordered slots of the incidence table.
+//   out     residuals written to the element's scratch block.
 template <int P, typename T, bool CONV, bool DIFF, bool CONS>
 __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
 {
@@ -136,17 +172,17 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
     constexpr int F_P = F_C, F_H = F_C + 1, F_RH = F_C + 2, F_Q = F_C + 3;
     constexpr int NF1 = CONV ? F_C + 6 : F_C;
     constexpr int NF2 = DIFF ? 18 : 0;
-    // the viscous fluxes (pass 2) reuse the nodal-field area: one LDS region
-    // of max(NF1, NF2) fields keeps the block small enough for several blocks
-    // per CU
-    __shared__ T s1[(NF1 > NF2 ? NF1 : NF2) * NT];
-    T *const s2 = s1;
+    // nodal fields (s1) and contravariant viscous fluxes (s2): the fluxes are
+    // staged while the convective lines still read the nodal fields
+    __shared__ T s1[NF1 * NT];
+    __shared__ T s2[(NF2 > 0 ? NF2 : 1) * NT];
 
     const int lane = threadIdx.x;
     const bool act = lane < NN;
     const int a = act ? lane : 0;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1); // scratch position within the element
     // line derivative d/dxi_k of LDS field f at the lane's point
     auto dxi = [&](int f, int k, T own) {
         T v = lt.ds[k] * own;
@@ -162,7 +198,6 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
         double cr, cq[3], cE;                // conserved state (CONS)
         T rho, u[3], q[3], pr, E, Tem, rhon, muf; // primitive inputs (drop-in)
         T mue, musgs;
-        int slot;
     };
     auto load = [&](In &x, int e, int n) {
         const size_t ea = (size_t)e * NN + a;
@@ -190,8 +225,31 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
             }
         }
         if constexpr (DIFF) x.mue = A.mu_e[e + (size_t)A.nelem * a];
-        x.slot = A.slot[ea];
     };
+
+    // Geometry of one element (He, then gpvol) is copied HBM -> LDS by the
+    // direct-to-LDS loads of gfx950 (no registers involved), one element ahead:
+    // the copy of element e+stride runs during the arithmetic of element e.  A
+    // wave instruction copies one row of 64 lanes x W bytes (W = 16 when the
+    // per-element blocks are 16-byte aligned, else 4); the waves of the
+    // workgroup take alternate rows.
+    constexpr int HE_BYTES = 9 * NN * (int)sizeof(T), GV_BYTES = NN * (int)sizeof(T);
+    constexpr int NW = NT / 64;
+    constexpr int ROWS16 = (HE_BYTES + GV_BYTES + 1023) / 1024;
+    constexpr int ROWS4 = (HE_BYTES + GV_BYTES + 255) / 256;
+    __shared__ __attribute__((aligned(16))) unsigned char
+        geo[(ROWS16 * 1024 > ROWS4 * 256 ? ROWS16 * 1024 : ROWS4 * 256)];
+    const int wl = lane & 63, wv = lane >> 6;
+    auto geo_issue = [&](int e) {
+        const unsigned char *he = (const unsigned char *)(A.He + (size_t)e * 9 * NN);
+        const unsigned char *gv = (const unsigned char *)(A.gpvol + (size_t)e * NN);
+        if (A.geo16)
+            geo_copy<16, HE_BYTES, GV_BYTES, NW>(he, gv, geo, wl, wv);
+        else
+            geo_copy<4, HE_BYTES, GV_BYTES, NW>(he, gv, geo, wl, wv);
+    };
+    auto esync = [&]() { lds_barrier(); };
+    constexpr int WAIT_VM0 = kWaitVm0, WAIT_LGKM0 = kWaitLgkm0;
 
     const ElemRange er = elem_range(A.nelem);
     if (er.begin >= er.end) return;
@@ -199,10 +257,27 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
     // the extra loads are harmless and keep the load stream unconditional
     const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
     In cur, nxt;
+    geo_issue(er.begin);
     load(cur, er.begin, A.connecT[(size_t)er.begin * NN + a]);
     int n_nx = A.connecT[(size_t)min(er.begin + er.stride, elast) * NN + a];
     for (int e = er.begin; e < er.end; e += er.stride) {
         const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
+        // geometry of element e: wait for its copy (and the state loads issued
+        // with it), move it to registers, then start the copies of element e2
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+        esync();
+        T He[3][3];
+        {
+            const T *hp = (const T *)geo + 9 * a;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
+        }
+        const T vol = ((const T *)geo)[9 * NN + a];
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+        esync();
+        geo_issue(e2);
         load(nxt, e2, n_nx);
         const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
         T mue = T(0), musgs = T(0), muf = T(0), rhon = T(0);
@@ -270,16 +345,7 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
                 }
             }
         }
-        T He[3][3];
-        {
-            const T *hp = A.He + 9 * ((size_t)e * NN + a);
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
-        }
-        const T vol = A.gpvol[(size_t)e * NN + a];
-        __syncthreads();
+        esync();
 
         T Rm = 0, Rq[3] = {0, 0, 0}, Re = 0;
         T gR[3] = {0, 0, 0};                           // DIFF: grad rho
@@ -349,46 +415,96 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
                 for (int i = 0; i < 3; ++i) Rq[i] = T(0.25) * (Rq[i] + rho * divFuu[i]);
             }
         }
-        // ---------------- scalar and momentum lines, one field per (rolled)
-        // iteration: line derivative d (and, for the convection, the W-weighted
-        // line sum `div` = div(s u)) folded at once into the residuals
-        constexpr int NLF = CONV ? 7 : 1; // rho[, p, H, rhoH, q1, q2, q3]
-        T divQ = 0;
-#pragma unroll 1
-        for (int fi = 0; fi < NLF; ++fi) {
-            const int f = fi == 0 ? F_RHO : (fi == 1 ? F_P : (fi == 2 ? F_H : (fi == 3 ? F_RH : F_Q + fi - 4)));
-            const T *sf = s1 + f * NT;
-            const T own = sf[a];
-            T d[3], div = 0;
+        // ---------------- density line: grad rho (viscous) and u.grad rho terms
+        // (convection)
+        {
+            const T *sf = s1 + F_RHO * NT;
+            T d[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                T v = lt.ds[k] * own, w = T(0);
-                if constexpr (CONV) w = Wo[k] * own;
+                T v = lt.ds[k] * rho;
 #pragma unroll
-                for (int o = 0; o < P; ++o) {
-                    const T x = sf[lt.off[k][o]];
-                    v += lt.d[k][o] * x;
-                    if constexpr (CONV) w += Wx[k][o] * x;
-                }
+                for (int o = 0; o < P; ++o) v += lt.d[k][o] * sf[lt.off[k][o]];
                 d[k] = v;
-                div += w;
             }
-            if (fi == 0) {
-                if constexpr (DIFF) {
+            if constexpr (DIFF) {
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) gR[i] = He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                for (int i = 0; i < 3; ++i) gR[i] = He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+            }
+            if constexpr (CONV) {
+                T Ut[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
+                const T urho = Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2];
+                Rm += T(0.5) * urho;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * u[i] * urho;
+                Re += T(0.25) * s1[F_H * NT + a] * urho;
+            }
+        }
+
+        // ---------------- viscous fluxes, staged as contravariant components
+        // G_k(F) = gpvol sum_d He(d,k) F_d for the weak divergence of pass 2
+        if constexpr (DIFF) {
+            const T mufg = muf + rhon * musgs;
+            const T mut = mufg + mue;
+            const T divU = gU[0][0] + gU[1][1] + gU[2][2];
+            const T tt = mufg * T(2.0 / 3.0) * divU;
+            T tau[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tau[i][i] -= tt;
+            T dT[3];
+            const T Tk = s1[F_T * NT + a];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dT[k] = dxi(F_T, k, Tk);
+            T gt[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
+                T tuk = 0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const T g = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
+                    if (act) s2[(9 + 3 * r + k) * NT + a] = g;
+                    tuk += g * u[r]; // tau is symmetric: contravariant (tau u)_k
                 }
-                if constexpr (CONV) {
-                    T Ut[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
-                    const T urho = Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2];
-                    Rm += T(0.5) * urho;
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * u[i] * urho;
-                    Re += T(0.25) * s1[F_H * NT + a] * urho;
+                if (act) {
+                    s2[(6 + k) * NT + a] = tuk;
+                    s2[(0 + k) * NT + a] = h0 * gR[0] + h1 * gR[1] + h2 * gR[2];
+                    s2[(3 + k) * NT + a] = h0 * gt[0] + h1 * gt[1] + h2 * gt[2];
                 }
-            } else if constexpr (CONV) {
+            }
+        }
+
+        // ---------------- convective scalar and momentum lines, one field per
+        // (rolled) iteration: line derivative d and the W-weighted line sum
+        // `div` = div(s u) folded at once into the residuals
+        T divQ = 0;
+        if constexpr (CONV) {
+#pragma unroll 1
+            for (int fi = 1; fi < 7; ++fi) { // p, H, rhoH, q1, q2, q3
+                const int f = fi == 1 ? F_P : (fi == 2 ? F_H : (fi == 3 ? F_RH : F_Q + fi - 4));
+                const T *sf = s1 + f * NT;
+                const T own = sf[a];
+                T d[3], div = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    T v = lt.ds[k] * own, w = Wo[k] * own;
+#pragma unroll
+                    for (int o = 0; o < P; ++o) {
+                        const T x = sf[lt.off[k][o]];
+                        v += lt.d[k][o] * x;
+                        w += Wx[k][o] * x;
+                    }
+                    d[k] = v;
+                    div += w;
+                }
                 if (fi == 1) {
 #pragma unroll
                     for (int i = 0; i < 3; ++i) Rq[i] += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
@@ -431,56 +547,11 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
             for (int i = 0; i < 3; ++i) Rq[i] *= vol;
         }
 
-        // ---------------- viscous operator
+        // ---------------- viscous operator, pass 2
         if constexpr (DIFF) {
-            T G[18]; // contravariant viscous fluxes G_k(F) = gpvol sum_d He(d,k) F_d
-            {
-                const T mufg = muf + rhon * musgs;
-                const T mut = mufg + mue;
-                const T divU = gU[0][0] + gU[1][1] + gU[2][2];
-                const T tt = mufg * T(2.0 / 3.0) * divU;
-                T tau[3][3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
-#pragma unroll
-                for (int i = 0; i < 3; ++i) tau[i][i] -= tt;
-                T dT[3];
-                const T Tk = s1[F_T * NT + a];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) dT[k] = dxi(F_T, k, Tk);
-                T gt[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
-                    T tuk = 0;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const T g = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
-                        G[9 + 3 * r + k] = g;
-                        tuk += g * u[r]; // tau is symmetric: contravariant (tau u)_k
-                    }
-                    G[6 + k] = tuk;
-                    G[0 + k] = h0 * gR[0] + h1 * gR[1] + h2 * gR[2];
-                    G[3 + k] = h0 * gt[0] + h1 * gt[1] + h2 * gt[2];
-                }
-            }
-            __syncthreads(); // every lane is done with the nodal fields
-            if (act) {
-#pragma unroll
-                for (int f = 0; f < 18; ++f) s2[f * NT + a] = G[f];
-            }
-            __syncthreads();
+            esync(); // every lane's fluxes are in s2
             // pass 2: weak divergence with the transposed coefficients
-            // dl(b,k,I_k(a)) of the lane's lines (cache-resident table)
-            T cT[3][P];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int o = 0; o < P; ++o) cT[k][o] = A.dl[lt.off[k][o] + NN * (k + 3 * lt.own[k])];
+            // dl(b,k,I_k(a)) of the lane's lines
             T W6[6];
 #pragma unroll 1
             for (int fl = 0; fl < 6; ++fl) {
@@ -490,7 +561,7 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
                     const T *sf = s2 + (3 * fl + k) * NT;
                     T v = lt.ds[k] * sf[a];
 #pragma unroll
-                    for (int o = 0; o < P; ++o) v += cT[k][o] * sf[lt.off[k][o]];
+                    for (int o = 0; o < P; ++o) v += lt.t[k][o] * sf[lt.off[k][o]];
                     acc += v;
                 }
                 if (fl == 0) W6[0] = acc;
@@ -510,7 +581,7 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
         }
 
         if (act) {
-            const size_t s = (size_t)cur.slot;
+            const size_t s = (size_t)e * NN + pa;
             T *o = A.out;
             o[s] = Rm;
             o[A.EN + s] = Rq[0];
@@ -520,8 +591,10 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
         }
         cur = nxt;
         n_nx = n_nx2;
-        __syncthreads();
+        esync();
     }
+    // no copy may still be writing LDS when the workgroup ends
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
 }
 
 // generic_scalar_convec_ijk (elem_convec.f90:472-594):
@@ -537,9 +610,9 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
     const int a = act ? lane : 0;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1);
     struct In {
         T et, u[3], fv[3], He[9], vol;
-        int slot;
     };
     auto load = [&](In &x, int e) {
         const size_t ea = (size_t)e * NN + a;
@@ -554,13 +627,14 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
 #pragma unroll
         for (int i = 0; i < 9; ++i) x.He[i] = hp[i];
         x.vol = A.gpvol[ea];
-        x.slot = A.slot[ea];
     };
     const ElemRange er = elem_range(A.nelem);
+    if (er.begin >= er.end) return;
+    const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
     In cur, nxt;
-    if (er.begin < er.end) load(cur, er.begin);
+    load(cur, er.begin);
     for (int e = er.begin; e < er.end; e += er.stride) {
-        if (e + er.stride < er.end) load(nxt, e + er.stride);
+        load(nxt, min(e + er.stride, elast));
         if (act) {
             s[a] = cur.et;
 #pragma unroll
@@ -569,7 +643,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
                 s[(4 + d) * NT + a] = cur.fv[d];
             }
         }
-        __syncthreads();
+        lds_barrier();
         T gE[3] = {0, 0, 0}, divU = 0, divF = 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -602,16 +676,17 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
         T r = T(0.5) * (divF + cur.et * divU);
 #pragma unroll
         for (int i = 0; i < 3; ++i) r += T(0.5) * (cur.u[i] * gE[i]);
-        if (act) A.out[cur.slot] = cur.vol * r;
+        if (act) A.out[(size_t)e * NN + pa] = cur.vol * r;
         cur = nxt;
-        __syncthreads();
+        lds_barrier();
     }
 }
 
-// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(j), j over
-// the node's slots (contiguous; the element kernels write in slot order).
+// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(src[j]), j
+// over the node's slots row[n]..row[n+1]-1 in ascending (element, node) order.
 template <typename TS, typename TR>
 __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict__ row,
+                                                const int *__restrict__ src,
                                                 const TS *__restrict__ scr, size_t EN, int nf,
                                                 TR *__restrict__ R0, TR *__restrict__ R1,
                                                 TR *__restrict__ R2, TR *__restrict__ R3,
@@ -624,16 +699,433 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
     for (int f = 0; f < nf; ++f) {
         TR acc = init ? TR(0) : R[f][n];
         const TS *s = scr + (size_t)f * EN;
-        for (int j = j0; j < j1; ++j) acc += fact * TR(s[j]);
+        for (int j = j0; j < j1; ++j) acc += fact * TR(s[src[j]]);
         R[f][n] = acc;
     }
 }
 
 // ---------------------------------------------------------------- launch helpers
+// ---------------------------------------------------------------------------
+// Fused-stage element kernel with the work of each element split over two
+// waves that share its LDS: wave 0 evaluates the split-form convection, wave 1
+// the viscous operator and hands its residuals to wave 0 through LDS; wave 0
+// writes their sum.  Each wave holds only its own operator's live state, which
+// keeps both free of register spills at three waves per SIMD (a single wave
+// doing both spills, and its spill reloads would wait for the prefetch of the
+// next element).  Requires (p+1)^3 <= 64 (one wave per operator).
+//
+// Per element, both waves pass four LDS-only barriers:
+//   B1  geometry of e copied to LDS and state of e loaded (prefetched)
+//   B2  geometry read into registers: the copy of the next element may start
+//   B3  nodal fields of e staged
+//   B4  viscous residuals of e handed over
+template <int P, typename T> struct SplitLDS {
+    static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
+    static constexpr int HE_BYTES = 9 * NN * (int)sizeof(T), GV_BYTES = NN * (int)sizeof(T);
+    static constexpr int ROWS16 = (HE_BYTES + GV_BYTES + 1023) / 1024;
+    static constexpr int ROWS4 = (HE_BYTES + GV_BYTES + 255) / 256;
+    static constexpr int GEO = ROWS16 * 1024 > ROWS4 * 256 ? ROWS16 * 1024 : ROWS4 * 256;
+    // nodal fields: rho, u1..3, T, p, H, rho*H, q1..3
+    static constexpr int F_RHO = 0, F_U = 1, F_T = 4, F_P = 5, F_H = 6, F_RH = 7, F_Q = 8,
+                         NF1 = 11;
+};
+
+template <int P, typename T> struct SplitState {
+    double cr, cq[3], cE;
+    T mue;
+};
+
+template <int P, typename T>
+__device__ __forceinline__ void split_load(const RhsArgs<T> &A, SplitState<P, T> &x, int e, int n,
+                                           int a, bool visc)
+{
+    constexpr int NN = SplitLDS<P, T>::NN;
+    x.cr = A.cr[n];
+    x.cq[0] = A.cq[n];
+    x.cq[1] = A.cq[n + A.npoin];
+    x.cq[2] = A.cq[n + 2 * (size_t)A.npoin];
+    x.cE = A.cE[n];
+    (void)NN;
+    if (visc) x.mue = A.mu_e[e + (size_t)A.nelem * a];
+}
+
+// geometry of element e (both waves: alternate rows) and the state of node n
+template <int P, typename T>
+__device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned char *geo, int e,
+                                            int wl, int role)
+{
+    using L = SplitLDS<P, T>;
+    const unsigned char *he = (const unsigned char *)(A.He + (size_t)e * 9 * L::NN);
+    const unsigned char *gv = (const unsigned char *)(A.gpvol + (size_t)e * L::NN);
+    if (A.geo16)
+        geo_copy<16, L::HE_BYTES, L::GV_BYTES, 2>(he, gv, geo, wl, role);
+    else
+        geo_copy<4, L::HE_BYTES, L::GV_BYTES, 2>(he, gv, geo, wl, role);
+}
+
+template <int P, typename T>
+__global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T> A)
+{
+    using L = SplitLDS<P, T>;
+    constexpr int NN = L::NN, NT = 64;
+    static_assert(NN <= 64, "one wave per operator");
+    __shared__ T s1[L::NF1 * NT];
+    __shared__ T s2[18 * NT]; // viscous fluxes, then the hand-over of the residuals
+    __shared__ __attribute__((aligned(16))) unsigned char geo[L::GEO];
+
+    const int role = threadIdx.x >> 6; // 0: convection, 1: viscous (wave-uniform)
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    const bool visc = role == 1;
+
+    const ElemRange er = elem_range(A.nelem);
+    if (er.begin >= er.end) return;
+    const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
+
+    LineTab<P, T> lt;
+    build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1); // scratch position within the element
+
+    SplitState<P, T> cur, nxt;
+    split_issue<P, T>(A, geo, er.begin, lane, role);
+    split_load<P, T>(A, cur, er.begin, A.connecT[(size_t)er.begin * NN + a], a, visc);
+    int n_nx = A.connecT[(size_t)min(er.begin + er.stride, elast) * NN + a];
+
+    // primitives of updateF's EOS block at the lane's node, from the FP64 state
+    auto prim = [&](const SplitState<P, T> &c, double &r, double ud[3], double &pd, double &hd,
+                    double &Td, double &umag) {
+        r = c.cr;
+        const double ri = 1.0 / r;
+        umag = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ud[d] = c.cq[d] * ri;
+            umag += ud[d] * ud[d];
+        }
+        const double eint = c.cE * ri - 0.5 * umag;
+        pd = r * (A.gamma - 1.0) * eint;
+        hd = (A.gamma / (A.gamma - 1.0)) * pd * ri;
+        Td = pd * ri / A.Rgas;
+    };
+    auto line = [&](const T *sf, int k, T own) {
+        T v = lt.ds[k] * own;
+#pragma unroll
+        for (int o = 0; o < P; ++o) v += lt.d[k][o] * sf[lt.off[k][o]];
+        return v;
+    };
+
+    if (!visc) {
+        // ======================= convection wave
+        for (int e = er.begin; e < er.end; e += er.stride) {
+            const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            lds_barrier(); // B1
+            T He[3][3];
+            const T *hp = (const T *)geo + 9 * a;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
+            const T vol = ((const T *)geo)[9 * NN + a];
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+            lds_barrier(); // B2
+            split_issue<P, T>(A, geo, e2, lane, role);
+            split_load<P, T>(A, nxt, e2, n_nx, a, false);
+            const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
+            // stage rho, u, p - p_ref, H = h + k, rho H, q
+            T rho, u[3], q[3], H, rhoH;
+            {
+                double r, ud[3], pd, hd, Td, umag;
+                prim(cur, r, ud, pd, hd, Td, umag);
+                rho = T(r);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    u[d] = T(ud[d]);
+                    q[d] = T(cur.cq[d]);
+                }
+                H = T(hd + 0.5 * umag);
+                rhoH = T(r * (hd + 0.5 * umag));
+                if (act) {
+                    s1[L::F_RHO * NT + a] = rho;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        s1[(L::F_U + d) * NT + a] = u[d];
+                        s1[(L::F_Q + d) * NT + a] = q[d];
+                    }
+                    s1[L::F_P * NT + a] = T(pd - A.p_ref);
+                    s1[L::F_H * NT + a] = H;
+                    s1[L::F_RH * NT + a] = rhoH;
+                }
+            }
+            lds_barrier(); // B3
+
+            T Rm = 0, Rq[3] = {0, 0, 0}, Re = 0;
+            T Ut[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
+            // velocity lines: contravariant line weights W_k(m), div(u_i u), div u
+            // and q_k d/dxi_k u_i
+            T Wo[3], Wx[3][P];
+            T divUw = 0;
+            {
+                T Qt[3], divFuu[3] = {0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Qt[k] = He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const T c0 = lt.ds[k];
+                    T du[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) du[i] = c0 * u[i];
+                    Wo[k] = c0 * Ut[k];
+                    divUw += Wo[k];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) divFuu[i] += Wo[k] * u[i];
+#pragma unroll
+                    for (int o = 0; o < P; ++o) {
+                        const int b = lt.off[k][o];
+                        const T c = lt.d[k][o];
+                        T ub[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            ub[i] = s1[(L::F_U + i) * NT + b];
+                            du[i] += c * ub[i];
+                        }
+                        const T w = c * (He[0][k] * ub[0] + He[1][k] * ub[1] + He[2][k] * ub[2]);
+                        Wx[k][o] = w;
+                        divUw += w;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) divFuu[i] += w * ub[i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Rq[i] += Qt[k] * du[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Rq[i] = T(0.25) * (Rq[i] + rho * divFuu[i]);
+            }
+            // density line: u.grad rho terms
+            {
+                T d[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) d[k] = line(s1 + L::F_RHO * NT, k, rho);
+                const T urho = Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2];
+                Rm += T(0.5) * urho;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * u[i] * urho;
+                Re += T(0.25) * H * urho;
+            }
+            // p, H, rho H, q_i lines
+            T divQ = 0;
+#pragma unroll 1
+            for (int fi = 1; fi < 7; ++fi) {
+                const int f = fi == 1 ? L::F_P : (fi == 2 ? L::F_H : (fi == 3 ? L::F_RH : L::F_Q + fi - 4));
+                const T *sf = s1 + f * NT;
+                const T own = sf[a];
+                T d[3], div = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    T v = lt.ds[k] * own, w = Wo[k] * own;
+#pragma unroll
+                    for (int o = 0; o < P; ++o) {
+                        const T x = sf[lt.off[k][o]];
+                        v += lt.d[k][o] * x;
+                        w += Wx[k][o] * x;
+                    }
+                    d[k] = v;
+                    div += w;
+                }
+                if (fi == 1) {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Rq[i] += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                } else if (fi == 2) { // H: q.grad H + rho div(H u)
+                    T qt = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        qt += (He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2]) * d[k];
+                    Re += T(0.25) * (qt + rho * div);
+                } else { // rho H, q_i: u.grad s + div(s u)
+                    const T c = T(0.25) * (Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2] + div);
+                    if (fi == 3) {
+                        Re += c;
+                    } else {
+                        const int i = fi - 4;
+                        divQ += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
+                        if (i == 0) Rq[0] += c;
+                        else if (i == 1) Rq[1] += c;
+                        else Rq[2] += c;
+                    }
+                }
+            }
+            Rm += T(0.5) * (divQ + rho * divUw);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * (u[i] * divQ + q[i] * divUw);
+            Re += T(0.25) * (H * divQ + rhoH * divUw);
+            lds_barrier(); // B4: the viscous residuals are in s2
+            if (act) {
+                const size_t s = (size_t)e * NN + pa;
+                T *o = A.out;
+                o[s] = Rm * vol + s2[0 * NT + a];
+                o[A.EN + s] = Rq[0] * vol + s2[1 * NT + a];
+                o[2 * A.EN + s] = Rq[1] * vol + s2[2 * NT + a];
+                o[3 * A.EN + s] = Rq[2] * vol + s2[3 * NT + a];
+                o[4 * A.EN + s] = Re * vol + s2[4 * NT + a];
+            }
+            cur = nxt;
+            n_nx = n_nx2;
+        }
+    } else {
+        // ======================= viscous wave
+        // transposed coefficients dl(b,k,I_k(a)) of the lane's lines
+        T tt[3][P];
+        {
+            constexpr int N1 = P + 1;
+            const int own[3] = {min(max(A.gI[a] - 1, 0), P), min(max(A.gJ[a] - 1, 0), P),
+                                min(max(A.gK[a] - 1, 0), P)};
+            (void)N1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int o = 0; o < P; ++o) tt[k][o] = A.dl[lt.off[k][o] + NN * (k + 3 * own[k])];
+        }
+        for (int e = er.begin; e < er.end; e += er.stride) {
+            const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            lds_barrier(); // B1
+            T He[3][3];
+            const T *hp = (const T *)geo + 9 * a;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
+            const T vol = ((const T *)geo)[9 * NN + a];
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+            lds_barrier(); // B2
+            split_issue<P, T>(A, geo, e2, lane, role);
+            split_load<P, T>(A, nxt, e2, n_nx, a, true);
+            const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
+            T rho, u[3], Tk, muf;
+            const T mue = cur.mue;
+            {
+                double r, ud[3], pd, hd, Td, umag;
+                prim(cur, r, ud, pd, hd, Td, umag);
+                rho = T(r);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) u[d] = T(ud[d]);
+                Tk = T(Td);
+                muf = T(sutherland<double>(A.mu_factor, Td));
+                if (act) s1[L::F_T * NT + a] = Tk;
+            }
+            lds_barrier(); // B3
+            // grad u, grad rho, grad T
+            T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                T du[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) du[i] = line(s1 + (L::F_U + i) * NT, k, u[i]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) gU[i][j] += He[j][k] * du[i];
+            }
+            T gR[3], gt[3];
+            {
+                T dr[3], dT[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    dr[k] = line(s1 + L::F_RHO * NT, k, rho);
+                    dT[k] = line(s1 + L::F_T * NT, k, Tk);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    gR[i] = He[i][0] * dr[0] + He[i][1] * dr[1] + He[i][2] * dr[2];
+                    gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
+                }
+            }
+            // viscous fluxes as contravariant components G_k(F) = gpvol sum_d He(d,k) F_d
+            {
+                const T mut = muf + mue;
+                const T divU = gU[0][0] + gU[1][1] + gU[2][2];
+                const T tdiv = muf * T(2.0 / 3.0) * divU;
+                T tau[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) tau[i][i] -= tdiv;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
+                    T tuk = 0;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const T g = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
+                        if (act) s2[(9 + 3 * r + k) * NT + a] = g;
+                        tuk += g * u[r]; // tau is symmetric: contravariant (tau u)_k
+                    }
+                    if (act) {
+                        s2[(6 + k) * NT + a] = tuk;
+                        s2[(0 + k) * NT + a] = h0 * gR[0] + h1 * gR[1] + h2 * gR[2];
+                        s2[(3 + k) * NT + a] = h0 * gt[0] + h1 * gt[1] + h2 * gt[2];
+                    }
+                }
+            }
+            // this wave alone reads s2 next: its LDS instructions execute in order
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            // pass 2: weak divergence sum_k sum_b dl(b,k,I_k(a)) G_k(F)(b)
+            T W6[6];
+#pragma unroll 1
+            for (int fl = 0; fl < 6; ++fl) {
+                T acc = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const T *sf = s2 + (3 * fl + k) * NT;
+                    T v = lt.ds[k] * sf[a];
+#pragma unroll
+                    for (int o = 0; o < P; ++o) v += tt[k][o] * sf[lt.off[k][o]];
+                    acc += v;
+                }
+                if (fl == 0) W6[0] = acc;
+                else if (fl == 1) W6[1] = acc;
+                else if (fl == 2) W6[2] = acc;
+                else if (fl == 3) W6[3] = acc;
+                else if (fl == 4) W6[4] = acc;
+                else W6[5] = acc;
+            }
+            const T nu_e = A.c_rho * mue / rho;
+            const T kappa = muf * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            if (act) {
+                s2[0 * NT + a] = nu_e * W6[0];
+                s2[1 * NT + a] = W6[3];
+                s2[2 * NT + a] = W6[4];
+                s2[3 * NT + a] = W6[5];
+                s2[4 * NT + a] = W6[2] + kappa * W6[1];
+            }
+            lds_barrier(); // B4
+            cur = nxt;
+            n_nx = n_nx2;
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+}
+
 template <typename T>
 static int launch_rhs(sodhip_ctx ctx, int porder, bool conv, bool diff, bool cons,
-                      const RhsArgs<T> &A, int grid)
+                      const RhsArgs<T> &A0, int grid)
 {
+    // 16-byte geometry copies need every element's He and gpvol block on a
+    // 16-byte boundary
+    RhsArgs<T> A = A0;
+    const size_t nn = (size_t)nnode_of(porder);
+    A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
+              (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
 #define SODHIP_RHS_CASE(PP)                                                                \
     case PP:                                                                               \
         if (conv && diff && cons)                                                          \
@@ -663,8 +1155,8 @@ static int launch_gather(sodhip_ctx ctx, const MeshCache *m, const TS *scr, size
 {
     const int bs = 256;
     const int grid = (m->npoin + bs - 1) / bs;
-    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, scr, EN, nf, R0, R1, R2, R3,
-                                                   R4, init, fact);
+    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, m->src, scr, EN, nf, R0, R1,
+                                                   R2, R3, R4, init, fact);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
 }
@@ -700,7 +1192,7 @@ static int full_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const i
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
-    A.slot = m->slot;
+    A.perm = m->perm;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -748,7 +1240,7 @@ static int full_diffusion(sodhip_ctx ctx, int porder, int nelem, int npoin, cons
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
-    A.slot = m->slot;
+    A.perm = m->perm;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -793,7 +1285,7 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     A.npoin = m->npoin;
     A.EN = (size_t)m->nelem * nnode_of(porder);
     A.connecT = m->connecT;
-    A.slot = m->slot;
+    A.perm = m->perm;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;
@@ -815,7 +1307,23 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     A.c_ener = T(c_ener);
     A.out = out;
     int tok = prof_begin(ctx, 0);
-    int rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 16));
+    int rc = SODHIP_OK;
+#if SODHIP_SPLIT_RHS
+    // FP64: one wave per operator (a single wave holding both spills);
+    // FP32 operators fit one wave without spills
+    if (sizeof(T) == 8 && (porder == 2 || porder == 3)) {
+        const size_t nn = (size_t)nnode_of(porder);
+        A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
+                  (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
+        const int grid = elem_grid(ctx, m->nelem, 16);
+        if (porder == 2)
+            k_rhs_split<2, T><<<grid, 128, 0, ctx->stream>>>(A);
+        else
+            k_rhs_split<3, T><<<grid, 128, 0, ctx->stream>>>(A);
+        if (hipGetLastError() != hipSuccess) rc = set_error(SODHIP_ERR_HIP, "k_rhs_split launch failed");
+    } else
+#endif
+        rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 16));
     prof_end(ctx, tok);
     return rc;
 }
@@ -850,7 +1358,7 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
     A.npoin = npoin;
     A.EN = EN;
     A.connecT = m->connecT;
-    A.slot = m->slot;
+    A.perm = m->perm;
     A.He = He;
     A.gpvol = gpvol;
     A.dl = dl;

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "device.hpp"
 
 namespace sodhip {
 
@@ -176,10 +177,14 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
         x.h = act ? A.helem[e + (size_t)A.nelem * a] : 0.0;
         x.hk = A.helem_k[e];
     };
+    if (eb + rk >= ee) return;
+    // elements past the block's range are clamped to its last one: the extra
+    // loads keep the prefetch unconditional
+    const int elast = eb + rk + (ee - 1 - eb - rk) / per * per;
     In cur, nxt;
-    if (eb + rk < ee) load(cur, eb + rk);
+    load(cur, eb + rk);
     for (int e = eb + rk; e < ee; e += per) {
-        if (e + per < ee) load(nxt, e + per);
+        load(nxt, min(e + per, elast));
         const double aux2 = sqrt(cur.u0 * cur.u0 + cur.u1 * cur.u1 + cur.u2 * cur.u2);
         const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * cur.T);
         double be = act ? (cur.rho * cur.hk) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
@@ -188,10 +193,10 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
             for (int s = 32; s > 0; s >>= 1) be = fmax(be, __shfl_xor(be, s, 64));
         } else {
             red[lane] = be;
-            __syncthreads();
+            lds_barrier();
             for (int s = NT / 2; s > 0; s >>= 1) {
                 if (lane < s) red[lane] = fmax(red[lane], red[lane + s]);
-                __syncthreads();
+                lds_barrier();
             }
             be = red[0];
         }
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
             sm[a] = ml;
             A.mue_l[e + (size_t)A.nelem * a] = ml;
         }
-        __syncthreads();
+        lds_barrier();
         double acc = 0.0;
 #pragma unroll
         for (int l = 0; l < 3; ++l)
@@ -213,7 +218,128 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
                 for (int nn = 0; nn < 3; ++nn) acc += w[l] * w[m] * w[nn] * sm[src[9 * l + 3 * m + nn]];
         if (act) A.mu_e[e + (size_t)A.nelem * a] = acc;
         cur = nxt;
-        __syncthreads();
+        lds_barrier();
+    }
+}
+
+// Chunked variant for (p+1)^3 <= 64 (one wave per workgroup): the wave walks
+// CH consecutive elements at a time so that the element-fastest arrays
+// helem(nelem,nnode), mue_l and mu_e move as whole 128-byte lines: the CH x NN
+// block is read and written through LDS with lane t holding (node t/CH,
+// element t%CH), while the per-element arithmetic keeps lane = node.
+template <int P, int CH>
+__global__ __launch_bounds__(64) void k_smart_visc_ch(SensorArgs A)
+{
+    constexpr int N1 = SG<P>::N1, NN = SG<P>::NN;
+    static_assert(NN <= 64, "one wave per element");
+    constexpr int BLK = CH * NN, ROWS = (BLK + 63) / 64;
+    __shared__ double lb[BLK]; // helem, then mue_l, of the chunk: [c][a]
+    __shared__ double ob[BLK]; // mu_e of the chunk
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    double norm = 1.0;
+    if (A.pmax) { // fold the partial maxima across the wave
+        norm = 0.0;
+        for (int i = lane; i < A.nparts; i += 64) norm = fmax(norm, A.pmax[i]);
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) norm = fmax(norm, __shfl_xor(norm, s, 64));
+    }
+    int I = 1, J = 1, K = 1;
+    for (int k = 0; k < N1; ++k)
+        for (int j = 0; j < N1; ++j)
+            for (int i = 0; i < N1; ++i)
+                if (A.inv[i + N1 * (j + N1 * k)] - 1 == a) {
+                    I = i + 1;
+                    J = j + 1;
+                    K = k + 1;
+                }
+    const int ii = geo_of(I, N1), jj = geo_of(J, N1), kk = geo_of(K, N1);
+    int src[27];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const int ci = convert_ijk(ii + l - 1, P) - 1, cj = convert_ijk(jj + m - 1, P) - 1,
+                          ck = convert_ijk(kk + n - 1, P) - 1;
+                src[9 * l + 3 * m + n] = min(max(A.inv[ci + N1 * (cj + N1 * ck)] - 1, 0), NN - 1);
+            }
+    const double w[3] = {0.25, 0.5, 0.25};
+    // XCD-aware: the blocks of one XCD walk chunks of one contiguous eighth
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7, rk = blockIdx.x >> 3;
+    const int span = (A.nelem + 7) >> 3;
+    const int xb = xcd * span, xe = min(A.nelem, xb + span);
+    const int nch = xe > xb ? (xe - xb + CH - 1) / CH : 0;
+    if (rk >= nch) return;
+    struct In {
+        double u0, u1, u2, T, rho, Reta, hk;
+    };
+    auto load = [&](In &x, int e) {
+        const int n = act ? A.connecT[(size_t)e * NN + a] : 0;
+        x.u0 = A.u[n];
+        x.u1 = A.u[n + A.npoin];
+        x.u2 = A.u[n + 2 * (size_t)A.npoin];
+        x.T = A.Tem[n];
+        x.rho = A.rho[n];
+        x.Reta = A.Reta[n];
+        x.hk = A.helem_k[e];
+    };
+    In cur, nxt;
+    load(cur, xb + rk * CH);
+    for (int ci = rk; ci < nch; ci += per) {
+        const int c0 = xb + ci * CH, cn = min(CH, xe - c0);
+        const int cnext = ci + per < nch ? xb + (ci + per) * CH : c0 + cn - 1;
+        // helem of the chunk, whole lines (elements past the range re-read the last)
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int t = r * 64 + lane;
+            if (t < BLK) {
+                const int an = t / CH, c = t % CH;
+                lb[c * NN + an] = A.helem[(c0 + min(c, cn - 1)) + (size_t)A.nelem * an];
+            }
+        }
+        lds_barrier();
+        for (int c = 0; c < cn; ++c) {
+            const int e = c0 + c;
+            load(nxt, c + 1 < cn ? e + 1 : cnext);
+            const double aux2 = sqrt(cur.u0 * cur.u0 + cur.u1 * cur.u1 + cur.u2 * cur.u2);
+            const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * cur.T);
+            double be = act ? (cur.rho * cur.hk) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) be = fmax(be, __shfl_xor(be, s, 64));
+            const double R1 = cur.rho * fabs(cur.Reta) / norm;
+            const double h = lb[c * NN + a];
+            const double Ve = A.ce * R1 * (h * h);
+            const double ml = A.cglob * fmin(Ve, be);
+            if (act) lb[c * NN + a] = ml; // the lane's own h slot
+            lds_barrier();
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 3; ++nn)
+                        acc += w[l] * w[m] * w[nn] * lb[c * NN + src[9 * l + 3 * m + nn]];
+            if (act) ob[c * NN + a] = acc;
+            cur = nxt;
+        }
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int t = r * 64 + lane;
+            if (t < BLK) {
+                const int an = t / CH, c = t % CH;
+                if (c < cn) {
+                    const size_t g = (c0 + c) + (size_t)A.nelem * an;
+                    A.mue_l[g] = lb[c * NN + an];
+                    A.mu_e[g] = ob[c * NN + an];
+                }
+            }
+        }
+        lds_barrier(); // the next chunk overwrites lb and ob
     }
 }
 
@@ -272,11 +398,10 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     A.nparts = kRedBlocks;
     A.mu_e = mu_e;
     A.mue_l = mue_l;
-    const int grid = elem_grid(ctx, nelem, 16);
     switch (porder) {
-    case 2: k_smart_visc<2><<<grid, SG<2>::NT, 0, ctx->stream>>>(A); break;
-    case 3: k_smart_visc<3><<<grid, SG<3>::NT, 0, ctx->stream>>>(A); break;
-    case 4: k_smart_visc<4><<<grid, SG<4>::NT, 0, ctx->stream>>>(A); break;
+    case 2: k_smart_visc_ch<2, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
+    case 3: k_smart_visc_ch<3, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
+    case 4: k_smart_visc<4><<<elem_grid(ctx, nelem, 16), SG<4>::NT, 0, ctx->stream>>>(A); break;
     }
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());

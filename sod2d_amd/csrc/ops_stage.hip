@@ -30,20 +30,37 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 template <typename TS>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ row,
-    const TS *__restrict__ scr, size_t EN,
+    const int *__restrict__ src, const TS *__restrict__ scr, size_t EN,
     const double *__restrict__ invMl, double fact, double lam1, double lams, double gdt,
     const double *__restrict__ rho1, const double *__restrict__ q1, const double *__restrict__ E1,
     double *__restrict__ rho, double *__restrict__ q, double *__restrict__ E,
     double *__restrict__ Rmass, double *__restrict__ Rmom, double *__restrict__ Rener)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware: the blocks of one XCD (blockIdx % 8) take one contiguous
+    // eighth of the nodes, whose partials share that XCD's L2 lines
+    const int per = gridDim.x >> 3; // grid is a multiple of 8
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
     const int j0 = row[n], j1 = row[n + 1];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int j = j0; j < j1; ++j) {
+    // four slots per step, all loads issued before the (in-order) additions
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) acc[f] += fact * (double)scr[(size_t)f * EN + j];
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
+        double v[4][5];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? (double)scr[(size_t)f * EN + sj[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) acc[f] += fact * v[u][f];
+            }
     }
     const double m = invMl[n];
     const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
@@ -84,8 +101,8 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
     int tok = prof_begin(ctx, 1);
-    k_gather_update<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(
-        npoin, npoin_w, lpoin_w, m->row, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
+    k_gather_update<T><<<((npoin_w + 255) / 256 + 7) / 8 * 8, 256, 0, ctx->stream>>>(
+        npoin, npoin_w, lpoin_w, m->row, m->src, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());

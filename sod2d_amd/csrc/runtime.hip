@@ -55,7 +55,8 @@ static void free_mesh(MeshCache *m)
     if (!m) return;
     (void)hipFree(m->connecT);
     (void)hipFree(m->row);
-    (void)hipFree(m->slot);
+    (void)hipFree(m->src);
+    (void)hipFree(m->perm);
 }
 
 int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
@@ -74,7 +75,17 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     SODHIP_TRY_HIP(hipMemcpyAsync(h.data(), connec, en * sizeof(int), hipMemcpyDeviceToHost,
                                   ctx->stream));
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<int> cT(en), row((size_t)npoin + 1, 0), slot(en);
+    std::vector<int> cT(en), row((size_t)npoin + 1, 0), src(en), perm(nn);
+    {
+        // element-local scratch position: lexicographic in the (i, j, k) line
+        // indices of the standard tables of this order
+        std::vector<int> gI(nn), gJ(nn), gK(nn);
+        if (sodhip_basis_tables(porder, nullptr, nullptr, gI.data(), gJ.data(), gK.data(),
+                                nullptr, nullptr, nullptr, nullptr) != SODHIP_OK)
+            return SODHIP_ERR_ARG;
+        const int n1 = porder + 1;
+        for (int a = 0; a < nn; ++a) perm[a] = (gI[a] - 1) + n1 * ((gJ[a] - 1) + n1 * (gK[a] - 1));
+    }
     for (size_t i = 0; i < en; ++i) {
         int v = h[i];
         if (v < 1 || v > npoin)
@@ -91,7 +102,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     for (long e = 0; e < nelem; ++e) // ascending (e,a): the reference's serial assembly order
         for (int a = 0; a < nn; ++a) {
             int n = cT[(size_t)e * nn + a];
-            slot[(size_t)e * nn + a] = fill[n]++;
+            src[fill[n]++] = (int)((size_t)e * nn + perm[a]);
         }
     auto m = std::make_unique<MeshCache>();
     m->connec_key = connec;
@@ -100,8 +111,9 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     m->npoin = npoin;
     hipError_t e1 = hipMalloc(&m->connecT, en * sizeof(int));
     hipError_t e2 = hipMalloc(&m->row, ((size_t)npoin + 1) * sizeof(int));
-    hipError_t e3 = hipMalloc(&m->slot, en * sizeof(int));
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e3 = hipMalloc(&m->src, en * sizeof(int));
+    hipError_t e4 = hipMalloc(&m->perm, nn * sizeof(int));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
         free_mesh(m.get());
         return set_error(SODHIP_ERR_ALLOC, "mesh cache allocation failed");
     }
@@ -109,7 +121,9 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
                                   ctx->stream));
     SODHIP_TRY_HIP(hipMemcpyAsync(m->row, row.data(), row.size() * sizeof(int),
                                   hipMemcpyHostToDevice, ctx->stream));
-    SODHIP_TRY_HIP(hipMemcpyAsync(m->slot, slot.data(), en * sizeof(int), hipMemcpyHostToDevice,
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->src, src.data(), en * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->perm, perm.data(), nn * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream));
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
     *out = m.get();
