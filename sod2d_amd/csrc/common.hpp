@@ -56,6 +56,13 @@ struct MeshCache {
     int *row = nullptr;     // npoin+1
     int *src = nullptr;     // nelem*nnode, scratch position of each slot
     int *perm = nullptr;    // nnode, element-local scratch position of node a
+    // host copies (for derived tables)
+    std::vector<int> h_row, h_src;
+    // the incidence restricted to a working-node list, in list order:
+    // working node i owns wsrc[wrow[i]..wrow[i+1]-1]
+    const int *lpw_key = nullptr; // device pointer of the caller's lpoin_w
+    int npoin_w = 0;
+    int *wrow = nullptr, *wsrc = nullptr;
 };
 
 struct PresentEntry {
@@ -98,6 +105,8 @@ int ensure_buf(DevBuf &b, size_t bytes);
 // mesh cache lookup/build for (connec, porder, nelem, npoin)
 int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
              MeshCache **out);
+// working-node incidence of mesh m for the list lpoin_w (1-based, npoin_w entries)
+int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w);
 
 // launch timing helpers: prof_begin returns a token (or -1); prof_end closes it
 int prof_begin(sodhip_ctx ctx, int kind);

@@ -12,8 +12,8 @@
 //  * blocks walk the elements in a grid-stride loop whose block -> element map
 //    gives each XCD (blockIdx % 8) one contiguous eighth of the elements, so
 //    nodes shared by neighbouring elements are re-read from the same L2;
-//  * element residuals go to a per element-node scratch array (SoA, whole
-//    elements: coalesced writes); a second kernel gathers them per node through
+//  * element residuals go to a scratch array of 5-field records per element-node
+//    (whole elements: coalesced writes); a second kernel gathers them per node through
 //    the node-ordered incidence table in the reference's serial (element, node)
 //    order -- deterministic, no atomics.
 //
@@ -136,7 +136,8 @@ template <typename T> struct RhsArgs {
     const double *cr, *cq, *cE;
     double gamma, Rgas, mu_factor, p_ref;
     const int *perm; // element-local scratch position of each node
-    T *out;          // SoA [5][EN]: mass, mom1, mom2, mom3, energy; (e,a) at e*NN + perm[a]
+    T *out;          // records of 5 (mass, mom1, mom2, mom3, energy) per element-node;
+                     // (e,a) at record e*NN + perm[a]
     int geo16;       // He and gpvol blocks of every element are 16-byte aligned
 };
 
@@ -581,13 +582,12 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
         }
 
         if (act) {
-            const size_t s = (size_t)e * NN + pa;
-            T *o = A.out;
-            o[s] = Rm;
-            o[A.EN + s] = Rq[0];
-            o[2 * A.EN + s] = Rq[1];
-            o[3 * A.EN + s] = Rq[2];
-            o[4 * A.EN + s] = Re;
+            T *o = A.out + 5 * ((size_t)e * NN + pa);
+            o[0] = Rm;
+            o[1] = Rq[0];
+            o[2] = Rq[1];
+            o[3] = Rq[2];
+            o[4] = Re;
         }
         cur = nxt;
         n_nx = n_nx2;
@@ -682,8 +682,9 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
     }
 }
 
-// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * s_f(src[j]), j
-// over the node's slots row[n]..row[n+1]-1 in ascending (element, node) order.
+// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * scr[src[j]*nf + f], j
+// over the node's slots row[n]..row[n+1]-1 in ascending (element, node) order
+// (records of nf fields per element-node).
 template <typename TS, typename TR>
 __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict__ row,
                                                 const int *__restrict__ src,
@@ -696,10 +697,10 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
     if (n >= npoin) return;
     const int j0 = row[n], j1 = row[n + 1];
     TR *R[5] = {R0, R1, R2, R3, R4};
+    (void)EN;
     for (int f = 0; f < nf; ++f) {
         TR acc = init ? TR(0) : R[f][n];
-        const TS *s = scr + (size_t)f * EN;
-        for (int j = j0; j < j1; ++j) acc += fact * TR(s[src[j]]);
+        for (int j = j0; j < j1; ++j) acc += fact * TR(scr[(size_t)src[j] * nf + f]);
         R[f][n] = acc;
     }
 }
@@ -963,13 +964,12 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             Re += T(0.25) * (H * divQ + rhoH * divUw);
             lds_barrier(); // B4: the viscous residuals are in s2
             if (act) {
-                const size_t s = (size_t)e * NN + pa;
-                T *o = A.out;
-                o[s] = Rm * vol + s2[0 * NT + a];
-                o[A.EN + s] = Rq[0] * vol + s2[1 * NT + a];
-                o[2 * A.EN + s] = Rq[1] * vol + s2[2 * NT + a];
-                o[3 * A.EN + s] = Rq[2] * vol + s2[3 * NT + a];
-                o[4 * A.EN + s] = Re * vol + s2[4 * NT + a];
+                T *o = A.out + 5 * ((size_t)e * NN + pa);
+                o[0] = Rm * vol + s2[0 * NT + a];
+                o[1] = Rq[0] * vol + s2[1 * NT + a];
+                o[2] = Rq[1] * vol + s2[2 * NT + a];
+                o[3] = Rq[2] * vol + s2[3 * NT + a];
+                o[4] = Re * vol + s2[4 * NT + a];
             }
             cur = nxt;
             n_nx = n_nx2;

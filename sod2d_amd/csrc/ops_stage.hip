@@ -29,8 +29,8 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 
 template <typename TS>
 __global__ __launch_bounds__(256) void k_gather_update(
-    int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ row,
-    const int *__restrict__ src, const TS *__restrict__ scr, size_t EN,
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
+    const int *__restrict__ wsrc, const TS *__restrict__ scr, size_t EN,
     const double *__restrict__ invMl, double fact, double lam1, double lams, double gdt,
     const double *__restrict__ rho1, const double *__restrict__ q1, const double *__restrict__ E1,
     double *__restrict__ rho, double *__restrict__ q, double *__restrict__ E,
@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void k_gather_update(
     const int i = blk * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
-    const int j0 = row[n], j1 = row[n + 1];
+    const int j0 = wrow[i], j1 = wrow[i + 1];
+    const int *__restrict__ src = wsrc;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // four slots per step, all loads issued before the (in-order) additions
     for (int j = j0; j < j1; j += 4) {
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256) void k_gather_update(
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? (double)scr[(size_t)f * EN + sj[u]] : 0.0;
+            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? (double)scr[5 * (size_t)sj[u] + f] : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (sj[u] >= 0) {
@@ -92,6 +93,7 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     MeshCache *m = nullptr;
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
+    if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
     const size_t EN = (size_t)nelem * nnode_of(porder);
     if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(T)))) return rc;
     T *scr = (T *)ctx->scratch.ptr;
@@ -102,7 +104,7 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     SODHIP_TRY_HIP(hipGetLastError());
     int tok = prof_begin(ctx, 1);
     k_gather_update<T><<<((npoin_w + 255) / 256 + 7) / 8 * 8, 256, 0, ctx->stream>>>(
-        npoin, npoin_w, lpoin_w, m->row, m->src, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
+        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());

@@ -57,6 +57,8 @@ static void free_mesh(MeshCache *m)
     (void)hipFree(m->row);
     (void)hipFree(m->src);
     (void)hipFree(m->perm);
+    (void)hipFree(m->wrow);
+    (void)hipFree(m->wsrc);
 }
 
 int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
@@ -126,8 +128,48 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     SODHIP_TRY_HIP(hipMemcpyAsync(m->perm, perm.data(), nn * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream));
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    m->h_row = std::move(row);
+    m->h_src = std::move(src);
     *out = m.get();
     ctx->meshes.push_back(std::move(m));
+    return SODHIP_OK;
+}
+
+int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
+{
+    if (m->lpw_key == lpoin_w && m->npoin_w == npoin_w && m->wrow) return SODHIP_OK;
+    std::vector<int> lw(npoin_w);
+    SODHIP_TRY_HIP(hipMemcpyAsync(lw.data(), lpoin_w, (size_t)npoin_w * sizeof(int),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> wrow((size_t)npoin_w + 1, 0);
+    for (int i = 0; i < npoin_w; ++i) {
+        const int n = lw[i] - 1;
+        if (n < 0 || n >= m->npoin)
+            return set_error(SODHIP_ERR_ARG, "lpoin_w entry %d outside 1..npoin=%d", lw[i], m->npoin);
+        wrow[(size_t)i + 1] = wrow[i] + (m->h_row[(size_t)n + 1] - m->h_row[n]);
+    }
+    std::vector<int> wsrc(wrow[npoin_w]);
+    for (int i = 0; i < npoin_w; ++i) {
+        const int n = lw[i] - 1;
+        std::copy(m->h_src.begin() + m->h_row[n], m->h_src.begin() + m->h_row[(size_t)n + 1],
+                  wsrc.begin() + wrow[i]);
+    }
+    (void)hipFree(m->wrow);
+    (void)hipFree(m->wsrc);
+    m->wrow = m->wsrc = nullptr;
+    m->lpw_key = nullptr;
+    if (hipMalloc(&m->wrow, wrow.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&m->wsrc, std::max<size_t>(1, wsrc.size()) * sizeof(int)) != hipSuccess)
+        return set_error(SODHIP_ERR_ALLOC, "working incidence allocation failed");
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->wrow, wrow.data(), wrow.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    if (!wsrc.empty())
+        SODHIP_TRY_HIP(hipMemcpyAsync(m->wsrc, wsrc.data(), wsrc.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    m->lpw_key = lpoin_w;
+    m->npoin_w = npoin_w;
     return SODHIP_OK;
 }
 
