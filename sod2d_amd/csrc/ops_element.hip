@@ -98,7 +98,9 @@ __device__ __forceinline__ void geo_copy(const unsigned char *he, const unsigned
         if (r < ROWS) {
             const int byte = r * R + wl * W;
             const unsigned char *src = byte < HEB ? he + byte : gv + min(byte - HEB, GVB - W);
-            auto *dst = (__attribute__((address_space(3))) void *)(geo + r * R);
+            // offset applied in the LDS address space (a generic round trip of a
+            // constant LDS address miscompiles)
+            auto *dst = (__attribute__((address_space(3))) unsigned char *)geo + r * R;
             if constexpr (W == 16)
                 __builtin_amdgcn_global_load_lds((const void *)src, dst, 16, 0, 0);
             else
@@ -711,9 +713,9 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 // waves that share its LDS: wave 0 evaluates the split-form convection, wave 1
 // the viscous operator and hands its residuals to wave 0 through LDS; wave 0
 // writes their sum.  Each wave holds only its own operator's live state, which
-// keeps both free of register spills at three waves per SIMD (a single wave
-// doing both spills, and its spill reloads would wait for the prefetch of the
-// next element).  Requires (p+1)^3 <= 64 (one wave per operator).
+// keeps both free of register spills in FP64 at p=3 (a single wave doing both
+// spills, and its spill reloads would wait for the prefetch of the next
+// element).  Requires (p+1)^3 <= 64 (one wave per operator).
 //
 // Per element, both waves pass four LDS-only barriers:
 //   B1  geometry of e copied to LDS and state of e loaded (prefetched)
@@ -764,14 +766,47 @@ __device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned char *
         geo_copy<4, L::HE_BYTES, L::GV_BYTES, 2>(he, gv, geo, wl, role);
 }
 
+// The two-wave kernel.  The staged nodal fields and the viscous fluxes are
+// node-major records, so one 16-byte LDS read returns two fields of a
+// neighbour and the field offsets are immediates of the read instruction:
+//   node record (12 values): u1 u2 | u3 rho | p H | rhoH q1 | q2 q3 | T -
+//   flux record (18 values): for k = 1..3, G_k of (grad rho, grad T, tau u,
+//                            tau_1., tau_2., tau_3.)
+//
+// SPLIT_PAIR: line derivatives D0, D1 and W-weighted line sums DIV0, DIV1 of
+// the field pair at record offset OFF (convection wave).
+#define SPLIT_PAIR(OFF, OWN0, OWN1, D0, D1, DIV0, DIV1)                                    \
+    do {                                                                                   \
+        const T o0_ = (OWN0), o1_ = (OWN1);                                                \
+        DIV0 = 0;                                                                          \
+        DIV1 = 0;                                                                          \
+        _Pragma("unroll") for (int k = 0; k < 3; ++k)                                      \
+        {                                                                                  \
+            T v0 = lt.ds[k] * o0_, v1 = lt.ds[k] * o1_;                                    \
+            T w0 = Wo[k] * o0_, w1 = Wo[k] * o1_;                                          \
+            _Pragma("unroll") for (int o = 0; o < P; ++o)                                  \
+            {                                                                              \
+                const V2 x = LD2(nb[k][o] + (OFF));                                        \
+                v0 += lt.d[k][o] * x.x;                                                    \
+                v1 += lt.d[k][o] * x.y;                                                    \
+                w0 += Wx[k][o] * x.x;                                                      \
+                w1 += Wx[k][o] * x.y;                                                      \
+            }                                                                              \
+            D0[k] = v0;                                                                    \
+            D1[k] = v1;                                                                    \
+            DIV0 += w0;                                                                    \
+            DIV1 += w1;                                                                    \
+        }                                                                                  \
+    } while (0)
 template <int P, typename T>
 __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T> A)
 {
     using L = SplitLDS<P, T>;
-    constexpr int NN = L::NN, NT = 64;
+    using V2 = T __attribute__((ext_vector_type(2)));
+    constexpr int NN = L::NN, REC = 12, GREC = 18;
     static_assert(NN <= 64, "one wave per operator");
-    __shared__ T s1[L::NF1 * NT];
-    __shared__ T s2[18 * NT]; // viscous fluxes, then the hand-over of the residuals
+    __shared__ V2 s1[64 * REC / 2]; // records of REC values as pairs
+    __shared__ V2 s2[64 * GREC / 2];
     __shared__ __attribute__((aligned(16))) unsigned char geo[L::GEO];
 
     const int role = threadIdx.x >> 6; // 0: convection, 1: viscous (wave-uniform)
@@ -786,14 +821,20 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
 
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
-    const int pa = min(max(A.perm[a], 0), NN - 1); // scratch position within the element
+    const int pa = min(max(A.perm[a], 0), NN - 1);
+    // neighbour records (element offsets into s1)
+    int nb[3][P];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 0; o < P; ++o) nb[k][o] = lt.off[k][o] * REC;
+    const int own = a * REC;
 
     SplitState<P, T> cur, nxt;
     split_issue<P, T>(A, geo, er.begin, lane, role);
     split_load<P, T>(A, cur, er.begin, A.connecT[(size_t)er.begin * NN + a], a, visc);
     int n_nx = A.connecT[(size_t)min(er.begin + er.stride, elast) * NN + a];
 
-    // primitives of updateF's EOS block at the lane's node, from the FP64 state
     auto prim = [&](const SplitState<P, T> &c, double &r, double ud[3], double &pd, double &hd,
                     double &Td, double &umag) {
         r = c.cr;
@@ -809,12 +850,10 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
         hd = (A.gamma / (A.gamma - 1.0)) * pd * ri;
         Td = pd * ri / A.Rgas;
     };
-    auto line = [&](const T *sf, int k, T own) {
-        T v = lt.ds[k] * own;
-#pragma unroll
-        for (int o = 0; o < P; ++o) v += lt.d[k][o] * sf[lt.off[k][o]];
-        return v;
-    };
+        // indices are even value offsets into the records
+#define LD1(i) (s1[(i) >> 1].x)
+#define LD2(i) (s1[(i) >> 1])
+#define GL2(i) (s2[(i) >> 1])
 
     if (!visc) {
         // ======================= convection wave
@@ -834,7 +873,6 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             split_issue<P, T>(A, geo, e2, lane, role);
             split_load<P, T>(A, nxt, e2, n_nx, a, false);
             const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
-            // stage rho, u, p - p_ref, H = h + k, rho H, q
             T rho, u[3], q[3], H, rhoH;
             {
                 double r, ud[3], pd, hd, Td, umag;
@@ -848,15 +886,11 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
                 H = T(hd + 0.5 * umag);
                 rhoH = T(r * (hd + 0.5 * umag));
                 if (act) {
-                    s1[L::F_RHO * NT + a] = rho;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        s1[(L::F_U + d) * NT + a] = u[d];
-                        s1[(L::F_Q + d) * NT + a] = q[d];
-                    }
-                    s1[L::F_P * NT + a] = T(pd - A.p_ref);
-                    s1[L::F_H * NT + a] = H;
-                    s1[L::F_RH * NT + a] = rhoH;
+                    s1[(own + 0) >> 1] = V2{u[0], u[1]};
+                    s1[(own + 2) >> 1] = V2{u[2], rho};
+                    s1[(own + 4) >> 1] = V2{T(pd - A.p_ref), H};
+                    s1[(own + 6) >> 1] = V2{rhoH, q[0]};
+                    s1[(own + 8) >> 1] = V2{q[1], q[2]};
                 }
             }
             lds_barrier(); // B3
@@ -865,10 +899,10 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             T Ut[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) Ut[k] = He[0][k] * u[0] + He[1][k] * u[1] + He[2][k] * u[2];
-            // velocity lines: contravariant line weights W_k(m), div(u_i u), div u
-            // and q_k d/dxi_k u_i
+            // velocity and density lines: contravariant line weights W_k(m),
+            // div(u_i u), div u, q_k d/dxi_k u_i and d/dxi_k rho
             T Wo[3], Wx[3][P];
-            T divUw = 0;
+            T divUw = 0, drho[3];
             {
                 T Qt[3], divFuu[3] = {0, 0, 0};
 #pragma unroll
@@ -879,84 +913,71 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
                     T du[3];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) du[i] = c0 * u[i];
+                    T dr = c0 * rho;
                     Wo[k] = c0 * Ut[k];
                     divUw += Wo[k];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) divFuu[i] += Wo[k] * u[i];
 #pragma unroll
                     for (int o = 0; o < P; ++o) {
-                        const int b = lt.off[k][o];
                         const T c = lt.d[k][o];
-                        T ub[3];
+                        const V2 x01 = LD2(nb[k][o] + 0), x2r = LD2(nb[k][o] + 2);
+                        const T ub[3] = {x01.x, x01.y, x2r.x};
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            ub[i] = s1[(L::F_U + i) * NT + b];
-                            du[i] += c * ub[i];
-                        }
+                        for (int i = 0; i < 3; ++i) du[i] += c * ub[i];
+                        dr += c * x2r.y;
                         const T w = c * (He[0][k] * ub[0] + He[1][k] * ub[1] + He[2][k] * ub[2]);
                         Wx[k][o] = w;
                         divUw += w;
 #pragma unroll
                         for (int i = 0; i < 3; ++i) divFuu[i] += w * ub[i];
                     }
+                    drho[k] = dr;
 #pragma unroll
                     for (int i = 0; i < 3; ++i) Rq[i] += Qt[k] * du[i];
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) Rq[i] = T(0.25) * (Rq[i] + rho * divFuu[i]);
             }
-            // density line: u.grad rho terms
             {
-                T d[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) d[k] = line(s1 + L::F_RHO * NT, k, rho);
-                const T urho = Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2];
+                const T urho = Ut[0] * drho[0] + Ut[1] * drho[1] + Ut[2] * drho[2];
                 Rm += T(0.5) * urho;
 #pragma unroll
                 for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * u[i] * urho;
                 Re += T(0.25) * H * urho;
             }
-            // p, H, rho H, q_i lines
+            // field pairs (p, H), (rho H, q1), (q2, q3): line derivative d and
+            // W-weighted line sum div = div(s u) of both fields
             T divQ = 0;
-#pragma unroll 1
-            for (int fi = 1; fi < 7; ++fi) {
-                const int f = fi == 1 ? L::F_P : (fi == 2 ? L::F_H : (fi == 3 ? L::F_RH : L::F_Q + fi - 4));
-                const T *sf = s1 + f * NT;
-                const T own = sf[a];
-                T d[3], div = 0;
+            auto grad_i = [&](int i, const T d[3]) { return He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2]; };
+            auto utd = [&](const T d[3]) { return Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2]; };
+            {
+                T dp[3], dH[3], dv0, dv1;
+                SPLIT_PAIR(4, LD1(own + 4), H, dp, dH, dv0, dv1);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    T v = lt.ds[k] * own, w = Wo[k] * own;
+                for (int i = 0; i < 3; ++i) Rq[i] += grad_i(i, dp);
+                T qt = 0;
 #pragma unroll
-                    for (int o = 0; o < P; ++o) {
-                        const T x = sf[lt.off[k][o]];
-                        v += lt.d[k][o] * x;
-                        w += Wx[k][o] * x;
-                    }
-                    d[k] = v;
-                    div += w;
-                }
-                if (fi == 1) {
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) Rq[i] += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
-                } else if (fi == 2) { // H: q.grad H + rho div(H u)
-                    T qt = 0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                        qt += (He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2]) * d[k];
-                    Re += T(0.25) * (qt + rho * div);
-                } else { // rho H, q_i: u.grad s + div(s u)
-                    const T c = T(0.25) * (Ut[0] * d[0] + Ut[1] * d[1] + Ut[2] * d[2] + div);
-                    if (fi == 3) {
-                        Re += c;
-                    } else {
-                        const int i = fi - 4;
-                        divQ += He[i][0] * d[0] + He[i][1] * d[1] + He[i][2] * d[2];
-                        if (i == 0) Rq[0] += c;
-                        else if (i == 1) Rq[1] += c;
-                        else Rq[2] += c;
-                    }
-                }
+                for (int k = 0; k < 3; ++k)
+                    qt += (He[0][k] * q[0] + He[1][k] * q[1] + He[2][k] * q[2]) * dH[k];
+                Re += T(0.25) * (qt + rho * dv1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {
+                T dRH[3], dq0[3], dv0, dv1;
+                SPLIT_PAIR(6, rhoH, q[0], dRH, dq0, dv0, dv1);
+                Re += T(0.25) * (utd(dRH) + dv0);
+                divQ += grad_i(0, dq0);
+                Rq[0] += T(0.25) * (utd(dq0) + dv1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            {
+                T dq1[3], dq2[3], dv0, dv1;
+                SPLIT_PAIR(8, q[1], q[2], dq1, dq2, dv0, dv1);
+                divQ += grad_i(1, dq1);
+                Rq[1] += T(0.25) * (utd(dq1) + dv0);
+                divQ += grad_i(2, dq2);
+                Rq[2] += T(0.25) * (utd(dq2) + dv1);
             }
             Rm += T(0.5) * (divQ + rho * divUw);
 #pragma unroll
@@ -964,30 +985,33 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             Re += T(0.25) * (H * divQ + rhoH * divUw);
             lds_barrier(); // B4: the viscous residuals are in s2
             if (act) {
+                const V2 h01 = s2[3 * a], h23 = s2[3 * a + 1], h4 = s2[3 * a + 2];
                 T *o = A.out + 5 * ((size_t)e * NN + pa);
-                o[0] = Rm * vol + s2[0 * NT + a];
-                o[1] = Rq[0] * vol + s2[1 * NT + a];
-                o[2] = Rq[1] * vol + s2[2 * NT + a];
-                o[3] = Rq[2] * vol + s2[3 * NT + a];
-                o[4] = Re * vol + s2[4 * NT + a];
+                o[0] = Rm * vol + h01.x;
+                o[1] = Rq[0] * vol + h01.y;
+                o[2] = Rq[1] * vol + h23.x;
+                o[3] = Rq[2] * vol + h23.y;
+                o[4] = Re * vol + h4.x;
             }
             cur = nxt;
             n_nx = n_nx2;
         }
     } else {
         // ======================= viscous wave
-        // transposed coefficients dl(b,k,I_k(a)) of the lane's lines
         T tt[3][P];
         {
-            constexpr int N1 = P + 1;
-            const int own[3] = {min(max(A.gI[a] - 1, 0), P), min(max(A.gJ[a] - 1, 0), P),
-                                min(max(A.gK[a] - 1, 0), P)};
-            (void)N1;
+            const int own3[3] = {min(max(A.gI[a] - 1, 0), P), min(max(A.gJ[a] - 1, 0), P),
+                                 min(max(A.gK[a] - 1, 0), P)};
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int o = 0; o < P; ++o) tt[k][o] = A.dl[lt.off[k][o] + NN * (k + 3 * own[k])];
+                for (int o = 0; o < P; ++o) tt[k][o] = A.dl[lt.off[k][o] + NN * (k + 3 * own3[k])];
         }
+        int gnb[3][P];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int o = 0; o < P; ++o) gnb[k][o] = lt.off[k][o] * GREC + 6 * k;
         for (int e = er.begin; e < er.end; e += er.stride) {
             const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
             __builtin_amdgcn_s_waitcnt(kWaitVm0);
@@ -1014,36 +1038,42 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
                 for (int d = 0; d < 3; ++d) u[d] = T(ud[d]);
                 Tk = T(Td);
                 muf = T(sutherland<double>(A.mu_factor, Td));
-                if (act) s1[L::F_T * NT + a] = Tk;
+                if (act) s1[(own + 10) >> 1] = V2{Tk, T(0)};
             }
             lds_barrier(); // B3
             // grad u, grad rho, grad T
             T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            T dr[3], dT[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                T du[3];
+                const T c0 = lt.ds[k];
+                T du[3] = {c0 * u[0], c0 * u[1], c0 * u[2]};
+                T vr = c0 * rho, vt = c0 * Tk;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) du[i] = line(s1 + (L::F_U + i) * NT, k, u[i]);
+                for (int o = 0; o < P; ++o) {
+                    const T c = lt.d[k][o];
+                    const V2 x01 = LD2(nb[k][o] + 0), x2r = LD2(nb[k][o] + 2);
+                    const T xt = LD1(nb[k][o] + 10);
+                    du[0] += c * x01.x;
+                    du[1] += c * x01.y;
+                    du[2] += c * x2r.x;
+                    vr += c * x2r.y;
+                    vt += c * xt;
+                }
+                dr[k] = vr;
+                dT[k] = vt;
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) gU[i][j] += He[j][k] * du[i];
             }
             T gR[3], gt[3];
-            {
-                T dr[3], dT[3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    dr[k] = line(s1 + L::F_RHO * NT, k, rho);
-                    dT[k] = line(s1 + L::F_T * NT, k, Tk);
-                }
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    gR[i] = He[i][0] * dr[0] + He[i][1] * dr[1] + He[i][2] * dr[2];
-                    gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
-                }
+            for (int i = 0; i < 3; ++i) {
+                gR[i] = He[i][0] * dr[0] + He[i][1] * dr[1] + He[i][2] * dr[2];
+                gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
             }
-            // viscous fluxes as contravariant components G_k(F) = gpvol sum_d He(d,k) F_d
+            // contravariant fluxes G_k(F) = gpvol sum_d He(d,k) F_d into the flux record
             {
                 const T mut = muf + mue;
                 const T divU = gU[0][0] + gU[1][1] + gU[2][2];
@@ -1055,20 +1085,21 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
                     for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) tau[i][i] -= tdiv;
+                const int g = a * GREC;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
-                    T tuk = 0;
+                    T gr[3], tuk = 0;
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
-                        const T g = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
-                        if (act) s2[(9 + 3 * r + k) * NT + a] = g;
-                        tuk += g * u[r]; // tau is symmetric: contravariant (tau u)_k
+                        gr[r] = h0 * tau[r][0] + h1 * tau[r][1] + h2 * tau[r][2];
+                        tuk += gr[r] * u[r]; // tau is symmetric: contravariant (tau u)_k
                     }
                     if (act) {
-                        s2[(6 + k) * NT + a] = tuk;
-                        s2[(0 + k) * NT + a] = h0 * gR[0] + h1 * gR[1] + h2 * gR[2];
-                        s2[(3 + k) * NT + a] = h0 * gt[0] + h1 * gt[1] + h2 * gt[2];
+                        s2[(g + 6 * k + 0) >> 1] = V2{h0 * gR[0] + h1 * gR[1] + h2 * gR[2],
+                                                    h0 * gt[0] + h1 * gt[1] + h2 * gt[2]};
+                        s2[(g + 6 * k + 2) >> 1] = V2{tuk, gr[0]};
+                        s2[(g + 6 * k + 4) >> 1] = V2{gr[1], gr[2]};
                     }
                 }
             }
@@ -1077,24 +1108,28 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             // pass 2: weak divergence sum_k sum_b dl(b,k,I_k(a)) G_k(F)(b)
-            T W6[6];
-#pragma unroll 1
-            for (int fl = 0; fl < 6; ++fl) {
-                T acc = 0;
+            T W6[6] = {0, 0, 0, 0, 0, 0};
+            {
+                const int g = a * GREC;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const T *sf = s2 + (3 * fl + k) * NT;
-                    T v = lt.ds[k] * sf[a];
+                    const T c0 = lt.ds[k];
+                    const V2 o01 = GL2(g + 6 * k), o23 = GL2(g + 6 * k + 2), o45 = GL2(g + 6 * k + 4);
+                    T v[6] = {c0 * o01.x, c0 * o01.y, c0 * o23.x, c0 * o23.y, c0 * o45.x, c0 * o45.y};
 #pragma unroll
-                    for (int o = 0; o < P; ++o) v += tt[k][o] * sf[lt.off[k][o]];
-                    acc += v;
+                    for (int o = 0; o < P; ++o) {
+                        const T c = tt[k][o];
+                        const V2 x01 = GL2(gnb[k][o]), x23 = GL2(gnb[k][o] + 2), x45 = GL2(gnb[k][o] + 4);
+                        v[0] += c * x01.x;
+                        v[1] += c * x01.y;
+                        v[2] += c * x23.x;
+                        v[3] += c * x23.y;
+                        v[4] += c * x45.x;
+                        v[5] += c * x45.y;
+                    }
+#pragma unroll
+                    for (int f = 0; f < 6; ++f) W6[f] += v[f];
                 }
-                if (fl == 0) W6[0] = acc;
-                else if (fl == 1) W6[1] = acc;
-                else if (fl == 2) W6[2] = acc;
-                else if (fl == 3) W6[3] = acc;
-                else if (fl == 4) W6[4] = acc;
-                else W6[5] = acc;
             }
             const T nu_e = A.c_rho * mue / rho;
             const T kappa = muf * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr;
@@ -1102,11 +1137,9 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             if (act) {
-                s2[0 * NT + a] = nu_e * W6[0];
-                s2[1 * NT + a] = W6[3];
-                s2[2 * NT + a] = W6[4];
-                s2[3 * NT + a] = W6[5];
-                s2[4 * NT + a] = W6[2] + kappa * W6[1];
+                s2[3 * a] = V2{nu_e * W6[0], W6[3]};
+                s2[3 * a + 1] = V2{W6[4], W6[5]};
+                s2[3 * a + 2] = V2{W6[2] + kappa * W6[1], T(0)};
             }
             lds_barrier(); // B4
             cur = nxt;
@@ -1114,6 +1147,10 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
         }
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
+#undef LD1
+#undef SPLIT_PAIR
+#undef LD2
+#undef GL2
 }
 
 template <typename T>
@@ -1311,17 +1348,20 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 #if SODHIP_SPLIT_RHS
     // FP64: one wave per operator (a single wave holding both spills);
     // FP32 operators fit one wave without spills
-    if (sizeof(T) == 8 && (porder == 2 || porder == 3)) {
-        const size_t nn = (size_t)nnode_of(porder);
-        A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
-                  (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
-        const int grid = elem_grid(ctx, m->nelem, 16);
-        if (porder == 2)
-            k_rhs_split<2, T><<<grid, 128, 0, ctx->stream>>>(A);
-        else
+    bool done = false;
+    if constexpr (sizeof(T) == 8) {
+        if (porder == 3) { // at p=2 a single wave holds both operators with fewer spills
+            const size_t nn = (size_t)nnode_of(porder);
+            A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
+                      (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
+            const int grid = elem_grid(ctx, m->nelem, 16);
             k_rhs_split<3, T><<<grid, 128, 0, ctx->stream>>>(A);
-        if (hipGetLastError() != hipSuccess) rc = set_error(SODHIP_ERR_HIP, "k_rhs_split launch failed");
-    } else
+            if (hipGetLastError() != hipSuccess)
+                rc = set_error(SODHIP_ERR_HIP, "k_rhs_split launch failed");
+            done = true;
+        }
+    }
+    if (!done)
 #endif
         rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 16));
     prof_end(ctx, tok);
