@@ -5,15 +5,16 @@
 
 namespace sodhip {
 
-// Workgroup barrier that orders LDS accesses only.  __syncthreads also waits
-// for every outstanding global load of the wave (its workgroup fence covers
-// global memory), which would drain the prefetch of the next element; the
-// kernels exchange data between lanes through LDS alone.
+// Workgroup barrier for data exchanged through LDS by ordinary LDS stores:
+// the wave's own LDS operations complete (lgkmcnt), then all waves meet.  The
+// asm is a compiler barrier for memory accesses.  Neither __syncthreads nor a
+// workgroup fence on LDS is used: the former also waits for every
+// outstanding global load, the latter for every direct-to-LDS copy in flight,
+// which would drain the prefetch of the next elements.  Data written by
+// direct-to-LDS copies is read only after an explicit vmcnt wait.
 __device__ __forceinline__ void lds_barrier()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // s_waitcnt immediates (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] |

@@ -1346,11 +1346,12 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     int tok = prof_begin(ctx, 0);
     int rc = SODHIP_OK;
 #if SODHIP_SPLIT_RHS
-    // FP64: one wave per operator (a single wave holding both spills);
-    // FP32 operators fit one wave without spills
+    // p=3: one wave per operator (in FP64 a single wave holding both spills;
+    // in FP32 the split is also faster); at p=2 a single wave holds both
+    // operators with fewer spills, p=4 needs two waves per operator
     bool done = false;
-    if constexpr (sizeof(T) == 8) {
-        if (porder == 3) { // at p=2 a single wave holds both operators with fewer spills
+    {
+        if (porder == 3) {
             const size_t nn = (size_t)nnode_of(porder);
             A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
                       (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;

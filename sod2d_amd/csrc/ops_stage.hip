@@ -45,6 +45,12 @@ __global__ __launch_bounds__(256) void k_gather_update(
     const int n = lpoin_w[i] - 1;
     const int j0 = wrow[i], j1 = wrow[i + 1];
     const int *__restrict__ src = wsrc;
+    // the node's state is loaded before the slots, so that its latency overlaps
+    // the slot-index -> partial chain
+    const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
+    const double m = invMl[n];
+    const double b0 = rho1[n], b1 = q1[n], b2 = q1[n1], b3 = q1[n2], b4 = E1[n];
+    const double y0 = rho[n], y1 = q[n], y2 = q[n1], y3 = q[n2], y4 = E[n];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // four slots per step, all loads issued before the (in-order) additions
     for (int j = j0; j < j1; j += 4) {
@@ -63,15 +69,13 @@ __global__ __launch_bounds__(256) void k_gather_update(
                 for (int f = 0; f < 5; ++f) acc[f] += fact * v[u][f];
             }
     }
-    const double m = invMl[n];
-    const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
     const double rm = acc[0] * m, r0 = acc[1] * m, r1 = acc[2] * m, r2 = acc[3] * m,
                  re = acc[4] * m;
-    rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * rm;
-    q[n] = lam1 * q1[n] + lams * q[n] + gdt * r0;
-    q[n1] = lam1 * q1[n1] + lams * q[n1] + gdt * r1;
-    q[n2] = lam1 * q1[n2] + lams * q[n2] + gdt * r2;
-    E[n] = lam1 * E1[n] + lams * E[n] + gdt * re;
+    rho[n] = lam1 * b0 + lams * y0 + gdt * rm;
+    q[n] = lam1 * b1 + lams * y1 + gdt * r0;
+    q[n1] = lam1 * b2 + lams * y2 + gdt * r1;
+    q[n2] = lam1 * b3 + lams * y3 + gdt * r2;
+    E[n] = lam1 * b4 + lams * y4 + gdt * re;
     if (Rmass) {
         Rmass[n] = rm;
         Rmom[n] = r0;
