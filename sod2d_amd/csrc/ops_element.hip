@@ -38,6 +38,17 @@
 #ifndef SODHIP_SPLIT_RHS
 #define SODHIP_SPLIT_RHS 1
 #endif
+#ifndef SODHIP_SPLIT_WAVES_F32
+#define SODHIP_SPLIT_WAVES_F32 2
+#endif
+// ring depth (elements in flight per block) in FP64 and FP32 operators: in
+// FP64 two keep four blocks per CU within 160 KB of LDS
+#ifndef SODHIP_SPLIT_AHEAD64
+#define SODHIP_SPLIT_AHEAD64 2
+#endif
+#ifndef SODHIP_SPLIT_AHEAD32
+#define SODHIP_SPLIT_AHEAD32 3
+#endif
 
 namespace sodhip {
 
@@ -712,25 +723,49 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 // Fused-stage element kernel with the work of each element split over two
 // waves that share its LDS: wave 0 evaluates the split-form convection, wave 1
 // the viscous operator and hands its residuals to wave 0 through LDS; wave 0
-// writes their sum.  Each wave holds only its own operator's live state, which
+// puts their sum back into the same LDS slots and wave 1 stores it during the
+// next element.  Each wave holds only its own operator's live state, which
 // keeps both free of register spills in FP64 at p=3 (a single wave doing both
-// spills, and its spill reloads would wait for the prefetch of the next
-// element).  Requires (p+1)^3 <= 64 (one wave per operator).
+// spills).  Requires (p+1)^3 == 64: one wave per operator, one lane per node.
+//
+// The inputs stream through an LDS ring AH elements deep.  Every input of
+// element e (He and gpvol, the five conserved fields at its nodes, mu_e) is
+// copied global -> LDS by direct-to-LDS loads, which hold no registers while
+// in flight, AH elements before e is evaluated; its connectivity row arrives
+// AH elements earlier still, since the field addresses are read from it.
+// The copies are issued through inline assembly, outside the compiler's wait
+// tracking (which cannot tell ring slots apart and would wait for every copy
+// in flight before any read of the ring), and each wave waits for its own
+// copies with an explicit vector-memory count: a wave issues the same number
+// NOPS of copies per element, always in the same order, and loads complete in
+// issue order, so element e's copies are complete once at most (AH-1)*NOPS of
+// the wave's operations are outstanding.  The wave's stores count as well and
+// can only make that wait longer.
 //
 // Per element, both waves pass four LDS-only barriers:
-//   B1  geometry of e copied to LDS and state of e loaded (prefetched)
-//   B2  geometry read into registers: the copy of the next element may start
+//   B1  inputs of e in ring slot e%AH (each wave waited for its own copies)
+//   B2  slot read into registers: the copies of element e+AH may start
 //   B3  nodal fields of e staged
 //   B4  viscous residuals of e handed over
-template <int P, typename T> struct SplitLDS {
+template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
+    static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
+    // geometry, He then gpvol, copied in 64-lane rows of GW bytes per lane
     static constexpr int HE_BYTES = 9 * NN * (int)sizeof(T), GV_BYTES = NN * (int)sizeof(T);
-    static constexpr int ROWS16 = (HE_BYTES + GV_BYTES + 1023) / 1024;
-    static constexpr int ROWS4 = (HE_BYTES + GV_BYTES + 255) / 256;
-    static constexpr int GEO = ROWS16 * 1024 > ROWS4 * 256 ? ROWS16 * 1024 : ROWS4 * 256;
-    // nodal fields: rho, u1..3, T, p, H, rho*H, q1..3
-    static constexpr int F_RHO = 0, F_U = 1, F_T = 4, F_P = 5, F_H = 6, F_RH = 7, F_Q = 8,
-                         NF1 = 11;
+    static constexpr int GW = G16 ? 16 : 4, GROW = 64 * GW;
+    static constexpr int GROWS = (HE_BYTES + GV_BYTES + GROW - 1) / GROW, GEO = GROWS * GROW;
+    // conserved fields rho, q1, q2, q3, E (FP64, node-major), then mu_e
+    static constexpr int ST = 5 * NN * 8, MU = NN * (int)sizeof(T);
+    static constexpr int SLOT = GEO + ST + MU;
+    // connectivity rows: AH + 1 in use at a time
+    static constexpr int CROWS = AH + 1, CROW = NN * 4;
+    static constexpr int BYTES = AH * SLOT + CROWS * CROW;
+    static constexpr int MU_OPS = (int)sizeof(T) / 4;
+    // copies per element: wave 0 takes the connectivity row, the even
+    // geometry rows and rho, q1, q2; wave 1 the odd rows, q3, E and mu_e
+    static constexpr int NOPS0 = 1 + (GROWS + 1) / 2 + 6, NOPS1 = GROWS / 2 + 4 + MU_OPS;
+    static_assert(SLOT % 16 == 0, "ring slots must stay 16-byte aligned");
+    static_assert(AH >= 2 && AH * NOPS0 <= 63 && AH * NOPS1 <= 63, "vmcnt range");
 };
 
 template <int P, typename T> struct SplitState {
@@ -738,32 +773,101 @@ template <int P, typename T> struct SplitState {
     T mue;
 };
 
-template <int P, typename T>
-__device__ __forceinline__ void split_load(const RhsArgs<T> &A, SplitState<P, T> &x, int e, int n,
-                                           int a, bool visc)
+// direct-to-LDS copy of W (4 or 16) bytes per lane from g to lds + W*lane
+// (lds wave-uniform), issued outside the compiler's wait tracking
+template <int W> __device__ __forceinline__ void dma_lds(const void *g, unsigned lds)
 {
-    constexpr int NN = SplitLDS<P, T>::NN;
-    x.cr = A.cr[n];
-    x.cq[0] = A.cq[n];
-    x.cq[1] = A.cq[n + A.npoin];
-    x.cq[2] = A.cq[n + 2 * (size_t)A.npoin];
-    x.cE = A.cE[n];
-    (void)NN;
-    if (visc) x.mue = A.mu_e[e + (size_t)A.nelem * a];
+    const unsigned m = __builtin_amdgcn_readfirstlane(lds);
+    if constexpr (W == 16)
+        asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "{m0}"(m) : "memory");
+    else
+        asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "{m0}"(m) : "memory");
 }
 
-// geometry of element e (both waves: alternate rows) and the state of node n
-template <int P, typename T>
-__device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned char *geo, int e,
-                                            int wl, int role)
+// wait until at most N of this wave's vector-memory operations are outstanding
+template <int N> __device__ __forceinline__ void wait_vm()
 {
-    using L = SplitLDS<P, T>;
-    const unsigned char *he = (const unsigned char *)(A.He + (size_t)e * 9 * L::NN);
-    const unsigned char *gv = (const unsigned char *)(A.gpvol + (size_t)e * L::NN);
-    if (A.geo16)
-        geo_copy<16, L::HE_BYTES, L::GV_BYTES, 2>(he, gv, geo, wl, role);
-    else
-        geo_copy<4, L::HE_BYTES, L::GV_BYTES, 2>(he, gv, geo, wl, role);
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | kWaitVm0 | ((N >> 4) << 14));
+}
+
+// Copies of element e into the ring slot at LDS address slot and, by wave 0,
+// the connectivity row of element ec to LDS address crow.  n0 and n1 are the
+// global nodes of e at lexicographic (I fastest) positions lane/2 and
+// 32 + lane/2: a 64-lane row of a field moves both 4-byte halves of 32 nodes,
+// so the field lands as doubles in lexicographic node order, and on a mesh
+// numbered along I each row reads 8 runs of p+1 consecutive nodes.
+template <int P, typename T, bool G16, int ROLE>
+__device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned slot, unsigned crow,
+                                            int e, int ec, int n0, int n1, int lane)
+{
+    using L = SplitLDS<P, T, G16>;
+    constexpr int NN = L::NN;
+    const int hw = 4 * (lane & 1), nd = lane >> 1;
+    if constexpr (ROLE == 0) dma_lds<4>(A.connecT + (size_t)ec * NN + lane, crow);
+    const unsigned char *he = (const unsigned char *)(A.He + (size_t)e * 9 * NN);
+    const unsigned char *gv = (const unsigned char *)(A.gpvol + (size_t)e * NN);
+#pragma unroll
+    for (int r = ROLE; r < L::GROWS; r += 2) {
+        // lanes past the end of gpvol re-read its last GW bytes
+        const int byte = r * L::GROW + lane * L::GW;
+        const unsigned char *src =
+            byte < L::HE_BYTES ? he + byte : gv + min(byte - L::HE_BYTES, L::GV_BYTES - L::GW);
+        dma_lds<L::GW>(src, slot + r * L::GROW);
+    }
+    const size_t np = (size_t)A.npoin;
+    const double *fld[5] = {A.cr, A.cq, A.cq + np, A.cq + 2 * np, A.cE};
+    const unsigned st = slot + L::GEO;
+#pragma unroll
+    for (int f = ROLE == 0 ? 0 : 3; f < (ROLE == 0 ? 3 : 5); ++f) {
+        dma_lds<4>((const char *)(fld[f] + n0) + hw, st + f * NN * 8);
+        dma_lds<4>((const char *)(fld[f] + n1) + hw, st + f * NN * 8 + 256);
+    }
+    if constexpr (ROLE == 1) {
+        const unsigned mu = st + L::ST;
+        if constexpr (sizeof(T) == 8) {
+            dma_lds<4>((const char *)(A.mu_e + e + (size_t)A.nelem * nd) + hw, mu);
+            dma_lds<4>((const char *)(A.mu_e + e + (size_t)A.nelem * (32 + nd)) + hw, mu + 256);
+        } else {
+            dma_lds<4>(A.mu_e + e + (size_t)A.nelem * lane, mu);
+        }
+    }
+}
+
+// reads ring slot sl into registers: He, gpvol, the conserved fields and mu_e
+// of node a (lexicographic position pa)
+template <int P, typename T, bool G16>
+__device__ __forceinline__ void split_read(const unsigned char *sl, int a, int pa, bool visc,
+                                           T He[3][3], T &vol, SplitState<P, T> &x)
+{
+    using L = SplitLDS<P, T, G16>;
+    constexpr int NN = L::NN;
+    const T *hp = (const T *)sl + 9 * a;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
+    vol = ((const T *)sl)[9 * NN + a];
+    const double *st = (const double *)(sl + L::GEO);
+    x.cr = st[pa];
+    x.cq[0] = st[NN + pa];
+    x.cq[1] = st[2 * NN + pa];
+    x.cq[2] = st[3 * NN + pa];
+    x.cE = st[4 * NN + pa];
+    if (visc) x.mue = ((const T *)(sl + L::GEO + L::ST))[a];
+}
+
+// residual records of element e (5 values per node, lexicographic node
+// order) from their dense copy in LDS, 16 contiguous bytes per lane
+template <int P, typename T>
+__device__ __forceinline__ void split_store(const RhsArgs<T> &A, const void *dense, int e, int lane)
+{
+    constexpr int NN = (P + 1) * (P + 1) * (P + 1), CH = 5 * NN * (int)sizeof(T) / 16;
+    using U4 = unsigned __attribute__((ext_vector_type(4)));
+    const U4 *src = (const U4 *)dense;
+    U4 *o = (U4 *)(A.out + (size_t)e * 5 * NN);
+#pragma unroll
+    for (int q = lane; q < CH; q += 64) o[q] = src[q];
 }
 
 // The two-wave kernel.  The staged nodal fields and the viscous fluxes are
@@ -798,42 +902,85 @@ __device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned char *
             DIV1 += w1;                                                                    \
         }                                                                                  \
     } while (0)
-template <int P, typename T>
-__global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T> A)
+template <int P, typename T, bool G16>
+__global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_SPLIT_WAVES_F32) void k_rhs_split(RhsArgs<T> A)
 {
-    using L = SplitLDS<P, T>;
+    using L = SplitLDS<P, T, G16>;
     using V2 = T __attribute__((ext_vector_type(2)));
-    constexpr int NN = L::NN, REC = 12, GREC = 18;
-    static_assert(NN <= 64, "one wave per operator");
+    // node records of REC values: 112 bytes in FP64, an odd multiple of 16,
+    // so that 16 lanes reading 16 consecutive records with 16-byte reads hit
+    // distinct banks (96-byte records would pair them up); 144-byte flux records
+    constexpr int NN = L::NN, REC = sizeof(T) == 8 ? 14 : 12, GREC = 18, AH = L::AH;
+    static_assert(NN == 64, "one lane per node");
     __shared__ V2 s1[64 * REC / 2]; // records of REC values as pairs
     __shared__ V2 s2[64 * GREC / 2];
-    __shared__ __attribute__((aligned(16))) unsigned char geo[L::GEO];
+    __shared__ __attribute__((aligned(16))) unsigned char ring[L::BYTES];
+    const unsigned ring_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)ring;
 
     const int role = threadIdx.x >> 6; // 0: convection, 1: viscous (wave-uniform)
     const int lane = threadIdx.x & 63;
-    const bool act = lane < NN;
-    const int a = act ? lane : 0;
+    const bool act = true;
     const bool visc = role == 1;
 
     const ElemRange er = elem_range(A.nelem);
     if (er.begin >= er.end) return;
     const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
+    const int nit = (er.end - er.begin + er.stride - 1) / er.stride;
+    // element of iteration t; past the block's range the last element is
+    // copied again, which keeps the copy stream unconditional
+    auto elem_of = [&](int t) {
+        return (int)min((long)er.begin + (long)t * er.stride, (long)elast);
+    };
+    auto slot_of = [&](int t) { return (t % AH) * L::SLOT; };
+    auto crow_of = [&](int t) { return AH * L::SLOT + (t % L::CROWS) * L::CROW; };
+    const int nd = lane >> 1;
 
+    // Lane L evaluates the node a at lexicographic position L (I fastest);
+    // the LDS records are kept in that order.  Along every line direction
+    // the 16 lanes of an LDS access phase then read 16 distinct records
+    // (a permutation of 16 consecutive ones).  ia0, ia1: the nodes at
+    // lexicographic positions nd and 32 + nd.
+    auto lex = [&](int r) { return min(max(A.perm[r], 0), NN - 1); };
+    int a, ia0, ia1;
+    {
+        int *inv = (int *)s1;
+        if (role == 0) inv[lex(lane)] = lane;
+        lds_barrier();
+        a = min(max(inv[lane], 0), NN - 1);
+        ia0 = min(max(inv[nd], 0), NN - 1);
+        ia1 = min(max(inv[32 + nd], 0), NN - 1);
+        lds_barrier();
+    }
+    const int pa = lane;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
-    const int pa = min(max(A.perm[a], 0), NN - 1);
     // neighbour records (element offsets into s1)
     int nb[3][P];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int o = 0; o < P; ++o) nb[k][o] = lt.off[k][o] * REC;
-    const int own = a * REC;
+        for (int o = 0; o < P; ++o) nb[k][o] = lex(lt.off[k][o]) * REC;
+    const int own = pa * REC;
 
-    SplitState<P, T> cur, nxt;
-    split_issue<P, T>(A, geo, er.begin, lane, role);
-    split_load<P, T>(A, cur, er.begin, A.connecT[(size_t)er.begin * NN + a], a, visc);
-    int n_nx = A.connecT[(size_t)min(er.begin + er.stride, elast) * NN + a];
+    // the connectivity of the first AH elements, then the first AH elements
+    // in the ring in the order of the main loop
+    if (role == 0)
+        for (int t = 0; t < AH; ++t)
+            dma_lds<4>(A.connecT + (size_t)elem_of(t) * NN + lane, ring_lds + crow_of(t));
+    wait_vm<0>();
+    lds_barrier();
+    for (int t = 0; t < AH; ++t) {
+        const int *c = (const int *)(ring + crow_of(t));
+        const int n0 = c[ia0], n1 = c[ia1];
+        if (visc)
+            split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
+                                      elem_of(t + AH), n0, n1, lane);
+        else
+            split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
+                                      elem_of(t + AH), n0, n1, lane);
+        lds_barrier(); // the row read here is rewritten in a later pass
+    }
+    SplitState<P, T> cur;
 
     auto prim = [&](const SplitState<P, T> &c, double &r, double ud[3], double &pd, double &hd,
                     double &Td, double &umag) {
@@ -857,22 +1004,18 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
 
     if (!visc) {
         // ======================= convection wave
-        for (int e = er.begin; e < er.end; e += er.stride) {
-            const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        for (int t = 0; t < nit; ++t) {
+            wait_vm<(AH - 1) * L::NOPS0>();
             lds_barrier(); // B1
-            T He[3][3];
-            const T *hp = (const T *)geo + 9 * a;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
-            const T vol = ((const T *)geo)[9 * NN + a];
+            T He[3][3], vol;
+            split_read<P, T, G16>(ring + slot_of(t), a, pa, false, He, vol, cur);
             __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
             lds_barrier(); // B2
-            split_issue<P, T>(A, geo, e2, lane, role);
-            split_load<P, T>(A, nxt, e2, n_nx, a, false);
-            const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
+            {
+                const int *c = (const int *)(ring + crow_of(t + AH));
+                split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
+                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
+            }
             T rho, u[3], q[3], H, rhoH;
             {
                 double r, ud[3], pd, hd, Td, umag;
@@ -984,17 +1127,18 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * (u[i] * divQ + q[i] * divUw);
             Re += T(0.25) * (H * divQ + rhoH * divUw);
             lds_barrier(); // B4: the viscous residuals are in s2
+            // the summed residuals replace them in s2 as dense records in
+            // lexicographic node order (all lanes read before any writes); the
+            // viscous wave stores them to HBM during the next element
             if (act) {
-                const V2 h01 = s2[3 * a], h23 = s2[3 * a + 1], h4 = s2[3 * a + 2];
-                T *o = A.out + 5 * ((size_t)e * NN + pa);
-                o[0] = Rm * vol + h01.x;
-                o[1] = Rq[0] * vol + h01.y;
-                o[2] = Rq[1] * vol + h23.x;
-                o[3] = Rq[2] * vol + h23.y;
-                o[4] = Re * vol + h4.x;
+                const V2 h01 = s2[3 * pa], h23 = s2[3 * pa + 1], h4 = s2[3 * pa + 2];
+                T *d = (T *)s2 + 5 * pa;
+                d[0] = Rm * vol + h01.x;
+                d[1] = Rq[0] * vol + h01.y;
+                d[2] = Rq[1] * vol + h23.x;
+                d[3] = Rq[2] * vol + h23.y;
+                d[4] = Re * vol + h4.x;
             }
-            cur = nxt;
-            n_nx = n_nx2;
         }
     } else {
         // ======================= viscous wave
@@ -1011,23 +1155,22 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int o = 0; o < P; ++o) gnb[k][o] = lt.off[k][o] * GREC + 6 * k;
-        for (int e = er.begin; e < er.end; e += er.stride) {
-            const int e2 = min(e + er.stride, elast), e3 = min(e + 2 * er.stride, elast);
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            for (int o = 0; o < P; ++o) gnb[k][o] = lex(lt.off[k][o]) * GREC + 6 * k;
+        for (int t = 0; t < nit; ++t) {
+            wait_vm<(AH - 1) * L::NOPS1>();
             lds_barrier(); // B1
-            T He[3][3];
-            const T *hp = (const T *)geo + 9 * a;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) He[i][j] = hp[i + 3 * j];
-            const T vol = ((const T *)geo)[9 * NN + a];
+            T He[3][3], vol;
+            split_read<P, T, G16>(ring + slot_of(t), a, pa, true, He, vol, cur);
             __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
             lds_barrier(); // B2
-            split_issue<P, T>(A, geo, e2, lane, role);
-            split_load<P, T>(A, nxt, e2, n_nx, a, true);
-            const int n_nx2 = A.connecT[(size_t)e3 * NN + a];
+            {
+                const int *c = (const int *)(ring + crow_of(t + AH));
+                split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
+                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
+            }
+            // residual records of the previous element, left in s2 by the
+            // convection wave: their stores complete while this element runs
+            if (t != 0) split_store<P, T>(A, s2, er.begin + (t - 1) * er.stride, lane);
             T rho, u[3], Tk, muf;
             const T mue = cur.mue;
             {
@@ -1085,7 +1228,7 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
                     for (int j = 0; j < 3; ++j) tau[i][j] = mut * (gU[i][j] + gU[j][i]);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) tau[i][i] -= tdiv;
-                const int g = a * GREC;
+                const int g = pa * GREC;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const T h0 = vol * He[0][k], h1 = vol * He[1][k], h2 = vol * He[2][k];
@@ -1110,7 +1253,7 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             // pass 2: weak divergence sum_k sum_b dl(b,k,I_k(a)) G_k(F)(b)
             T W6[6] = {0, 0, 0, 0, 0, 0};
             {
-                const int g = a * GREC;
+                const int g = pa * GREC;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const T c0 = lt.ds[k];
@@ -1137,16 +1280,18 @@ __global__ __launch_bounds__(128, SODHIP_SPLIT_WAVES) void k_rhs_split(RhsArgs<T
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             if (act) {
-                s2[3 * a] = V2{nu_e * W6[0], W6[3]};
-                s2[3 * a + 1] = V2{W6[4], W6[5]};
-                s2[3 * a + 2] = V2{W6[2] + kappa * W6[1], T(0)};
+                s2[3 * pa] = V2{nu_e * W6[0], W6[3]};
+                s2[3 * pa + 1] = V2{W6[4], W6[5]};
+                s2[3 * pa + 2] = V2{W6[2] + kappa * W6[1], T(0)};
             }
             lds_barrier(); // B4
-            cur = nxt;
-            n_nx = n_nx2;
         }
     }
-    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    // the last element's residual records
+    lds_barrier();
+    if (visc) split_store<P, T>(A, s2, elast, lane);
+    // no copy may still be writing this block's LDS when it retires
+    wait_vm<0>();
 #undef LD1
 #undef SPLIT_PAIR
 #undef LD2
@@ -1356,7 +1501,10 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
             A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
                       (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
             const int grid = elem_grid(ctx, m->nelem, 16);
-            k_rhs_split<3, T><<<grid, 128, 0, ctx->stream>>>(A);
+            if (A.geo16)
+                k_rhs_split<3, T, true><<<grid, 128, 0, ctx->stream>>>(A);
+            else
+                k_rhs_split<3, T, false><<<grid, 128, 0, ctx->stream>>>(A);
             if (hipGetLastError() != hipSuccess)
                 rc = set_error(SODHIP_ERR_HIP, "k_rhs_split launch failed");
             done = true;
