@@ -213,8 +213,21 @@ class TGVRun:
         """One time step: evalTimeIteration's level copy, rk_4_ls_main, time += dt,
         then evalDt in adaptive mode."""
         L, G, S = self.L, self.G, self.S
-        for lv in (self.rho, self.q, self.E, self.u, self.pr, self.Tem, self.e_int):
-            lv[0].copy_(lv[1])
+        # level 2 -> level 1.  The primitive levels are exchanged rather than
+        # copied: level 2 of u, pr, Tem and e_int is rewritten by the EOS
+        # before anything reads it, and on nodes outside lpoin_w both levels
+        # keep their initial values.  With the reference sequencing stage 1
+        # reads y^n from level 1 only (lam1 = 1, lams = 0 gives the values
+        # of lam1 = 0, lams = 1 on equal levels), so the conserved levels are
+        # exchanged too; otherwise stage 1 evaluates its RHS at level 2 and
+        # they are copied.
+        for lv in (self.rho, self.q, self.E):
+            if self.parity:
+                lv[0], lv[1] = lv[1], lv[0]
+            else:
+                lv[0].copy_(lv[1])
+        for lv in (self.u, self.pr, self.Tem, self.e_int):
+            lv[0], lv[1] = lv[1], lv[0]
         n = self.npoin
         self.eta[:n].copy_(self.eta[n:])
         h = self.ctx.h
@@ -231,7 +244,7 @@ class TGVRun:
         if self.parity:
             # stage 1 with the kept residual (time_integ_ls.f90:323-331)
             check(lib().sodhip_rk_stage_update_f64(
-                h, n, self.npoin_w, _ptr(self.lpoin_w), None, self.dt, 0.0, 1.0, G[1, 0],
+                h, n, self.npoin_w, _ptr(self.lpoin_w), None, self.dt, 1.0, 0.0, G[1, 0],
                 _ptr(self.rho[0]), _ptr(self.q[0]), _ptr(self.E[0]), _ptr(self.rho[1]),
                 _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.Rmass), _ptr(self.Rmom),
                 _ptr(self.Rener)), "rk_stage_update")
