@@ -742,11 +742,12 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 // the wave's operations are outstanding.  The wave's stores count as well and
 // can only make that wait longer.
 //
-// Per element, both waves pass four LDS-only barriers:
-//   B1  inputs of e in ring slot e%AH (each wave waited for its own copies)
-//   B2  slot read into registers: the copies of element e+AH may start
-//   B3  nodal fields of e staged
-//   B4  viscous residuals of e handed over
+// Per element, both waves pass two LDS-only barriers:
+//   B1  inputs of e in ring slot e%AH (each wave waited for its own copies),
+//       viscous residuals of the previous element handed over in LDS; wave 0
+//       then completes that element's residual records
+//   B2  nodal fields of e staged, slot e%AH read into registers (the copies
+//       of element e+AH may start), previous element's records complete
 template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
     static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
@@ -982,6 +983,11 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     }
     SplitState<P, T> cur;
 
+    // reciprocals of the gas constant and of Pr (one rounding apart from the
+    // divisions of the reference's expressions)
+    const double rRgas = 1.0 / A.Rgas;
+    const T cp_pr = A.Cp / A.Pr;
+    double rinv = 0.0;
     auto prim = [&](const SplitState<P, T> &c, double &r, double ud[3], double &pd, double &hd,
                     double &Td, double &umag) {
         r = c.cr;
@@ -995,7 +1001,8 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
         const double eint = c.cE * ri - 0.5 * umag;
         pd = r * (A.gamma - 1.0) * eint;
         hd = (A.gamma / (A.gamma - 1.0)) * pd * ri;
-        Td = pd * ri / A.Rgas;
+        Td = pd * ri * rRgas;
+        rinv = ri;
     };
         // indices are even value offsets into the records
 #define LD1(i) (s1[(i) >> 1].x)
@@ -1004,18 +1011,26 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
 
     if (!visc) {
         // ======================= convection wave
+        // convection residuals (times gpvol) of the previous element
+        T pend[5] = {0, 0, 0, 0, 0};
+        // its records: pend plus the viscous residuals handed over in s2, as
+        // dense records in lexicographic node order over the same LDS (all
+        // lanes read before any writes); the viscous wave stores them to HBM
+        auto finish = [&]() {
+            const V2 h01 = s2[3 * pa], h23 = s2[3 * pa + 1], h4 = s2[3 * pa + 2];
+            T *d = (T *)s2 + 5 * pa;
+            d[0] = pend[0] + h01.x;
+            d[1] = pend[1] + h01.y;
+            d[2] = pend[2] + h23.x;
+            d[3] = pend[3] + h23.y;
+            d[4] = pend[4] + h4.x;
+        };
         for (int t = 0; t < nit; ++t) {
             wait_vm<(AH - 1) * L::NOPS0>();
             lds_barrier(); // B1
+            if (t != 0) finish();
             T He[3][3], vol;
             split_read<P, T, G16>(ring + slot_of(t), a, pa, false, He, vol, cur);
-            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-            lds_barrier(); // B2
-            {
-                const int *c = (const int *)(ring + crow_of(t + AH));
-                split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
-                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
-            }
             T rho, u[3], q[3], H, rhoH;
             {
                 double r, ud[3], pd, hd, Td, umag;
@@ -1036,7 +1051,12 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                     s1[(own + 8) >> 1] = V2{q[1], q[2]};
                 }
             }
-            lds_barrier(); // B3
+            lds_barrier(); // B2
+            {
+                const int *c = (const int *)(ring + crow_of(t + AH));
+                split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
+                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
+            }
 
             T Rm = 0, Rq[3] = {0, 0, 0}, Re = 0;
             T Ut[3];
@@ -1126,20 +1146,16 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
 #pragma unroll
             for (int i = 0; i < 3; ++i) Rq[i] += T(0.25) * (u[i] * divQ + q[i] * divUw);
             Re += T(0.25) * (H * divQ + rhoH * divUw);
-            lds_barrier(); // B4: the viscous residuals are in s2
-            // the summed residuals replace them in s2 as dense records in
-            // lexicographic node order (all lanes read before any writes); the
-            // viscous wave stores them to HBM during the next element
-            if (act) {
-                const V2 h01 = s2[3 * pa], h23 = s2[3 * pa + 1], h4 = s2[3 * pa + 2];
-                T *d = (T *)s2 + 5 * pa;
-                d[0] = Rm * vol + h01.x;
-                d[1] = Rq[0] * vol + h01.y;
-                d[2] = Rq[1] * vol + h23.x;
-                d[3] = Rq[2] * vol + h23.y;
-                d[4] = Re * vol + h4.x;
-            }
+            pend[0] = Rm * vol;
+            pend[1] = Rq[0] * vol;
+            pend[2] = Rq[1] * vol;
+            pend[3] = Rq[2] * vol;
+            pend[4] = Re * vol;
         }
+        // the last element's records
+        lds_barrier();
+        finish();
+        lds_barrier();
     } else {
         // ======================= viscous wave
         T tt[3][P];
@@ -1161,16 +1177,6 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             lds_barrier(); // B1
             T He[3][3], vol;
             split_read<P, T, G16>(ring + slot_of(t), a, pa, true, He, vol, cur);
-            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-            lds_barrier(); // B2
-            {
-                const int *c = (const int *)(ring + crow_of(t + AH));
-                split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
-                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
-            }
-            // residual records of the previous element, left in s2 by the
-            // convection wave: their stores complete while this element runs
-            if (t != 0) split_store<P, T>(A, s2, er.begin + (t - 1) * er.stride, lane);
             T rho, u[3], Tk, muf;
             const T mue = cur.mue;
             {
@@ -1183,7 +1189,16 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                 muf = T(sutherland<double>(A.mu_factor, Td));
                 if (act) s1[(own + 10) >> 1] = V2{Tk, T(0)};
             }
-            lds_barrier(); // B3
+            lds_barrier(); // B2
+            {
+                const int *c = (const int *)(ring + crow_of(t + AH));
+                split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
+                                          elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
+            }
+            // residual records of the previous element, completed in s2 by the
+            // convection wave: their stores complete while this element runs
+            // (read here, before this wave's flux records overwrite s2)
+            if (t != 0) split_store<P, T>(A, s2, er.begin + (t - 1) * er.stride, lane);
             // grad u, grad rho, grad T
             T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             T dr[3], dT[3];
@@ -1274,8 +1289,8 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                     for (int f = 0; f < 6; ++f) W6[f] += v[f];
                 }
             }
-            const T nu_e = A.c_rho * mue / rho;
-            const T kappa = muf * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr;
+            const T nu_e = A.c_rho * mue * T(rinv);
+            const T kappa = (muf + A.c_ener * mue) * cp_pr;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
@@ -1284,12 +1299,12 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                 s2[3 * pa + 1] = V2{W6[4], W6[5]};
                 s2[3 * pa + 2] = V2{W6[2] + kappa * W6[1], T(0)};
             }
-            lds_barrier(); // B4
         }
+        // the last element's records, completed by the convection wave
+        lds_barrier();
+        lds_barrier();
+        split_store<P, T>(A, s2, elast, lane);
     }
-    // the last element's residual records
-    lds_barrier();
-    if (visc) split_store<P, T>(A, s2, elast, lane);
     // no copy may still be writing this block's LDS when it retires
     wait_vm<0>();
 #undef LD1
