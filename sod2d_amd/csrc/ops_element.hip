@@ -43,6 +43,12 @@
 #endif
 // ring depth (elements in flight per block) in FP64 and FP32 operators: in
 // FP64 two keep four blocks per CU within 160 KB of LDS
+// consecutive elements a block evaluates in a row (runs of one block, XCD-
+// aware over runs): they share the 128-byte lines and pages of the element-
+// fastest mu_e(nelem,ngaus)
+#ifndef SODHIP_SPLIT_RUN
+#define SODHIP_SPLIT_RUN 4
+#endif
 #ifndef SODHIP_SPLIT_AHEAD64
 #define SODHIP_SPLIT_AHEAD64 2
 #endif
@@ -756,12 +762,23 @@ template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int GW = G16 ? 16 : 4, GROW = 64 * GW;
     static constexpr int GROWS = (HE_BYTES + GV_BYTES + GROW - 1) / GROW, GEO = GROWS * GROW;
     // conserved fields rho, q1, q2, q3, E (FP64, node-major), then mu_e
-    static constexpr int ST = 5 * NN * 8, MU = NN * (int)sizeof(T);
+    // mu_e(nelem,ngaus) is element-fastest: the 64 values of one element lie
+    // nelem apart.  In FP64 it is copied per run of RUN consecutive elements,
+    // RUN values per node, into one of three blocks, two runs ahead (each
+    // 64-lane row then reads 8 runs of 32 contiguous bytes instead of 32
+    // scattered doubles); with FP32 operators per element into its slot.
+    static constexpr int RUN = SODHIP_SPLIT_RUN;
+    static constexpr bool MUBLK = sizeof(T) == 8;
+    static constexpr int ST = 5 * NN * 8, MU = MUBLK ? 0 : NN * (int)sizeof(T);
     static constexpr int SLOT = GEO + ST + MU;
     // connectivity rows: AH + 1 in use at a time
     static constexpr int CROWS = AH + 1, CROW = NN * 4;
-    static constexpr int BYTES = AH * SLOT + CROWS * CROW;
+    static constexpr int MUB = RUN * NN * (int)sizeof(T), MUBS = MUBLK ? 3 : 0;
+    static constexpr int MUB_AT = AH * SLOT + CROWS * CROW;
+    static constexpr int BYTES = MUB_AT + MUBS * MUB;
     static constexpr int MU_OPS = (int)sizeof(T) / 4;
+    static_assert(!MUBLK || (RUN == 4 && MUB == 256 * MU_OPS * RUN),
+                  "one run's mu_e block in MU_OPS rows per element");
     // copies per element: wave 0 takes the connectivity row, the even
     // geometry rows and rho, q1, q2; wave 1 the odd rows, q3, E and mu_e
     static constexpr int NOPS0 = 1 + (GROWS + 1) / 2 + 6, NOPS1 = GROWS / 2 + 4 + MU_OPS;
@@ -824,7 +841,7 @@ __device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned slot, 
         dma_lds<4>((const char *)(fld[f] + n0) + hw, st + f * NN * 8);
         dma_lds<4>((const char *)(fld[f] + n1) + hw, st + f * NN * 8 + 256);
     }
-    if constexpr (ROLE == 1) {
+    if constexpr (ROLE == 1 && !L::MUBLK) {
         const unsigned mu = st + L::ST;
         if constexpr (sizeof(T) == 8) {
             dma_lds<4>((const char *)(A.mu_e + e + (size_t)A.nelem * nd) + hw, mu);
@@ -832,6 +849,23 @@ __device__ __forceinline__ void split_issue(const RhsArgs<T> &A, unsigned slot, 
         } else {
             dma_lds<4>(A.mu_e + e + (size_t)A.nelem * lane, mu);
         }
+    }
+}
+
+// FP64 mu_e copies of one iteration: rows 2j and 2j+1 of the block of the run
+// starting at element rfirst (j = iteration within its run).  The block is
+// node-major and element-fastest: lane L moves half L&1 of
+// mu_e(e0 + (L/2)&3, 8i + L/8) of row i.
+template <typename T>
+__device__ __forceinline__ void mu_issue(const RhsArgs<T> &A, unsigned blk, int rfirst, int j,
+                                         int lane)
+{
+    const int hw = 4 * (lane & 1), el = (lane >> 1) & 3, nd8 = lane >> 3;
+    const int e = min(rfirst + el, A.nelem - 1);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * j + h;
+        dma_lds<4>((const char *)(A.mu_e + e + (size_t)A.nelem * (8 * i + nd8)) + hw, blk + i * 256);
     }
 }
 
@@ -855,7 +889,8 @@ __device__ __forceinline__ void split_read(const unsigned char *sl, int a, int p
     x.cq[1] = st[2 * NN + pa];
     x.cq[2] = st[3 * NN + pa];
     x.cE = st[4 * NN + pa];
-    if (visc) x.mue = ((const T *)(sl + L::GEO + L::ST))[a];
+    if constexpr (!L::MUBLK)
+        if (visc) x.mue = ((const T *)(sl + L::GEO + L::ST))[a];
 }
 
 // residual records of element e (5 values per node, lexicographic node
@@ -923,14 +958,31 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     const bool act = true;
     const bool visc = role == 1;
 
-    const ElemRange er = elem_range(A.nelem);
+    constexpr int RUN = SODHIP_SPLIT_RUN;
+    const ElemRange er = elem_range((A.nelem + RUN - 1) / RUN); // over runs
     if (er.begin >= er.end) return;
-    const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
-    const int nit = (er.end - er.begin + er.stride - 1) / er.stride;
-    // element of iteration t; past the block's range the last element is
-    // copied again, which keeps the copy stream unconditional
+    const int rlast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
+    const int nit = RUN * ((er.end - er.begin + er.stride - 1) / er.stride);
+    // element of iteration t.  Past the block's last run, and past nelem in
+    // the last run, elements repeat: the copies stay unconditional and a
+    // repeated element stores the same records again.
     auto elem_of = [&](int t) {
-        return (int)min((long)er.begin + (long)t * er.stride, (long)elast);
+        const long r = min((long)er.begin + (long)(t / RUN) * er.stride, (long)rlast);
+        return (int)min(r * RUN + t % RUN, (long)A.nelem - 1);
+    };
+    const int elast = elem_of(nit - 1);
+    // FP64 mu_e blocks: iteration t (t >= -2 RUN) copies its share of the
+    // block of run t/RUN + 2 (floor); the block of run r is read from
+    // iteration RUN r on
+    auto mu_at = [&](int t) {
+        if constexpr (L::MUBLK) {
+            const int r = (t >> 2) + 2;
+            const long g = min((long)er.begin + (long)r * er.stride, (long)rlast);
+            mu_issue<T>(A, ring_lds + L::MUB_AT + (r % 3) * L::MUB, (int)(g * RUN), t & 3, lane);
+        }
+    };
+    auto mu_read = [&](int t, int node) {
+        return ((const T *)(ring + L::MUB_AT + ((t >> 2) % 3) * L::MUB))[node * RUN + (t & 3)];
     };
     auto slot_of = [&](int t) { return (t % AH) * L::SLOT; };
     auto crow_of = [&](int t) { return AH * L::SLOT + (t % L::CROWS) * L::CROW; };
@@ -968,15 +1020,18 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     if (role == 0)
         for (int t = 0; t < AH; ++t)
             dma_lds<4>(A.connecT + (size_t)elem_of(t) * NN + lane, ring_lds + crow_of(t));
+    else
+        for (int t = -2 * RUN; t < -AH; ++t) mu_at(t);
     wait_vm<0>();
     lds_barrier();
     for (int t = 0; t < AH; ++t) {
         const int *c = (const int *)(ring + crow_of(t));
         const int n0 = c[ia0], n1 = c[ia1];
-        if (visc)
+        if (visc) {
             split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
                                       elem_of(t + AH), n0, n1, lane);
-        else
+            mu_at(t - AH);
+        } else
             split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
                                       elem_of(t + AH), n0, n1, lane);
         lds_barrier(); // the row read here is rewritten in a later pass
@@ -1177,6 +1232,7 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             lds_barrier(); // B1
             T He[3][3], vol;
             split_read<P, T, G16>(ring + slot_of(t), a, pa, true, He, vol, cur);
+            if constexpr (L::MUBLK) cur.mue = mu_read(t, a);
             T rho, u[3], Tk, muf;
             const T mue = cur.mue;
             {
@@ -1194,11 +1250,12 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                 const int *c = (const int *)(ring + crow_of(t + AH));
                 split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
                                           elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
+                mu_at(t);
             }
             // residual records of the previous element, completed in s2 by the
             // convection wave: their stores complete while this element runs
             // (read here, before this wave's flux records overwrite s2)
-            if (t != 0) split_store<P, T>(A, s2, er.begin + (t - 1) * er.stride, lane);
+            if (t != 0) split_store<P, T>(A, s2, elem_of(t - 1), lane);
             // grad u, grad rho, grad T
             T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             T dr[3], dT[3];
