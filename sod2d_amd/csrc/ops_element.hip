@@ -748,12 +748,15 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 // the wave's operations are outstanding.  The wave's stores count as well and
 // can only make that wait longer.
 //
-// Per element, both waves pass two LDS-only barriers:
+// Per element, both waves pass three LDS-only barriers:
 //   B1  inputs of e in ring slot e%AH (each wave waited for its own copies),
 //       viscous residuals of the previous element handed over in LDS; wave 0
 //       then completes that element's residual records
 //   B2  nodal fields of e staged, slot e%AH read into registers (the copies
 //       of element e+AH may start), previous element's records complete
+//   B3  line derivatives of u and rho handed over: wave 0 forms them for the
+//       convection terms, and wave 1 takes them for grad u and grad rho
+//       instead of forming the same sums again
 template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
     static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
@@ -947,6 +950,12 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     // so that 16 lanes reading 16 consecutive records with 16-byte reads hit
     // distinct banks (96-byte records would pair them up); 144-byte flux records
     constexpr int NN = L::NN, REC = sizeof(T) == 8 ? 14 : 12, GREC = 18, AH = L::AH;
+    // s2 holds, in turn: the viscous flux records (GREC values per node) and
+    // the handed-over viscous residuals (3 pairs per node), then the dense
+    // residual records (5 values per node) followed by the line derivatives
+    // of u and rho (6 pairs per node) from DREC on
+    constexpr int DREC = 5 * NN / 2;
+    static_assert(DREC + 6 * NN <= 64 * GREC / 2, "derivative records fit in s2");
     static_assert(NN == 64, "one lane per node");
     __shared__ V2 s1[64 * REC / 2]; // records of REC values as pairs
     __shared__ V2 s2[64 * GREC / 2];
@@ -1151,12 +1160,15 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                         for (int i = 0; i < 3; ++i) divFuu[i] += w * ub[i];
                     }
                     drho[k] = dr;
+                    s2[DREC + 6 * pa + 2 * k] = V2{du[0], du[1]};
+                    s2[DREC + 6 * pa + 2 * k + 1] = V2{du[2], dr};
 #pragma unroll
                     for (int i = 0; i < 3; ++i) Rq[i] += Qt[k] * du[i];
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) Rq[i] = T(0.25) * (Rq[i] + rho * divFuu[i]);
             }
+            lds_barrier(); // B3
             {
                 const T urho = Ut[0] * drho[0] + Ut[1] * drho[1] + Ut[2] * drho[2];
                 Rm += T(0.5) * urho;
@@ -1256,27 +1268,24 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             // convection wave: their stores complete while this element runs
             // (read here, before this wave's flux records overwrite s2)
             if (t != 0) split_store<P, T>(A, s2, elem_of(t - 1), lane);
-            // grad u, grad rho, grad T
-            T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-            T dr[3], dT[3];
+            // grad T from the T lines; grad u and grad rho from the line
+            // derivatives handed over by the convection wave
+            T dT[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const T c0 = lt.ds[k];
-                T du[3] = {c0 * u[0], c0 * u[1], c0 * u[2]};
-                T vr = c0 * rho, vt = c0 * Tk;
+                T vt = lt.ds[k] * Tk;
 #pragma unroll
-                for (int o = 0; o < P; ++o) {
-                    const T c = lt.d[k][o];
-                    const V2 x01 = LD2(nb[k][o] + 0), x2r = LD2(nb[k][o] + 2);
-                    const T xt = LD1(nb[k][o] + 10);
-                    du[0] += c * x01.x;
-                    du[1] += c * x01.y;
-                    du[2] += c * x2r.x;
-                    vr += c * x2r.y;
-                    vt += c * xt;
-                }
-                dr[k] = vr;
+                for (int o = 0; o < P; ++o) vt += lt.d[k][o] * LD1(nb[k][o] + 10);
                 dT[k] = vt;
+            }
+            lds_barrier(); // B3
+            T gU[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            T dr[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const V2 d01 = s2[DREC + 6 * pa + 2 * k], d2r = s2[DREC + 6 * pa + 2 * k + 1];
+                const T du[3] = {d01.x, d01.y, d2r.x};
+                dr[k] = d2r.y;
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
