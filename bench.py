@@ -92,6 +92,103 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
                       f"{dt * 1e3:.2f} ms each, OpenMP element loop on {cores} threads"}
 
 
+def run_operators(a) -> dict:
+    """The isolated-operator workload: full_convec_ijk, full_diffusion_ijk and
+    smart_visc_spectral once each on an n^3-element structured box (2,097,152
+    hexahedra at n=128), node positions jittered per axis by U(-0.05, 0.05) x
+    the local GLL spacing, fields from case.operator_bench_fields; median of 20
+    calls after 3 warm-ups.  Algorithmic bytes per call (N nodes, EN
+    element-nodes, s operator bytes): convection (4+10s)EN + 9sN read and 5sN
+    written; diffusion (4+12s)EN + 7sN read and 5sN written (outputs
+    initialised, not accumulated); sensor 7sN + (4+2s)EN (FP64)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    from sod2d_amd import case
+    from sod2d_amd import device as D
+    from sod2d_amd.device import to_dev
+
+    p, n = a.porder, a.op_n
+    torch.cuda.set_device(0)
+    t_setup = time.perf_counter()
+    T = case.basis_tables(p)
+    m = case.box_mesh(p, n, periodic=False, jitter=0.05, seed=20261202)
+    He, gpvol = case.elem_jacobian(p, m.connec_orig, m.coord)
+    helem, helem_l = case.char_length(p, m.connec_orig, m.coord, spectral=(p == 3))
+    if helem_l is None:
+        helem_l = np.asfortranarray(np.repeat(helem[:, None], T.nnode, axis=1))
+    del m.coord
+    f = case.operator_bench_fields(m.npoin, m.nelem, T.nnode, seed=20261202 + p)
+    dt = torch.float64 if a.precision == 8 else torch.float32
+    dv = lambda x: to_dev(x).to(dt)  # noqa: E731
+    dcon = to_dev(m.connec)
+    tabs = [to_dev(x) for x in (T.invAtoIJK, T.gmshAtoI, T.gmshAtoJ, T.gmshAtoK)]
+    dHe, dgv, ddl = dv(He), dv(gpvol), dv(T.dlxigp_ip)
+    fl = {k: dv(f[k]) for k in ("u", "q", "rho", "pr", "h", "Tem", "mu_fluid", "mu_e", "mu_sgs")}
+    del He, gpvol
+    rng = np.random.default_rng(20261202)
+    Reta = to_dev(rng.normal(0, 1.0, m.npoin) * 10.0 ** rng.uniform(-2, 4, m.npoin))
+    eta = to_dev(rng.normal(0, 1.0, m.npoin))
+    rho64, u64, T64 = (to_dev(f[k]) for k in ("rho", "u", "Tem"))
+    dhl, dhk, dlw = to_dev(helem_l), to_dev(helem), to_dev(m.lpoin_w)
+    NE, NP, nn = m.nelem, m.npoin, T.nnode
+    R = [torch.zeros(k, dtype=dt, device="cuda") for k in (NP, 3 * NP, NP)]
+    mu = torch.zeros(NE * nn, dtype=torch.float64, device="cuda")
+    ml = torch.zeros_like(mu)
+    prm = D.visc_params(287.0, flag_normalise_entropy=1)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    def convec():
+        D.full_convec_ijk(NE, NP, dcon, None, None, dHe, dgv, ddl, None, *tabs,
+                          *[fl[k] for k in ("u", "q", "rho", "pr", "h")], *R, True, -1.0)
+
+    def diffusion():
+        D.full_diffusion_ijk(NE, NP, dcon, None, dHe, dgv, ddl, *tabs, 1004.0, 0.71, fl["rho"],
+                             fl["rho"], fl["u"], fl["Tem"], fl["mu_fluid"], fl["mu_e"],
+                             fl["mu_sgs"], None, *R, True, -1.0)
+
+    def sensor():
+        D.smart_visc_spectral(NE, NP, m.npoin_w, dcon, dlw, Reta, None, None, None, None, None,
+                              1.4, rho64, u64, None, T64, eta, dhl, dhk, None, mu, tabs[0], None,
+                              None, None, ml, prm)
+
+    s_op = 8 if a.precision == 8 else 4
+    EN, N = NE * nn, NP
+    ops = {"full_convec_ijk": (convec, (4 + 10 * s_op) * EN + 9 * s_op * N + 5 * s_op * N),
+           "full_diffusion_ijk": (diffusion, (4 + 12 * s_op) * EN + 7 * s_op * N + 5 * s_op * N),
+           "smart_visc_spectral": (sensor, 7 * 8 * N + (4 + 2 * 8) * EN)}
+    res, total = {}, 0.0
+    for name, (fn, B) in ops.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        med = float(np.median(ts))
+        total += med
+        res[name] = {"ms": round(med * 1e3, 4), "algorithmic_bytes": int(B),
+                     "GB_s": round(B / med / 1e9, 1), "frac": round(B / med / 1e9 / HBM_PEAK_GBS, 4)}
+    dom = max(res, key=lambda k: res[k]["ms"])
+    return {
+        "metric": "isolated operators: ms per call of full_convec_ijk + full_diffusion_ijk + "
+                  "smart_visc_spectral",
+        "value": round(total * 1e3, 4), "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64" if a.precision == 8 else "f32 (sensor f64)", "data": "synthetic",
+        "config": {"workload": f"structured {n}^3-element box, p={p}, jitter 5% of local GLL "
+                               f"spacing, seed 20261202", "porder": p, "elements": NE,
+                   "nodes": NP, "element_nodes": EN, "setup_s": round(t_setup, 1)},
+        "per_operator": res,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": res[dom]["GB_s"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": res[dom]["frac"],
+                     "traffic": None},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +201,11 @@ def main(argv=None):
     ap.add_argument("--clean", action="store_true",
                     help="fresh RHS at every stage instead of the reference's sequencing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("tgv", "operators"), default="tgv",
+                    help="tgv: the TGV time step (default); operators: the isolated "
+                         "full_convec_ijk / full_diffusion_ijk / smart_visc_spectral calls")
+    ap.add_argument("--op-n", type=int, default=128,
+                    help="elements per direction of the operator workload's box")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run 3 plain FP64 copy kernels (y = 1.0 * x) of "
                          "MIB MiB each: the reference launches for calibrating HBM byte counters")
@@ -115,6 +217,9 @@ def main(argv=None):
               f"WORLD_SIZE={world}); the single-GPU figure is the whole-job value.",
               file=sys.stderr)
         return 2
+    if a.workload == "operators":
+        print(json.dumps(run_operators(a)), flush=True)
+        return 0
 
     import torch
     sys.path.insert(0, ROOT)

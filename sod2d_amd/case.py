@@ -146,3 +146,26 @@ def build_case(porder: int, n, periodic: bool = True, lengths=None, jitter: floa
     Ml, invMl = lumped_mass(porder, m.connec, gpvol, m.npoin, m.lpoin_w)
     helem, helem_l = char_length(porder, m.connec_orig, m.coord, spectral_lengths)
     return Geometry(T, m, He, gpvol, Ml, invMl, helem, helem_l, float(gpvol.sum(dtype=np.float64)))
+
+
+def operator_bench_fields(npoin: int, nelem: int, nnode: int, seed: int = 1, gamma: float = 1.4,
+                          Rgas: float = 1.0) -> dict:
+    """Synthetic inputs of the isolated-operator workload, in the reference
+    layouts: rho ~ U(0.9, 1.1), u_i ~ N(0, 0.3), p ~ U(0.9, 1.1)/gamma, E
+    consistent with them, the E slot of full_convec_ijk as the enthalpy
+    h = gamma p / ((gamma - 1) rho) (time_integ_ls.f90:584), T = p/(rho R),
+    mu_fluid ~ U(1e-4, 2e-4), mu_e ~ U(0, 1e-3) and mu_sgs ~ U(0, 1e-4) per
+    element-GP.  Drawn in that order from one seeded generator."""
+    r = np.random.default_rng(seed)
+    rho = r.uniform(0.9, 1.1, npoin)
+    u = np.asfortranarray(r.normal(0.0, 0.3, (npoin, 3)))
+    pr = r.uniform(0.9, 1.1, npoin) / gamma
+    q = np.asfortranarray(rho[:, None] * u)
+    h = gamma / (gamma - 1.0) * pr / rho
+    Tem = pr / (rho * Rgas)
+    mu_fluid = r.uniform(1e-4, 2e-4, npoin)
+    mu_e = np.asfortranarray(r.uniform(0.0, 1e-3, (nelem, nnode)))
+    mu_sgs = np.asfortranarray(r.uniform(0.0, 1e-4, (nelem, nnode)))
+    E = pr / (gamma - 1.0) + 0.5 * rho * (u * u).sum(1)
+    return dict(rho=rho, u=u, q=q, pr=pr, h=h, E=E, Tem=Tem, mu_fluid=mu_fluid, mu_e=mu_e,
+                mu_sgs=mu_sgs)

@@ -1,0 +1,186 @@
+"""The isolated-operator workload at its full size: full_convec_ijk,
+full_diffusion_ijk and smart_visc_spectral on a 128^3-element structured box
+(2,097,152 hexahedra) at p=3 (HEX64) and p=4 (HEX125), node positions
+jittered per axis by U(-0.05, 0.05) x the local GLL spacing (the +-10 % of
+the workload's description folds one element of 2.1 M at p=3 for this seed,
+so the Jacobian-positivity requirement fixes +-5 %), fields from
+case.operator_bench_fields.
+
+The device runs every element of the full mesh.  The oracle cannot evaluate
+2 M elements in a test's time, so it evaluates blocks of 6^3 elements (a
+corner, the centre and an off-centre block) of the same mesh and the
+comparison is made where a block determines the result alone: the assembled
+residuals at the nodes strictly inside a block (all their elements are in
+the block) and mu_e / mue_l of the block's elements (the entropy
+normalisation still runs over every working node).  Full-size properties:
+two launches give bitwise-identical residuals, and the FP32 entry points stay
+within 2e-5 normwise of the FP64 ones over the whole mesh.
+
+Bar: FP64 <= 1e-13 normwise relative to the oracle on each block; FP32 <= 2e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from sod2d_amd import case
+
+from helpers import fz, relerr
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+JITTER = 0.05
+SEED = 20261202
+BLK = 6
+BLOCKS = [(0, 0, 0), (N // 2 - BLK // 2,) * 3, (17, 90, 41)]
+
+
+def _dev(a, dtype=None):
+    from sod2d_amd.device import to_dev
+    t = to_dev(a)
+    return t.to(dtype) if dtype is not None and t.dtype.is_floating_point else t
+
+
+@pytest.fixture(scope="module", params=[3, 4])
+def c3(request):
+    p = request.param
+    T = case.basis_tables(p)
+    m = case.box_mesh(p, N, periodic=False, jitter=JITTER, seed=SEED)
+    He, gpvol = case.elem_jacobian(p, m.connec_orig, m.coord)
+    # element lengths: corner-edge minimum; per node the spectral minimum at
+    # p=3, the element's value at p=4 (an input of the sensor only)
+    if p == 3:
+        helem, helem_l = case.char_length(p, m.connec_orig, m.coord, spectral=True)
+    else:
+        helem, _ = case.char_length(p, m.connec_orig, m.coord, spectral=False)
+        helem_l = np.asfortranarray(np.repeat(helem[:, None], T.nnode, axis=1))
+    s = case.operator_bench_fields(m.npoin, m.nelem, T.nnode, seed=SEED + p)
+    del m.coord
+    g = dict(p=p, T=T, m=m, He=He, gpvol=gpvol, helem=helem, helem_l=helem_l, s=s)
+    # device copies, uploaded once; FP32 copies made on the device
+    d64 = {"He": _dev(He), "gpvol": _dev(gpvol), "dl": _dev(T.dlxigp_ip),
+           "rho_n": _dev(s["rho"] * 1.01)}
+    for k in ("u", "q", "rho", "pr", "h", "Tem", "mu_fluid", "mu_e", "mu_sgs"):
+        d64[k] = _dev(s[k])
+    g["d"] = {torch.float64: d64}
+    g["dconnec"] = _dev(m.connec)
+    g["dtabs"] = [_dev(x) for x in (T.invAtoIJK, T.gmshAtoI, T.gmshAtoJ, T.gmshAtoK)]
+    yield g
+    g.clear()
+    torch.cuda.empty_cache()
+
+
+def _fields(g, dtype):
+    if dtype not in g["d"]:
+        g["d"][dtype] = {k: v.to(dtype) for k, v in g["d"][torch.float64].items()}
+    return g["d"][dtype]
+
+
+def _block(p, b0):
+    """Elements of the BLK^3 block at element offset b0 and the global nodes
+    strictly inside it (0-based)."""
+    ox, oy, oz = b0
+    ex, ey, ez = np.meshgrid(np.arange(ox, ox + BLK), np.arange(oy, oy + BLK),
+                             np.arange(oz, oz + BLK), indexing="ij")
+    elems = (ex + N * (ey + N * ez)).ravel()
+    NX = N * p + 1
+    I, J, K = np.meshgrid(np.arange(ox * p + 1, (ox + BLK) * p), np.arange(oy * p + 1, (oy + BLK) * p),
+                          np.arange(oz * p + 1, (oz + BLK) * p), indexing="ij")
+    nodes = (I + NX * (J + NX * K)).ravel()
+    return np.sort(elems), np.sort(nodes)
+
+
+def _sub(g, elems):
+    return dict(connec=np.asfortranarray(g["m"].connec[elems, :]),
+                He=np.asfortranarray(g["He"][..., elems]),
+                gpvol=np.asfortranarray(g["gpvol"][..., elems]))
+
+
+def _convec(g, dtype):
+    from sod2d_amd import device as D
+    m, f = g["m"], _fields(g, dtype)
+    R = [torch.zeros(k, dtype=dtype, device="cuda") for k in (m.npoin, 3 * m.npoin, m.npoin)]
+    D.full_convec_ijk(m.nelem, m.npoin, g["dconnec"], None, None, f["He"], f["gpvol"], f["dl"],
+                      None, *g["dtabs"], *[f[k] for k in ("u", "q", "rho", "pr", "h")], *R, True,
+                      -1.0)
+    torch.cuda.synchronize()
+    return R
+
+
+def _diffusion(g, dtype, Cp=1004.0, Pr=0.71):
+    from sod2d_amd import device as D
+    m, f = g["m"], _fields(g, dtype)
+    R = [torch.zeros(k, dtype=dtype, device="cuda") for k in (m.npoin, 3 * m.npoin, m.npoin)]
+    D.full_diffusion_ijk(m.nelem, m.npoin, g["dconnec"], None, f["He"], f["gpvol"], f["dl"],
+                         *g["dtabs"], Cp, Pr, f["rho_n"], f["rho"], f["u"], f["Tem"],
+                         f["mu_fluid"], f["mu_e"], f["mu_sgs"], None, *R, True, -1.0)
+    torch.cuda.synchronize()
+    return R
+
+
+def test_c3_convec_diffusion(c3, oracle):
+    g = c3
+    p, m, s = g["p"], g["m"], g["s"]
+    print(f"C3 p={p}: {m.nelem} elements, {m.npoin} nodes", flush=True)
+    OT = oracle.Tables(p)
+    Rc = [x.cpu().numpy() for x in _convec(g, torch.float64)]
+    Rc2 = [x.cpu().numpy() for x in _convec(g, torch.float64)]
+    for a, b in zip(Rc, Rc2):
+        assert np.array_equal(a, b), "convection residuals differ between two launches"
+    Rd = [x.cpu().numpy() for x in _diffusion(g, torch.float64)]
+    npo = m.npoin
+    for b0 in BLOCKS:
+        elems, nodes = _block(p, b0)
+        sub = _sub(g, elems)
+        oc = oracle.full_convec_ijk(OT, sub["connec"], sub["He"], sub["gpvol"], s["u"], s["q"],
+                                    s["rho"], s["pr"], s["h"], fact=-1.0)
+        od = oracle.full_diffusion_ijk(OT, sub["connec"], sub["He"], sub["gpvol"], 1004.0, 0.71,
+                                       s["rho"] * 1.01, s["rho"], s["u"], s["Tem"], s["mu_fluid"],
+                                       np.asfortranarray(s["mu_e"][elems, :]),
+                                       np.asfortranarray(s["mu_sgs"][elems, :]), fact=-1.0)
+        for dev, orc, what in ((Rc, oc, "convec"), (Rd, od, "diffusion")):
+            qn = np.concatenate([nodes + d * npo for d in range(3)])
+            assert relerr(dev[0][nodes], orc[0][nodes]) < 1e-13, (what, b0, "mass")
+            assert relerr(dev[1][qn], orc[1].ravel(order="F")[qn]) < 1e-13, (what, b0, "mom")
+            assert relerr(dev[2][nodes], orc[2][nodes]) < 1e-13, (what, b0, "ener")
+    # FP32 entry points against the FP64 device result over the whole mesh
+    Rc32 = [x.double().cpu().numpy() for x in _convec(g, torch.float32)]
+    Rd32 = [x.double().cpu().numpy() for x in _diffusion(g, torch.float32)]
+    for a, b in zip(Rc32 + Rd32, Rc + Rd):
+        assert relerr(a, b) < 2e-5
+
+
+def test_c3_smart_visc_spectral(c3, oracle):
+    from sod2d_amd import device as D
+    g = c3
+    p, m, T = g["p"], g["m"], g["T"]
+    g["d"].pop(torch.float32, None)
+    ne, n, nn = m.nelem, m.npoin, T.nnode
+    r = np.random.default_rng(SEED + 10 * p)
+    rho = r.uniform(0.9, 1.1, n)
+    u = np.asfortranarray(r.normal(0, 0.3, (n, 3)))
+    Tem = r.uniform(0.9, 1.1, n)
+    eta = r.normal(0, 1.0, n)
+    Reta = r.normal(0, 1.0, n) * 10.0 ** r.uniform(-2, 4, n)
+    gamma, Rgas = 1.4, 287.0
+    prm = D.visc_params(Rgas, flag_normalise_entropy=1)
+    dmu = torch.zeros(ne * nn, dtype=torch.float64, device="cuda")
+    dml = torch.zeros(ne * nn, dtype=torch.float64, device="cuda")
+    D.smart_visc_spectral(ne, n, m.npoin_w, g["dconnec"], _dev(m.lpoin_w), _dev(Reta), None, None,
+                          None, None, None, gamma, _dev(rho), _dev(u), None, _dev(Tem), _dev(eta),
+                          _dev(g["helem_l"]), _dev(g["helem"]), None, dmu, g["dtabs"][0], None,
+                          None, None, dml, prm)
+    torch.cuda.synchronize()
+    mu = dmu.cpu().numpy().reshape((ne, nn), order="F")
+    ml = dml.cpu().numpy().reshape((ne, nn), order="F")
+    OT = oracle.Tables(p)
+    prm_o = oracle.visc_params(Rgas, flag_normalise_entropy=1)
+    for b0 in BLOCKS:
+        elems, _ = _block(p, b0)
+        sub = _sub(g, elems)
+        omu, oml = oracle.smart_visc_spectral(
+            OT, sub["connec"], m.lpoin_w, Reta, gamma, rho, u, Tem, eta,
+            np.asfortranarray(g["helem_l"][elems, :]), np.ascontiguousarray(g["helem"][elems]),
+            prm_o, fz((len(elems), nn)))
+        assert relerr(ml[elems, :], oml) < 1e-13, b0
+        assert relerr(mu[elems, :], omu) < 1e-13, b0
