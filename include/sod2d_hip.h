@@ -192,6 +192,62 @@ int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int np
                                    const double *helem_k, const int *invAtoIJK,
                                    const sodhip_visc_params *prm, double *mu_e, double *mue_l);
 
+/* smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227), the sensor of the
+ * reference's live LS-RK path (time_integ_ls.f90:292,455): per element
+ * beta_e = max rho helem_k (cmax/p)(|u| + c) and mu_e = cglob min(ce rho |Reta| /
+ * norm (helem_k/p)^2, beta_e), no filter; mue_l receives the incoming mu_e.
+ * helem_k is the caller's helem(nelem) (the reference's helem(nelem,nnode)
+ * argument is not read).  With flag_normalise_entropy and flag_high_mach the
+ * norm is the minimum density over the working nodes and prm->ce is set to
+ * ce_comp, as the reference assigns its module global. */
+int sodhip_smart_visc_spectral_imex_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                        int npoin_w, const int *connec, const int *lpoin_w,
+                                        const double *Reta, const double *rho, const double *u,
+                                        const double *csound, const double *eta,
+                                        const double *helem_k, sodhip_visc_params *prm,
+                                        double ce_comp, int flag_high_mach, double *mu_e,
+                                        double *mue_l);
+
+/* eval_divergence (mod_operators.f90:485-570): divU(npoin) = assembled gpvol * div u
+ * (zeroed first). */
+int sodhip_eval_divergence_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                               const int *connec, const double *He, const double *gpvol,
+                               const double *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+                               const int *gmshAtoJ, const int *gmshAtoK, const double *u,
+                               double *divU);
+
+/* compute_vorticity with makeConti (mod_operators.f90:684-775): curlU(npoin,3) =
+ * assembled eps_ijk du_j/dx_k (the reference's leviCivi of CFDSolverBase.f90:
+ * 1605-1611, i.e. minus the curl), divided by lelpn on the working nodes.  The
+ * reference's dNgp, leviCivi and atoIJK arguments are not needed. */
+int sodhip_compute_vorticity_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                 const int *lpoin_w, const int *connec, const int *lelpn,
+                                 const double *He, const double *dlxigp_ip,
+                                 const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                                 const int *gmshAtoK, const double *u, double *curlU);
+
+/* eval_divergence and compute_vorticity of the same u in one element pass (the
+ * pair the live path calls back to back, time_integ_ls.f90:459-460). */
+int sodhip_div_curl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                        const int *lpoin_w, const int *connec, const int *lelpn, const double *He,
+                        const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                        const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                        const double *u, double *divU, double *curlU);
+
+/* ducros_visc_limit (mod_entropy_viscosity.f90:277-314): mu_e(nelem,ngaus) scaled
+ * in place by max(phi - ducros_min_val, 0) phi2 + ducros_min_val with divUl =
+ * divU/Ml, phi = min(4/3 divUl^2/(divUl^2 + |vorti|^2 + eps), 1), phi2 =
+ * divUl^2/(divUl^2 + (|u|/helem)^2 + eps); helem(nelem,nnode) is the caller's
+ * helem_l. */
+int sodhip_ducros_visc_limit_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                 const int *connec, const double *u, const double *divU,
+                                 const double *vorti, const double *Ml, const double *helem,
+                                 double ducros_min_val, double *mu_e);
+
+/* elemPerNode (mod_geom.f90:68-99), host: lelpn(npoin) = incidences of each node
+ * in connec(nelem,nnode). */
+int sodhip_elem_per_node(int nelem, int nnode, int npoin, const int *connec, int *lelpn);
+
 /* lumped_solver_scal_opt / lumped_solver_vect_opt (mod_solver.f90:40-104) */
 int sodhip_lumped_solver_scal_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
                                   const double *invMl, double *R);

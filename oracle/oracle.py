@@ -230,6 +230,57 @@ def smart_visc_spectral(T, connec, lpoin_w, Reta, gamma_gas, rho, u, Tem, eta, h
     return mu_e, mue_l
 
 
+def smart_visc_spectral_imex(T, connec, lpoin_w, Reta, rho, u, csound, eta, helem_k, prm, mu_e,
+                             mue_l=None, flag_high_mach=0):
+    """smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227); mu_e is
+    overwritten, mue_l receives its incoming value; with the normalisation and
+    flag_high_mach, prm.ce becomes prm.ce_comp."""
+    nelem = connec.shape[0]
+    if mue_l is None:
+        mue_l = fzeros(mu_e.shape)
+    lib().or_smart_visc_spectral_imex(T.p, nelem, rho.shape[0], lpoin_w.shape[0], _i(connec),
+                                      _i(lpoin_w), _d(Reta), _d(rho), _d(u), _d(csound), _d(eta),
+                                      _d(helem_k), C.byref(prm), int(flag_high_mach), _d(mu_e),
+                                      _d(mue_l))
+    return mu_e, mue_l
+
+
+def eval_divergence(T, connec, He, gpvol, u, divU=None):
+    """eval_divergence (mod_operators.f90:485-570)."""
+    npoin = u.shape[0]
+    if divU is None:
+        divU = np.zeros(npoin)
+    lib().or_eval_divergence(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
+                             _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ), _i(T.gK),
+                             _d(u), _d(divU))
+    return divU
+
+
+def elem_per_node(connec, npoin):
+    """elemPerNode (mod_geom.f90:68-99): incidences of each node in connec."""
+    lelpn = np.zeros(npoin, dtype=np.int32)
+    lib().or_elem_per_node(connec.shape[0], connec.shape[1], npoin, _i(connec), _i(lelpn))
+    return lelpn
+
+
+def compute_vorticity(T, connec, lpoin_w, lelpn, He, u, curlU=None):
+    """compute_vorticity with makeConti (mod_operators.f90:684-775)."""
+    npoin = u.shape[0]
+    if curlU is None:
+        curlU = fzeros((npoin, 3))
+    lib().or_compute_vorticity(T.p, connec.shape[0], npoin, lpoin_w.shape[0], _i(lpoin_w),
+                               _i(connec), _i(lelpn), _d(He), _d(T.dlxigp_ip), _i(T.invAtoIJK),
+                               _i(T.gI), _i(T.gJ), _i(T.gK), _d(u), _d(curlU))
+    return curlU
+
+
+def ducros_visc_limit(T, connec, u, divU, vorti, Ml, helem, mu_e, ducros_min_val=0.1):
+    """ducros_visc_limit (mod_entropy_viscosity.f90:277-314); mu_e scaled in place."""
+    lib().or_ducros_visc_limit(T.p, connec.shape[0], u.shape[0], _i(connec), _d(u), _d(divU),
+                               _d(vorti), _d(Ml), _d(helem), C.c_double(ducros_min_val), _d(mu_e))
+    return mu_e
+
+
 def thermo_entropy(lpoin_w, gamma_gas, p_inf, rho_inf, rho, pr, u, eta, feta):
     lib().or_thermo_entropy(rho.shape[0], lpoin_w.shape[0], _i(lpoin_w), C.c_double(gamma_gas),
                             C.c_double(p_inf), C.c_double(rho_inf), _d(rho), _d(pr), _d(u),
@@ -333,16 +384,19 @@ def tgv_ic(coord, t):
 class TGVOracle:
     """rk_4_ls_main (time_integ_ls.f90:202-487) with updateF (:489-691) and the
     time loop of CFDSolverBase_evalTimeIteration (CFDSolverBase.f90:1929-2164),
-    restated on the oracle's primitives for the periodic TGV case, with
-    smart_visc_spectral as the entropy-viscosity sensor (the reference's live
-    path calls its _imex variant at the same points).
+    restated on the oracle's primitives for the periodic TGV case.  The
+    entropy-viscosity sensor is smart_visc_spectral (``sensor_path`` "spectral")
+    or the reference's live path (``sensor_path`` "live"):
+    smart_visc_spectral_imex at the first step (time_integ_ls.f90:292) and,
+    after each step, smart_visc_spectral_imex, eval_divergence,
+    compute_vorticity and ducros_visc_limit (:455-463).
 
     ``geo`` provides the mesh and geometry (connec_orig/connec/lpoin_w/coord,
     He, gpvol, invMl, helem, helem_l, vol_tot) and ``ic`` the initial fields;
     ``t`` the TGV constants (an oracle TGVParams); ``cfl`` (conv, diff)."""
 
     def __init__(self, p, geo, ic, t, stages=5, variant=1, parity=True, dt_mode="adaptive",
-                 cfl=(0.5, 0.5), sensor=True):
+                 cfl=(0.5, 0.5), sensor=True, sensor_path="spectral"):
         self.p = p
         self.T = Tables(p)
         self.geo = geo
@@ -357,6 +411,8 @@ class TGVOracle:
         self.dt_mode = dt_mode
         self.cfl = cfl
         self.sensor = sensor
+        self.sensor_path = sensor_path
+        self.lelpn = elem_per_node(m.connec, m.npoin)
         c = lambda a: np.array(a, order="F", copy=True)  # noqa: E731
         self.rho = [c(ic["rho"]), c(ic["rho"])]
         self.q = [c(ic["q"]), c(ic["q"])]
@@ -414,10 +470,21 @@ class TGVOracle:
         lumped_solver_scal(self.lpoin_w, self.geo.invMl, out)
         self.Reta[:, col] = out
 
-    def _sensor(self, Reta, rho, u, Tem, eta):
-        smart_visc_spectral(self.T, self.connec, self.lpoin_w, np.ascontiguousarray(Reta),
-                            self.t.gamma_gas, rho, u, Tem, np.ascontiguousarray(eta),
-                            self.geo.helem_l, self.geo.helem, self.prm, self.mu_e, self.mue_l)
+    def _sensor(self, Reta, rho, u, Tem, eta, ducros=False):
+        if self.sensor_path == "spectral":
+            smart_visc_spectral(self.T, self.connec, self.lpoin_w, np.ascontiguousarray(Reta),
+                                self.t.gamma_gas, rho, u, Tem, np.ascontiguousarray(eta),
+                                self.geo.helem_l, self.geo.helem, self.prm, self.mu_e, self.mue_l)
+            return
+        g = self.geo
+        smart_visc_spectral_imex(self.T, self.connec, self.lpoin_w, np.ascontiguousarray(Reta),
+                                 rho, u, self.csound, np.ascontiguousarray(eta), g.helem,
+                                 self.prm, self.mu_e, self.mue_l)
+        if ducros:
+            divU = eval_divergence(self.T, self.connec, g.He, g.gpvol, u)
+            vorti = compute_vorticity(self.T, self.connec, self.lpoin_w, self.lelpn, g.He, u)
+            ducros_visc_limit(self.T, self.connec, u, divU, vorti, g.Ml, g.helem_l, self.mu_e,
+                              0.1)
 
     def step(self):
         for lv in (self.rho, self.q, self.E, self.u, self.pr, self.Tem, self.e_int):
@@ -452,7 +519,8 @@ class TGVOracle:
             1.0 * (self.eta[lw, 1] - self.eta[lw, 0]) / self.dt
         self.Reta[lw, 0] = self.Reta[lw, 1]
         if self.sensor:
-            self._sensor(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[:, 1])
+            self._sensor(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[:, 1],
+                         ducros=True)
         sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
         self.time += self.dt
         if self.dt_mode == "adaptive":

@@ -879,6 +879,188 @@ void or_smart_visc_spectral(int p, int nelem, int npoin, int npoin_w, const int 
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Live-path sensor: smart_visc_spectral_imex, nodal divergence and         */
+/* vorticity, Ducros limiter                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227).  helem_k is the
+ * caller's helem(nelem) (the helem(nelem,nnode) argument is not read).  mue_l
+ * receives the incoming mu_e.  With the normalisation on and flag_high_mach the
+ * norm is the minimum density over the working nodes and prm->ce becomes
+ * prm->ce_comp, as the reference assigns its module global (:148-166). */
+void or_smart_visc_spectral_imex(int p, int nelem, int npoin, int npoin_w, const int *connec,
+                                 const int *lpoin_w, const double *Reta, const double *rho,
+                                 const double *u, const double *csound, const double *eta,
+                                 const double *helem_k, or_visc_params *prm, int flag_high_mach,
+                                 double *mu_e, double *mue_l)
+{
+    const int n1 = p + 1, nnode = n1 * n1 * n1;
+    const double fact_low_mach = prm->flag_drop_c_in_envit ? 0.0 : 1.0;
+    for (size_t i = 0; i < (size_t)nelem * nnode; ++i) mue_l[i] = mu_e[i];
+    double norm = 1.0;
+    if (prm->flag_normalise_entropy == 1) {
+        if (flag_high_mach) {
+            double rmin = 100000.0;
+            for (int i = 0; i < npoin_w; ++i) rmin = fmin(rmin, rho[lpoin_w[i] - 1]);
+            norm = rmin;
+            prm->ce = prm->ce_comp;
+        } else {
+            double sum = 0.0;
+            for (int i = 0; i < npoin_w; ++i) sum = sum + eta[lpoin_w[i] - 1];
+            const double mean = sum / (double)npoin_w;
+            double dev = 0.0;
+            for (int i = 0; i < npoin_w; ++i) dev = fmax(dev, fabs(eta[lpoin_w[i] - 1] - mean));
+            norm = dev;
+        }
+    }
+    for (int e = 0; e < nelem; ++e) {
+        const double ced = prm->ce;
+        double betae = 0.0;
+        for (int a = 0; a < nnode; ++a) {
+            const int n = CONNEC(e, a);
+            const double u0 = NODAL(u, n, 0), u1 = NODAL(u, n, 1), u2 = NODAL(u, n, 2);
+            const double speed = sqrt(u0 * u0 + u1 * u1 + u2 * u2) + fact_low_mach * csound[n];
+            betae = fmax(betae, (rho[n] * helem_k[e]) * (prm->cmax / (double)p) * speed);
+        }
+        const double hp = helem_k[e] / (double)p;
+        for (int a = 0; a < nnode; ++a) {
+            const int n = CONNEC(e, a);
+            const double R1 = rho[n] * fabs(Reta[n]) / norm;
+            const double Ve = ced * R1 * (hp * hp);
+            ELGP(mu_e, e, a) = prm->cglob * fmin(Ve, betae);
+        }
+    }
+}
+
+/* line derivatives of the velocity at GLL point g of an element whose nodal
+ * velocities are ul: gI[d][k] = d u_d / d xi_k, as the reference's gradIsoU
+ * (summed over the line index ii in order) */
+static void or_grad_iso(int n1, int ngaus, const double *dlxigp_ip, const int *invAtoIJK, int g,
+                        int I, int J, int K, double ul[][3], double gI[3][3])
+{
+    for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < 3; ++k) gI[d][k] = 0.0;
+    for (int ii = 0; ii < n1; ++ii) {
+        const int a1 = INV(ii, J, K), a2 = INV(I, ii, K), a3 = INV(I, J, ii);
+        for (int d = 0; d < 3; ++d) {
+            gI[d][0] = gI[d][0] + DL(g, 0, ii) * ul[a1][d];
+            gI[d][1] = gI[d][1] + DL(g, 1, ii) * ul[a2][d];
+            gI[d][2] = gI[d][2] + DL(g, 2, ii) * ul[a3][d];
+        }
+    }
+}
+
+/* eval_divergence (mod_operators.f90:485-570): divU(n) = sum over the elements
+ * of n of gpvol * div u at n's GLL point, in element order */
+void or_eval_divergence(int p, int nelem, int npoin, const int *connec, const double *He,
+                        const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                        const int *gI, const int *gJ, const int *gK, const double *u,
+                        double *divU)
+{
+    const int n1 = p + 1, ngaus = n1 * n1 * n1, nnode = ngaus;
+    for (int n = 0; n < npoin; ++n) divU[n] = 0.0;
+    if (nnode > 128) return;
+    for (int e = 0; e < nelem; ++e) {
+        int ip[128];
+        double ul[128][3], Re[128];
+        for (int a = 0; a < nnode; ++a) {
+            ip[a] = CONNEC(e, a);
+            for (int d = 0; d < 3; ++d) ul[a][d] = NODAL(u, ip[a], d);
+        }
+        for (int g = 0; g < ngaus; ++g) {
+            double gi[3][3];
+            or_grad_iso(n1, ngaus, dlxigp_ip, invAtoIJK, g, gI[g] - 1, gJ[g] - 1, gK[g] - 1, ul,
+                        gi);
+            double dv = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) dv = dv + HE(i, j, g, e) * gi[i][j];
+            Re[g] = GPVOL(g, e) * dv;
+        }
+        for (int a = 0; a < nnode; ++a) divU[ip[a]] = divU[ip[a]] + Re[a];
+    }
+}
+
+/* compute_vorticity with makeConti (mod_operators.f90:684-775):
+ * curlU(n,i) = sum over the elements of n of eps_ijk du_j/dx_k (minus the
+ * curl) at n's GLL point (leviCivi of CFDSolverBase.f90:1605-1611: eps_123 =
+ * +1), each term
+ * added in (i,j,k) order, then divided by lelpn(n) on the working nodes */
+void or_compute_vorticity(int p, int nelem, int npoin, int npoin_w, const int *lpoin_w,
+                          const int *connec, const int *lelpn, const double *He,
+                          const double *dlxigp_ip, const int *invAtoIJK, const int *gI,
+                          const int *gJ, const int *gK, const double *u, double *curlU)
+{
+    const int n1 = p + 1, ngaus = n1 * n1 * n1, nnode = ngaus;
+    double lc[3][3][3] = {{{0}}};
+    lc[1][2][0] = 1.0;
+    lc[2][1][0] = -1.0;
+    lc[0][2][1] = -1.0;
+    lc[2][0][1] = 1.0;
+    lc[0][1][2] = 1.0;
+    lc[1][0][2] = -1.0;
+    for (size_t n = 0; n < 3 * (size_t)npoin; ++n) curlU[n] = 0.0;
+    if (nnode > 128) return;
+    for (int e = 0; e < nelem; ++e) {
+        double ul[128][3];
+        for (int a = 0; a < nnode; ++a)
+            for (int d = 0; d < 3; ++d) ul[a][d] = NODAL(u, CONNEC(e, a), d);
+        for (int g = 0; g < ngaus; ++g) {
+            double gi[3][3], gu[3][3];
+            or_grad_iso(n1, ngaus, dlxigp_ip, invAtoIJK, g, gI[g] - 1, gJ[g] - 1, gK[g] - 1, ul,
+                        gi);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    gu[i][j] = 0.0;
+                    for (int k = 0; k < 3; ++k) gu[i][j] = gu[i][j] + HE(j, k, g, e) * gi[i][k];
+                }
+            const int n = CONNEC(e, g);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k)
+                        NODAL(curlU, n, i) = NODAL(curlU, n, i) + lc[i][j][k] * gu[j][k];
+        }
+    }
+    for (int w = 0; w < npoin_w; ++w) {
+        const int n = lpoin_w[w] - 1;
+        for (int d = 0; d < 3; ++d) NODAL(curlU, n, d) = NODAL(curlU, n, d) / (double)lelpn[n];
+    }
+}
+
+/* ducros_visc_limit (mod_entropy_viscosity.f90:277-314): mu_e scaled in place by
+ * max(phi - ducros_min_val, 0) phi2 + ducros_min_val, with divUl = divU/Ml,
+ * phi = min(4/3 divUl^2 / (divUl^2 + |vorti|^2 + eps), 1) and
+ * phi2 = divUl^2 / (divUl^2 + (|u|/helem)^2 + eps), eps the machine epsilon.
+ * helem(nelem,nnode) is the caller's helem_l. */
+void or_ducros_visc_limit(int p, int nelem, int npoin, const int *connec, const double *u,
+                          const double *divU, const double *vorti, const double *Ml,
+                          const double *helem, double ducros_min_val, double *mu_e)
+{
+    const int n1 = p + 1, nnode = n1 * n1 * n1;
+    for (int e = 0; e < nelem; ++e)
+        for (int a = 0; a < nnode; ++a) {
+            const int n = CONNEC(e, a);
+            const double dv = divU[n] / Ml[n];
+            const double u0 = NODAL(u, n, 0), u1 = NODAL(u, n, 1), u2 = NODAL(u, n, 2);
+            const double w0 = NODAL(vorti, n, 0), w1 = NODAL(vorti, n, 1), w2 = NODAL(vorti, n, 2);
+            const double umag = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+            const double phi =
+                fmin((4.0 / 3.0) * (dv * dv) / (dv * dv + (w0 * w0 + w1 * w1 + w2 * w2) + DBL_EPSILON),
+                     1.0);
+            const double t = 1.0 * umag / ELGP(helem, e, a);
+            const double phi2 = (dv * dv) / (dv * dv + t * t + DBL_EPSILON);
+            ELGP(mu_e, e, a) = ELGP(mu_e, e, a) * (fmax(phi - ducros_min_val, 0.0) * phi2 + ducros_min_val);
+        }
+}
+
+/* elemPerNode (mod_geom.f90:68-99): lelpn(n) = incidences of n in connec */
+void or_elem_per_node(int nelem, int nnode, int npoin, const int *connec, int *lelpn)
+{
+    for (int n = 0; n < npoin; ++n) lelpn[n] = 0;
+    for (int e = 0; e < nelem; ++e)
+        for (int a = 0; a < nnode; ++a) lelpn[CONNEC(e, a)] += 1;
+}
+
 /* thermo_entropy (mod_entropy_viscosity.f90:411-433); eta is (npoin,2) */
 void or_thermo_entropy(int npoin, int npoin_w, const int *lpoin_w, double gamma_gas,
                        double p_inf, double rho_inf, const double *rho, const double *pr,

@@ -83,6 +83,24 @@ void or_smart_visc_spectral(int p, int nelem, int npoin, int npoin_w, const int 
                             const double *eta, const double *helem, const double *helem_k,
                             const int *invAtoIJK, const or_visc_params *prm, double *mu_e,
                             double *mue_l);
+/* ---- live-path sensor (mod_entropy_viscosity.f90:113-314, mod_operators.f90) ---- */
+void or_smart_visc_spectral_imex(int p, int nelem, int npoin, int npoin_w, const int *connec,
+                                 const int *lpoin_w, const double *Reta, const double *rho,
+                                 const double *u, const double *csound, const double *eta,
+                                 const double *helem_k, or_visc_params *prm, int flag_high_mach,
+                                 double *mu_e, double *mue_l);
+void or_eval_divergence(int p, int nelem, int npoin, const int *connec, const double *He,
+                        const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                        const int *gI, const int *gJ, const int *gK, const double *u,
+                        double *divU);
+void or_compute_vorticity(int p, int nelem, int npoin, int npoin_w, const int *lpoin_w,
+                          const int *connec, const int *lelpn, const double *He,
+                          const double *dlxigp_ip, const int *invAtoIJK, const int *gI,
+                          const int *gJ, const int *gK, const double *u, double *curlU);
+void or_ducros_visc_limit(int p, int nelem, int npoin, const int *connec, const double *u,
+                          const double *divU, const double *vorti, const double *Ml,
+                          const double *helem, double ducros_min_val, double *mu_e);
+void or_elem_per_node(int nelem, int nnode, int npoin, const int *connec, int *lelpn);
 void or_thermo_entropy(int npoin, int npoin_w, const int *lpoin_w, double gamma_gas,
                        double p_inf, double rho_inf, const double *rho, const double *pr,
                        const double *u, double *eta /* (npoin,2) */, double *feta);

@@ -115,6 +115,15 @@ def lumped_mass(porder: int, connec: np.ndarray, gpvol: np.ndarray, npoin: int,
     return Ml, invMl
 
 
+def elem_per_node(connec: np.ndarray, npoin: int) -> np.ndarray:
+    """elemPerNode (mod_geom.f90:68-99): incidences of each node in connec."""
+    nelem, nn = connec.shape
+    lelpn = np.zeros(npoin, dtype=np.int32)
+    check(lib().sodhip_elem_per_node(nelem, nn, npoin, _p(connec), _p(lelpn)),
+          "sodhip_elem_per_node")
+    return lelpn
+
+
 def char_length(porder: int, connec_orig: np.ndarray, coord: np.ndarray, spectral: bool = True):
     nelem, nn = connec_orig.shape
     helem = np.zeros(nelem)

@@ -701,6 +701,99 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
     }
 }
 
+// eval_divergence (mod_operators.f90:485-570) and compute_vorticity
+// (mod_operators.f90:684-775) at each GLL point.  With g_ik = du_i/dxi_k the
+// line derivatives and grad u(i,j) = sum_k He(j,k) g_ik: the divergence record
+// is gpvol * div u, the vorticity record eps_ijk du_j/dx_k (the reference's
+// leviCivi sum, minus the curl).  Records of DIV + 3 CURL values per
+// element-node at e*NN + perm[a].
+template <int P, typename T, bool DIV, bool CURL>
+__global__ __launch_bounds__(EG<P>::NT) void k_div_curl(RhsArgs<T> A)
+{
+    constexpr int NN = EG<P>::NN, NT = EG<P>::NT, NF = (DIV ? 1 : 0) + (CURL ? 3 : 0);
+    __shared__ T s[3 * NT]; // u(3)
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    LineTab<P, T> lt;
+    build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1);
+    struct In {
+        T u[3], He[9], vol;
+    };
+    auto load = [&](In &x, int e) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? A.connecT[ea] : 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x.u[d] = A.u[n + (size_t)A.npoin * d];
+        const T *hp = A.He + 9 * ea;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) x.He[i] = hp[i];
+        x.vol = DIV ? A.gpvol[ea] : T(0);
+    };
+    const ElemRange er = elem_range(A.nelem);
+    if (er.begin >= er.end) return;
+    const int elast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
+    In cur, nxt;
+    load(cur, er.begin);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        load(nxt, min(e + er.stride, elast));
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[d * NT + a] = cur.u[d];
+        }
+        lds_barrier();
+        T g[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const T c0 = lt.ds[k];
+            T du[3] = {c0 * cur.u[0], c0 * cur.u[1], c0 * cur.u[2]};
+#pragma unroll
+            for (int o = 0; o < P; ++o) {
+                const int b = lt.off[k][o];
+                const T c = lt.d[k][o];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) du[d] += c * s[d * NT + b];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) g[d][k] = du[d];
+        }
+        T gu[3][3]; // gu[i][j] = du_i/dx_j
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                gu[i][j] = cur.He[j] * g[i][0] + cur.He[j + 3] * g[i][1] + cur.He[j + 6] * g[i][2];
+        if (act) {
+            T *o = A.out + NF * ((size_t)e * NN + pa);
+            int f = 0;
+            if constexpr (DIV) o[f++] = cur.vol * (gu[0][0] + gu[1][1] + gu[2][2]);
+            if constexpr (CURL) {
+                o[f++] = gu[1][2] - gu[2][1];
+                o[f++] = gu[2][0] - gu[0][2];
+                o[f++] = gu[0][1] - gu[1][0];
+            }
+        }
+        cur = nxt;
+        lds_barrier();
+    }
+}
+
+// curlU(n,:) /= lelpn(n) over the working nodes (compute_vorticity's makeConti)
+template <typename T>
+__global__ __launch_bounds__(256) void k_per_elem_count(int npoin, int npoin_w,
+                                                        const int *__restrict__ lpoin_w,
+                                                        const int *__restrict__ lelpn,
+                                                        T *__restrict__ curl)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const T c = T(lelpn[n]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) curl[n + (size_t)npoin * d] /= c;
+}
+
 // Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * scr[src[j]*nf + f], j
 // over the node's slots row[n]..row[n+1]-1 in ascending (element, node) order
 // (records of nf fields per element-node).
@@ -1651,6 +1744,74 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
                                initialize, fact);
 }
 
+// eval_divergence / compute_vorticity / both in one element pass
+template <typename T>
+static int div_curl(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                    const int *connec, const int *lpoin_w, const int *lelpn, const T *He,
+                    const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
+                    const int *gK, const T *u, T *divU, T *curlU)
+{
+    int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
+    if (rc) return rc;
+    SODHIP_REQUIRE(He && dl && inv && gI && gJ && gK && u && (divU || curlU),
+                   "null array argument");
+    SODHIP_REQUIRE(!divU || gpvol, "null gpvol");
+    SODHIP_REQUIRE(!curlU || (lpoin_w && lelpn && npoin_w > 0 && npoin_w <= npoin),
+                   "null lpoin_w / lelpn or invalid npoin_w");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    MeshCache *m = nullptr;
+    if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
+    const int nf = (divU ? 1 : 0) + (curlU ? 3 : 0);
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, nf * EN * sizeof(T)))) return rc;
+    RhsArgs<T> A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.EN = EN;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.u = u;
+    A.out = (T *)ctx->scratch.ptr;
+    const int grid = elem_grid(ctx, nelem, 16);
+    int tok = prof_begin(ctx, 3);
+#define SODHIP_DC(PP, DV, CU) \
+    k_div_curl<PP, T, DV, CU><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A)
+#define SODHIP_DC_P(PP)                                                        \
+    if (divU && curlU) SODHIP_DC(PP, true, true);                              \
+    else if (divU) SODHIP_DC(PP, true, false);                                 \
+    else SODHIP_DC(PP, false, true);
+    switch (porder) {
+    case 2: SODHIP_DC_P(2) break;
+    case 3: SODHIP_DC_P(3) break;
+    case 4: SODHIP_DC_P(4) break;
+    }
+#undef SODHIP_DC_P
+#undef SODHIP_DC
+    SODHIP_TRY_HIP(hipGetLastError());
+    T *c0 = curlU, *c1 = curlU ? curlU + npoin : nullptr, *c2 = curlU ? curlU + 2 * (size_t)npoin : nullptr;
+    if (divU && curlU)
+        rc = launch_gather<T, T>(ctx, m, A.out, EN, 4, divU, c0, c1, c2, nullptr, 1, T(1));
+    else if (divU)
+        rc = launch_gather<T, T>(ctx, m, A.out, EN, 1, divU, nullptr, nullptr, nullptr, nullptr, 1,
+                                 T(1));
+    else
+        rc = launch_gather<T, T>(ctx, m, A.out, EN, 3, c0, c1, c2, nullptr, nullptr, 1, T(1));
+    if (rc == SODHIP_OK && curlU) {
+        k_per_elem_count<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(npoin, npoin_w, lpoin_w,
+                                                                           lelpn, curlU);
+        SODHIP_TRY_HIP(hipGetLastError());
+    }
+    prof_end(ctx, tok);
+    return rc;
+}
+
 } // namespace sodhip
 
 using namespace sodhip;
@@ -1721,4 +1882,39 @@ extern "C" int sodhip_generic_scalar_convec_ijk_f64(
     return scalar_convec<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
                                  invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, q, eta, u, Rconvec,
                                  initialize, fact);
+}
+
+extern "C" int sodhip_eval_divergence_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                          const int *connec, const double *He,
+                                          const double *gpvol, const double *dlxigp_ip,
+                                          const int *invAtoIJK, const int *gmshAtoI,
+                                          const int *gmshAtoJ, const int *gmshAtoK,
+                                          const double *u, double *divU)
+{
+    return div_curl<double>(ctx, porder, nelem, npoin, 0, connec, nullptr, nullptr, He, gpvol,
+                            dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, divU, nullptr);
+}
+
+extern "C" int sodhip_compute_vorticity_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                            int npoin_w, const int *lpoin_w, const int *connec,
+                                            const int *lelpn, const double *He,
+                                            const double *dlxigp_ip, const int *invAtoIJK,
+                                            const int *gmshAtoI, const int *gmshAtoJ,
+                                            const int *gmshAtoK, const double *u, double *curlU)
+{
+    return div_curl<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, lelpn, He,
+                            nullptr, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u,
+                            nullptr, curlU);
+}
+
+extern "C" int sodhip_div_curl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                   const int *lpoin_w, const int *connec, const int *lelpn,
+                                   const double *He, const double *gpvol,
+                                   const double *dlxigp_ip, const int *invAtoIJK,
+                                   const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                                   const double *u, double *divU, double *curlU)
+{
+    SODHIP_REQUIRE(divU && curlU, "null output");
+    return div_curl<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, lelpn, He, gpvol,
+                            dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, divU, curlU);
 }

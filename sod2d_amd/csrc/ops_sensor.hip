@@ -10,6 +10,7 @@
 // kernel folds the partial maxima (max is order independent).  No host
 // synchronisation is needed between the three launches.
 #include <algorithm>
+#include <cfloat>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -80,6 +81,112 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
         m = fmax(m, fabs(eta[lpoin_w[i] - 1] - mean));
     m = block_max(m, sh);
     if (threadIdx.x == 0) pmax[blockIdx.x] = m;
+}
+
+template <typename T> __device__ __forceinline__ T block_min(T v, T *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmin(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// partial minima of rho over working nodes (contiguous slices), starting
+// from the reference's 100000
+__global__ __launch_bounds__(kRedThreads) void k_rho_min(int npoin_w, const int *__restrict__ lpoin_w,
+                                                         const double *__restrict__ rho,
+                                                         double *__restrict__ part)
+{
+    __shared__ double sh[kRedThreads];
+    const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    double m = 100000.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) m = fmin(m, rho[lpoin_w[i] - 1]);
+    m = block_min(m, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227), one thread per
+// element (connec in the reference layout: consecutive threads read
+// consecutive words): mue_l <- incoming mu_e; beta_e = max over the nodes of
+// rho helem_k (cmax/p) (|u| + fact c); mu_e = cglob min(ce rho |Reta|/norm
+// (helem_k/p)^2, beta_e).  norm: 1, or the max of the partials (max |eta -
+// mean|), or the min of the partials (min rho), folded per block.
+struct ImexArgs {
+    int nelem, nnode, porder;
+    const int *connec;
+    const double *Reta, *rho, *u, *csound, *helem_k;
+    int npoin;
+    double ce, cmax, cglob, fact_low_mach;
+    const double *part; // partials of the norm, or nullptr (norm = 1)
+    int nparts, norm_min;
+    double *mu_e, *mue_l;
+};
+
+__global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
+{
+    __shared__ double sh[256];
+    double norm = 1.0;
+    if (A.part) {
+        double v = A.norm_min ? 100000.0 : 0.0;
+        for (int i = threadIdx.x; i < A.nparts; i += blockDim.x)
+            v = A.norm_min ? fmin(v, A.part[i]) : fmax(v, A.part[i]);
+        norm = A.norm_min ? block_min(v, sh) : block_max(v, sh);
+    }
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= A.nelem) return;
+    const size_t NE = (size_t)A.nelem;
+    const double *u1 = A.u + A.npoin, *u2 = A.u + 2 * (size_t)A.npoin;
+    const double hk = A.helem_k[e], cp = A.cmax / (double)A.porder;
+    double betae = 0.0;
+    for (int a = 0; a < A.nnode; ++a) {
+        const size_t ea = e + NE * a;
+        A.mue_l[ea] = A.mu_e[ea];
+        const int n = A.connec[ea] - 1;
+        const double v0 = A.u[n], v1 = u1[n], v2 = u2[n];
+        const double speed = sqrt(v0 * v0 + v1 * v1 + v2 * v2) + A.fact_low_mach * A.csound[n];
+        betae = fmax(betae, (A.rho[n] * hk) * cp * speed);
+    }
+    const double hp = hk / (double)A.porder;
+    for (int a = 0; a < A.nnode; ++a) {
+        const size_t ea = e + NE * a;
+        const int n = A.connec[ea] - 1;
+        const double R1 = A.rho[n] * fabs(A.Reta[n]) / norm;
+        const double Ve = A.ce * R1 * (hp * hp);
+        A.mu_e[ea] = A.cglob * fmin(Ve, betae);
+    }
+}
+
+// ducros_visc_limit (mod_entropy_viscosity.f90:277-314), one thread per
+// element-node (mu_e(nelem,ngaus) and connec in the reference layout)
+__global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int npoin,
+                                                const int *__restrict__ connec,
+                                                const double *__restrict__ u,
+                                                const double *__restrict__ divU,
+                                                const double *__restrict__ vorti,
+                                                const double *__restrict__ Ml,
+                                                const double *__restrict__ helem, double dmin,
+                                                double *__restrict__ mu_e)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nelem * nnode) return;
+    const int n = connec[t] - 1;
+    const size_t P1 = (size_t)npoin, P2 = 2 * (size_t)npoin;
+    const double dv = divU[n] / Ml[n];
+    const double w0 = vorti[n], w1 = vorti[n + P1], w2 = vorti[n + P2];
+    const double v0 = u[n], v1 = u[n + P1], v2 = u[n + P2];
+    const double umag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    const double d2 = dv * dv;
+    const double phi = fmin((4.0 / 3.0) * d2 / (d2 + (w0 * w0 + w1 * w1 + w2 * w2) + DBL_EPSILON), 1.0);
+    const double r = 1.0 * umag / helem[t];
+    const double phi2 = d2 / (d2 + r * r + DBL_EPSILON);
+    mu_e[t] = mu_e[t] * (fmax(phi - dmin, 0.0) * phi2 + dmin);
 }
 
 template <int P> struct SG {
@@ -403,6 +510,83 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     case 3: k_smart_visc_ch<3, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
     case 4: k_smart_visc<4><<<elem_grid(ctx, nelem, 16), SG<4>::NT, 0, ctx->stream>>>(A); break;
     }
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_smart_visc_spectral_imex_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, const double *Reta, const double *rho, const double *u,
+    const double *csound, const double *eta, const double *helem_k, sodhip_visc_params *prm,
+    double ce_comp, int flag_high_mach, double *mu_e, double *mue_l)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && Reta && rho && u && csound && eta && helem_k && mu_e &&
+                       mue_l,
+                   "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    int rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    double *pmax = part + kRedBlocks;
+    const bool normalise = prm->flag_normalise_entropy == 1;
+    const bool high_mach = normalise && flag_high_mach;
+    int tok = prof_begin(ctx, 2);
+    if (high_mach) {
+        k_rho_min<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, rho, pmax);
+        // the reference assigns its module global ce (kept for later calls)
+        prm->ce = ce_comp;
+    } else if (normalise) {
+        k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+        k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+                                                                  kRedBlocks, pmax);
+    }
+    ImexArgs A{};
+    A.nelem = nelem;
+    A.nnode = (porder + 1) * (porder + 1) * (porder + 1);
+    A.porder = porder;
+    A.connec = connec;
+    A.Reta = Reta;
+    A.rho = rho;
+    A.u = u;
+    A.csound = csound;
+    A.helem_k = helem_k;
+    A.npoin = npoin;
+    A.ce = prm->ce;
+    A.cmax = prm->cmax;
+    A.cglob = prm->cglob;
+    A.fact_low_mach = prm->flag_drop_c_in_envit ? 0.0 : 1.0;
+    A.part = normalise ? pmax : nullptr;
+    A.nparts = kRedBlocks;
+    A.norm_min = high_mach ? 1 : 0;
+    A.mu_e = mu_e;
+    A.mue_l = mue_l;
+    k_visc_imex<<<(nelem + 255) / 256, 256, 0, ctx->stream>>>(A);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_ducros_visc_limit_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                            const int *connec, const double *u,
+                                            const double *divU, const double *vorti,
+                                            const double *Ml, const double *helem,
+                                            double ducros_min_val, double *mu_e)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0, "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && u && divU && vorti && Ml && helem && mu_e, "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    const size_t EN = (size_t)nelem * (porder + 1) * (porder + 1) * (porder + 1);
+    int tok = prof_begin(ctx, 2);
+    k_ducros<<<(unsigned)((EN + 255) / 256), 256, 0, ctx->stream>>>(
+        nelem, (porder + 1) * (porder + 1) * (porder + 1), npoin, connec, u, divU, vorti, Ml, helem,
+        ducros_min_val, mu_e);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;

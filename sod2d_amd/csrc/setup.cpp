@@ -506,3 +506,18 @@ extern "C" int sodhip_char_length(int porder, int nelem, int npoin, const int *c
     });
     return SODHIP_OK;
 }
+
+extern "C" int sodhip_elem_per_node(int nelem, int nnode, int npoin, const int *connec,
+                                    int *lelpn)
+{
+    SODHIP_REQUIRE(nelem > 0 && nnode > 0 && npoin > 0, "invalid sizes");
+    SODHIP_REQUIRE(connec && lelpn, "null array argument");
+    for (int n = 0; n < npoin; ++n) lelpn[n] = 0;
+    for (long a = 0; a < nnode; ++a)
+        for (long e = 0; e < nelem; ++e) {
+            const int v = connec[a * nelem + e];
+            SODHIP_REQUIRE(v >= 1 && v <= npoin, "connec entry %d outside 1..npoin=%d", v, npoin);
+            lelpn[v - 1] += 1;
+        }
+    return SODHIP_OK;
+}

@@ -212,6 +212,101 @@ def smart_visc_spectral(nelem, npoin, npoin_w, connec, lpoin_w, Reta, Ngp, coord
     check(rc, "smart_visc_spectral")
 
 
+def smart_visc_spectral_imex(nelem, npoin, npoin_w, connec, lpoin_w, Reta, Ngp, coord, dNgp,
+                             gpvol, wgp, gamma_gas, rho, u, csound, Tem, eta, helem, helem_k, Ml,
+                             mu_e, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mue_l,
+                             params: ViscParams, ce_comp: float = 1.0, flag_high_mach: int = 0,
+                             ctx: Context | None = None):
+    """smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227) on the device
+    (FP64), the reference's live-path sensor.  With the normalisation and
+    flag_high_mach, params.ce becomes ce_comp (the reference's module global)."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    dt, I32 = torch.float64, torch.int32
+    rc = lib().sodhip_smart_visc_spectral_imex_f64(
+        ctx.h, p, nelem, npoin, npoin_w, _ptr(connec, I32, nelem * nn, "connec"),
+        _ptr(lpoin_w, I32, npoin_w, "lpoin_w"), _ptr(Reta, dt, npoin, "Reta"),
+        _ptr(rho, dt, npoin, "rho"), _ptr(u, dt, 3 * npoin, "u"),
+        _ptr(csound, dt, npoin, "csound"), _ptr(eta, dt, npoin, "eta"),
+        _ptr(helem_k, dt, nelem, "helem_k"), C.byref(params), C.c_double(ce_comp),
+        int(flag_high_mach), _ptr(mu_e, dt, nelem * nn, "mu_e"),
+        _ptr(mue_l, dt, nelem * nn, "mue_l"))
+    check(rc, "smart_visc_spectral_imex")
+
+
+def eval_divergence(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                    gmshAtoK, u, divU, ctx: Context | None = None):
+    """eval_divergence (mod_operators.f90:485-570) on the device (FP64)."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    dt, I32 = torch.float64, torch.int32
+    check(lib().sodhip_eval_divergence_f64(
+        ctx.h, p, nelem, npoin, _ptr(connec, I32, nelem * nn, "connec"),
+        _ptr(He, dt, 9 * nn * nelem, "He"), _ptr(gpvol, dt, nn * nelem, "gpvol"),
+        _ptr(dlxigp_ip, dt, 3 * nn * (p + 1), "dlxigp_ip"), _ptr(invAtoIJK, I32, nn),
+        _ptr(gmshAtoI, I32, nn), _ptr(gmshAtoJ, I32, nn), _ptr(gmshAtoK, I32, nn),
+        _ptr(u, dt, 3 * npoin, "u"), _ptr(divU, dt, npoin, "divU")), "eval_divergence")
+
+
+def compute_vorticity(nelem, npoin, npoin_w, lpoin_w, connec, lelpn, He, dlxigp_ip, invAtoIJK,
+                      gmshAtoI, gmshAtoJ, gmshAtoK, u, curlU, ctx: Context | None = None):
+    """compute_vorticity with makeConti (mod_operators.f90:684-775) on the device."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    dt, I32 = torch.float64, torch.int32
+    check(lib().sodhip_compute_vorticity_f64(
+        ctx.h, p, nelem, npoin, npoin_w, _ptr(lpoin_w, I32, npoin_w, "lpoin_w"),
+        _ptr(connec, I32, nelem * nn, "connec"), _ptr(lelpn, I32, npoin, "lelpn"),
+        _ptr(He, dt, 9 * nn * nelem, "He"), _ptr(dlxigp_ip, dt, 3 * nn * (p + 1), "dlxigp_ip"),
+        _ptr(invAtoIJK, I32, nn), _ptr(gmshAtoI, I32, nn), _ptr(gmshAtoJ, I32, nn),
+        _ptr(gmshAtoK, I32, nn), _ptr(u, dt, 3 * npoin, "u"), _ptr(curlU, dt, 3 * npoin, "curlU")),
+        "compute_vorticity")
+
+
+def div_curl(nelem, npoin, npoin_w, lpoin_w, connec, lelpn, He, gpvol, dlxigp_ip, invAtoIJK,
+             gmshAtoI, gmshAtoJ, gmshAtoK, u, divU, curlU, ctx: Context | None = None):
+    """eval_divergence and compute_vorticity of one u in a single element pass."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    dt, I32 = torch.float64, torch.int32
+    check(lib().sodhip_div_curl_f64(
+        ctx.h, p, nelem, npoin, npoin_w, _ptr(lpoin_w, I32, npoin_w, "lpoin_w"),
+        _ptr(connec, I32, nelem * nn, "connec"), _ptr(lelpn, I32, npoin, "lelpn"),
+        _ptr(He, dt, 9 * nn * nelem, "He"), _ptr(gpvol, dt, nn * nelem, "gpvol"),
+        _ptr(dlxigp_ip, dt, 3 * nn * (p + 1), "dlxigp_ip"), _ptr(invAtoIJK, I32, nn),
+        _ptr(gmshAtoI, I32, nn), _ptr(gmshAtoJ, I32, nn), _ptr(gmshAtoK, I32, nn),
+        _ptr(u, dt, 3 * npoin, "u"), _ptr(divU, dt, npoin, "divU"),
+        _ptr(curlU, dt, 3 * npoin, "curlU")), "div_curl")
+
+
+def ducros_visc_limit(nelem, npoin, npoin_w, connec, lpoin_w, Ngp, coord, dNgp, gpvol, wgp, rho,
+                      u, divU, vorti, csound, helem, helem_k, Ml, mu_e, invAtoIJK, gmshAtoI,
+                      gmshAtoJ, gmshAtoK, mue_l, ducros_min_val: float = 0.1,
+                      ctx: Context | None = None):
+    """ducros_visc_limit (mod_entropy_viscosity.f90:277-314) on the device: mu_e
+    scaled in place (rho, csound, helem_k and the other reference arguments do
+    not enter the result)."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    dt, I32 = torch.float64, torch.int32
+    check(lib().sodhip_ducros_visc_limit_f64(
+        ctx.h, p, nelem, npoin, _ptr(connec, I32, nelem * nn, "connec"),
+        _ptr(u, dt, 3 * npoin, "u"), _ptr(divU, dt, npoin, "divU"),
+        _ptr(vorti, dt, 3 * npoin, "vorti"), _ptr(Ml, dt, npoin, "Ml"),
+        _ptr(helem, dt, nelem * nn, "helem"), C.c_double(ducros_min_val),
+        _ptr(mu_e, dt, nelem * nn, "mu_e")), "ducros_visc_limit")
+
+
 def lumped_solver_scal(npoin, npoin_w, lpoin_w, invMl, R, ctx: Context | None = None):
     ctx = ctx or default_context()
     ctx.sync_stream()

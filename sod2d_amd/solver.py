@@ -18,8 +18,12 @@ Modes:
   * ``dt_mode`` "adaptive" (adapt_dt_cfl after every step, as the reference) or
     "fixed" (the CFL time step of the initial state, held).
 
-The entropy-viscosity sensor is smart_visc_spectral (mod_entropy_viscosity.f90:
-14-111) at the points where the reference's live path calls its _imex variant.
+The entropy-viscosity sensor (``sensor_path``) is either smart_visc_spectral
+(mod_entropy_viscosity.f90:14-111, "spectral") at the points where the
+reference's live path calls its _imex variant, or that live path itself
+("live"): smart_visc_spectral_imex at the first step (time_integ_ls.f90:292)
+and, after every step, smart_visc_spectral_imex, eval_divergence,
+compute_vorticity and ducros_visc_limit (time_integ_ls.f90:455-463).
 """
 from __future__ import annotations
 
@@ -100,7 +104,7 @@ class TGVRun:
                  length: float = 2.0 * math.pi, stages: int = 5, variant: int = 1,
                  op_precision: int = 8, parity: bool = True, dt_mode: str = "adaptive",
                  uniform_rho: bool = False, sensor: bool = True, ctx: Context | None = None,
-                 geometry=None, initial=None):
+                 geometry=None, initial=None, sensor_path: str = "spectral"):
         self.t = params or TGVParams()
         self.ctx = ctx or Context(torch.cuda.current_device())
         self.p = porder
@@ -108,6 +112,9 @@ class TGVRun:
         self.dt_mode = dt_mode
         self.op_precision = op_precision
         self.sensor = sensor
+        if sensor_path not in ("spectral", "live"):
+            raise SodhipError(f"unknown sensor_path {sensor_path!r}")
+        self.sensor_path = sensor_path
         g = geometry or case_mod.build_case(porder, nelem_dir, periodic=True,
                                              lengths=(length,) * 3)
         self.geo = g
@@ -130,6 +137,9 @@ class TGVRun:
         self.helem = _dev(g.helem)
         self.helem_l = _dev(g.helem_l)
         self.vol_tot = g.vol_tot
+        if sensor_path == "live":
+            self.Ml = _dev(g.Ml)
+            self.lelpn = _dev(case_mod.elem_per_node(m.connec, m.npoin))
         # state
         ic = initial if initial is not None else tgv_initial_conditions(m.coord, self.t, uniform_rho)
         n, ne, nn = self.npoin, self.nelem, self.nnode
@@ -152,6 +162,8 @@ class TGVRun:
         self.Reta = z(2 * n)
         self.auxReta = z(n)
         self.f_eta = z(3 * n)
+        if sensor_path == "live":
+            self.divU, self.vorti = z(n), z(3 * n)
         self.first = True
         self.time = 0.0
         self.istep = 0
@@ -189,12 +201,30 @@ class TGVRun:
                                                   _ptr(self.lpoin_w), _ptr(self.invMl), _ptr(out)),
               "lumped_solver_scal")
 
-    def _smart_visc(self, Reta, rho, u, Tem, eta):
-        check(lib().sodhip_smart_visc_spectral_f64(
-            self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
-            _ptr(self.lpoin_w), _ptr(Reta), self.t.gamma_gas, _ptr(rho), _ptr(u), _ptr(Tem),
-            _ptr(eta), _ptr(self.helem_l), _ptr(self.helem), _ptr(self.tabs[0]),
-            C.byref(self.vp), _ptr(self.mu_e), _ptr(self.mue_l)), "smart_visc_spectral")
+    def _smart_visc(self, Reta, rho, u, Tem, eta, ducros=False):
+        h, p, ne, n, nw = self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w
+        if self.sensor_path == "spectral":
+            check(lib().sodhip_smart_visc_spectral_f64(
+                h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta),
+                self.t.gamma_gas, _ptr(rho), _ptr(u), _ptr(Tem), _ptr(eta), _ptr(self.helem_l),
+                _ptr(self.helem), _ptr(self.tabs[0]), C.byref(self.vp), _ptr(self.mu_e),
+                _ptr(self.mue_l)), "smart_visc_spectral")
+        else:
+            check(lib().sodhip_smart_visc_spectral_imex_f64(
+                h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta), _ptr(rho),
+                _ptr(u), _ptr(self.csound), _ptr(eta), _ptr(self.helem), C.byref(self.vp),
+                C.c_double(1.0), 0, _ptr(self.mu_e), _ptr(self.mue_l)),
+                "smart_visc_spectral_imex")
+            if ducros:
+                check(lib().sodhip_div_curl_f64(
+                    h, p, ne, n, nw, _ptr(self.lpoin_w), _ptr(self.connec), _ptr(self.lelpn),
+                    _ptr(self.He), _ptr(self.gpvol), _ptr(self.dl),
+                    *[_ptr(x) for x in self.tabs], _ptr(u), _ptr(self.divU), _ptr(self.vorti)),
+                    "div_curl")
+                check(lib().sodhip_ducros_visc_limit_f64(
+                    h, p, ne, n, _ptr(self.connec), _ptr(u), _ptr(self.divU), _ptr(self.vorti),
+                    _ptr(self.Ml), _ptr(self.helem_l), C.c_double(0.1), _ptr(self.mu_e)),
+                    "ducros_visc_limit")
         if self.op_precision == 4:
             self.mu_e_op.copy_(self.mu_e)
 
@@ -274,7 +304,8 @@ class TGVRun:
             h, n, self.npoin_w, _ptr(self.lpoin_w), 1.0, self.dt, _ptr(self.Reta), _ptr(self.eta),
             _ptr(self.auxReta)), "entropy_residual")
         if self.sensor:
-            self._smart_visc(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[n:])
+            self._smart_visc(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[n:],
+                             ducros=True)
         self._sutherland()
 
     # ------------------------------------------------------------ diagnostics

@@ -15,16 +15,17 @@ from helpers import make_case, relerr
 pytestmark = pytest.mark.gpu
 
 
-def _setup(oracle, p, n, stages=5, parity=True, op_precision=8, dt_mode="adaptive", M=0.1):
+def _setup(oracle, p, n, stages=5, parity=True, op_precision=8, dt_mode="adaptive", M=0.1,
+           sensor_path="spectral"):
     from sod2d_amd.solver import TGVParams, TGVRun
     g = make_case(p, n, periodic=True, lengths=(2 * np.pi,) * 3)
     tp = TGVParams(M=M)
     to = oracle.tgv_params(M=M)
     ic = oracle.tgv_ic(g.mesh.coord, to)
     dev = TGVRun(p, geometry=g, params=tp, initial=ic, stages=stages, parity=parity,
-                 op_precision=op_precision, dt_mode=dt_mode)
+                 op_precision=op_precision, dt_mode=dt_mode, sensor_path=sensor_path)
     orc = oracle.TGVOracle(p, g, ic, to, stages=stages, parity=parity, dt_mode=dt_mode,
-                           cfl=(tp.cfl_conv, tp.cfl_diff))
+                           cfl=(tp.cfl_conv, tp.cfl_diff), sensor_path=sensor_path)
     return dev, orc
 
 
@@ -45,6 +46,26 @@ def test_time_loop_fp64(oracle, p, n, stages, parity):
     dd, do = dev.diagnostics(), orc.diagnostics()
     for k in ("EK", "eps_S", "eps_D", "eps_T", "maxmach"):
         assert abs(dd[k] - do[k]) <= 1e-10 * abs(do[k]) + 1e-300, (k, dd[k], do[k])
+
+
+@pytest.mark.parametrize("p,n", [(3, 4), (2, 4)])
+def test_time_loop_live_sensor(oracle, p, n):
+    """The reference's live path: smart_visc_spectral_imex, divergence,
+    vorticity and the Ducros limit after every step (time_integ_ls.f90:
+    455-463), same bars as the plain sensor path."""
+    dev, orc = _setup(oracle, p, n, sensor_path="live")
+    for step in range(6):
+        dev.step()
+        orc.step()
+        s = dev.state()
+        w = orc.lpoin_w - 1
+        assert relerr(s["rho"][w], orc.rho[1][w]) < 1e-12, step
+        assert relerr(s["q"][w], orc.q[1][w]) < 1e-12, step
+        assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
+        assert relerr(s["mu_e"], orc.mu_e) < 1e-10, step
+        assert abs(dev.dt - orc.dt) <= 1e-10 * orc.dt, step
+    # the limiter acts: mu_e differs from the unlimited imex sensor's output
+    assert np.any(orc.mu_e != orc.mue_l)
 
 
 def test_time_loop_fp32_operators(oracle):
