@@ -218,8 +218,9 @@ int sodhip_eval_divergence_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
 
 /* compute_vorticity with makeConti (mod_operators.f90:684-775): curlU(npoin,3) =
  * assembled eps_ijk du_j/dx_k (the reference's leviCivi of CFDSolverBase.f90:
- * 1605-1611, i.e. minus the curl), divided by lelpn on the working nodes.  The
- * reference's dNgp, leviCivi and atoIJK arguments are not needed. */
+ * 1605-1611, i.e. minus the curl), divided by lelpn on the working nodes
+ * (makeConti; with lelpn NULL the sums are left undivided).  The reference's
+ * dNgp, leviCivi and atoIJK arguments are not needed. */
 int sodhip_compute_vorticity_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
                                  const int *lpoin_w, const int *connec, const int *lelpn,
                                  const double *He, const double *dlxigp_ip,

@@ -1756,8 +1756,8 @@ static int div_curl(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     SODHIP_REQUIRE(He && dl && inv && gI && gJ && gK && u && (divU || curlU),
                    "null array argument");
     SODHIP_REQUIRE(!divU || gpvol, "null gpvol");
-    SODHIP_REQUIRE(!curlU || (lpoin_w && lelpn && npoin_w > 0 && npoin_w <= npoin),
-                   "null lpoin_w / lelpn or invalid npoin_w");
+    SODHIP_REQUIRE(!curlU || !lelpn || (lpoin_w && npoin_w > 0 && npoin_w <= npoin),
+                   "null lpoin_w or invalid npoin_w");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     MeshCache *m = nullptr;
     if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
@@ -1803,7 +1803,7 @@ static int div_curl(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
                                  T(1));
     else
         rc = launch_gather<T, T>(ctx, m, A.out, EN, 3, c0, c1, c2, nullptr, nullptr, 1, T(1));
-    if (rc == SODHIP_OK && curlU) {
+    if (rc == SODHIP_OK && curlU && lelpn) {
         k_per_elem_count<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(npoin, npoin_w, lpoin_w,
                                                                            lelpn, curlU);
         SODHIP_TRY_HIP(hipGetLastError());

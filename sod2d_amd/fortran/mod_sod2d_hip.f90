@@ -35,6 +35,8 @@ module mod_sod2d_hip
    public :: full_convec_ijk, full_diffusion_ijk, generic_scalar_convec_ijk
    public :: smart_visc_spectral, lumped_solver_scal_opt, lumped_solver_vect_opt
    public :: rk_stage_update
+   public :: smart_visc_spectral_imex, eval_divergence, compute_vorticity, ducros_visc_limit
+   public :: sod2d_hip_set_live_params
 
    type, bind(C) :: visc_params_t
       real(c_double) :: ce, cmax, cglob, Rgas
@@ -50,6 +52,11 @@ module mod_sod2d_hip
    real(rp), save :: fact_convec = 1.0_rp, fact_diffu = 1.0_rp, fact_scalar = 1.0_rp
    ! likewise fact_low_mach of smart_visc_spectral, once dropped, stays dropped
    logical, save :: low_mach_dropped = .false.
+   ! module globals of the live-path sensor (mod_numerical_params.f90:36,172-173
+   ! and the case's flag_high_mach); smart_visc_spectral_imex's high-Mach
+   ! normalisation sets the module's ce to ce_comp for every later call
+   real(rp), save :: ce_comp = 1.0_rp, ducros_min_val = 0.1_rp
+   logical, save :: flag_high_mach = .false.
 
    interface sod2d_hip_enter_data
       module procedure enter_r8, enter_i4
@@ -164,6 +171,34 @@ module mod_sod2d_hip
          type(c_ptr), value :: c, lpoin_w, invMl, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener
          integer(c_int), value :: npoin, npoin_w
          real(c_double), value :: dt, lam1, lams, gam
+      end function
+      integer(c_int) function c_visc_imex(c, p, nelem, npoin, npoin_w, connec, lpoin_w, Reta, &
+            rho, u, csound, eta, helem_k, prm, ce_comp, high_mach, mu_e, mue_l) &
+            bind(C, name='sodhip_smart_visc_spectral_imex_f64')
+         import :: c_int, c_ptr, c_double, visc_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, Reta, rho, u, csound, eta, helem_k, mu_e, mue_l
+         integer(c_int), value :: p, nelem, npoin, npoin_w, high_mach
+         real(c_double), value :: ce_comp
+         type(visc_params_t), intent(inout) :: prm
+      end function
+      integer(c_int) function c_eval_div(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, gJ, &
+            gK, u, divU) bind(C, name='sodhip_eval_divergence_f64')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, u, divU
+         integer(c_int), value :: p, nelem, npoin
+      end function
+      integer(c_int) function c_vorticity(c, p, nelem, npoin, npoin_w, lpoin_w, connec, lelpn, &
+            He, dl, inv, gI, gJ, gK, u, curlU) bind(C, name='sodhip_compute_vorticity_f64')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, lpoin_w, connec, lelpn, He, dl, inv, gI, gJ, gK, u, curlU
+         integer(c_int), value :: p, nelem, npoin, npoin_w
+      end function
+      integer(c_int) function c_ducros(c, p, nelem, npoin, connec, u, divU, vorti, Ml, helem, &
+            dmin, mu_e) bind(C, name='sodhip_ducros_visc_limit_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, u, divU, vorti, Ml, helem, mu_e
+         integer(c_int), value :: p, nelem, npoin
+         real(c_double), value :: dmin
       end function
    end interface
 
@@ -292,6 +327,16 @@ contains
       write(*,'(a,a,a,i0,a,a)') 'mod_sod2d_hip: ', what, ' failed (', rc, '): ', trim(msg)
       error stop 1
    end subroutine check
+
+   ! Module globals of the live-path sensor: ce_comp, ducros_min_val
+   ! (mod_numerical_params.f90:36,173) and the case's flag_high_mach
+   subroutine sod2d_hip_set_live_params(ce_comp_in, flag_high_mach_in, ducros_min_val_in)
+      real(rp), intent(in) :: ce_comp_in, ducros_min_val_in
+      logical, intent(in) :: flag_high_mach_in
+      ce_comp = ce_comp_in
+      flag_high_mach = flag_high_mach_in
+      ducros_min_val = ducros_min_val_in
+   end subroutine sod2d_hip_set_live_params
 
    type(c_ptr) function dev(host, what)
       type(*), dimension(*), intent(in) :: host
@@ -422,6 +467,91 @@ contains
          dev(invAtoIJK,'invAtoIJK'), prm, dev(mu_e,'mu_e'), dev(mue_l,'mue_l')), &
          'smart_visc_spectral')
    end subroutine smart_visc_spectral
+
+   ! mod_entropy_viscosity.f90:113-227, the live path's sensor.  Ngp, coord,
+   ! dNgp, gpvol, wgp, Tem, helem, Ml, invAtoIJK and gmshAtoI/J/K are part of
+   ! the reference's interface and are not read.
+   subroutine smart_visc_spectral_imex(nelem,npoin,npoin_w,connec,lpoin_w,Reta,Ngp,coord,dNgp, &
+         gpvol,wgp,gamma_gas,rho,u,csound,Tem,eta,helem,helem_k,Ml,mu_e,invAtoIJK,gmshAtoI, &
+         gmshAtoJ,gmshAtoK,mue_l)
+      integer(4), intent(in)  :: nelem, npoin,npoin_w, connec(nelem,nnode),lpoin_w(npoin_w)
+      real(rp),   intent(in)  :: Reta(npoin), Ngp(ngaus,nnode),gamma_gas
+      real(rp),   intent(in)  :: rho(npoin), u(npoin,ndime),csound(npoin), Tem(npoin), eta(npoin)
+      real(rp),   intent(in)  :: helem(nelem,nnode),helem_k(nelem),Ml(npoin)
+      real(rp),   intent(out) :: mu_e(nelem,ngaus)
+      real(rp),   intent(in)  :: coord(npoin,ndime), dNgp(ndime,nnode,ngaus), wgp(ngaus)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode),gmshAtoK(nnode)
+      real(rp),intent(inout)  :: mue_l(nelem,nnode)
+      vprm%flag_drop_c_in_envit = merge(1, 0, low_mach_dropped)
+      call check(c_visc_imex(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+         dev(lpoin_w,'lpoin_w'), dev(Reta,'Reta'), dev(rho,'rho'), dev(u,'u'), &
+         dev(csound,'csound'), dev(eta,'eta'), dev(helem_k,'helem_k'), vprm, ce_comp, &
+         merge(1, 0, flag_high_mach), dev(mu_e,'mu_e'), dev(mue_l,'mue_l')), &
+         'smart_visc_spectral_imex')
+   end subroutine smart_visc_spectral_imex
+
+   ! mod_operators.f90:485-570
+   subroutine eval_divergence(nelem,npoin,connec,He,gpvol,dlxigp_ip,invAtoIJK,gmshAtoI,gmshAtoJ, &
+         gmshAtoK,u,Rmom)
+      integer(4), intent(in)  :: nelem, npoin
+      integer(4), intent(in)  :: connec(nelem,nnode)
+      real(rp),   intent(in)  :: He(ndime,ndime,ngaus,nelem),dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)  :: u(npoin,ndime)
+      real(rp),   intent(out) :: Rmom(npoin)
+      call check(c_eval_div(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(He,'He'), &
+         dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), &
+         dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), &
+         dev(u,'u'), dev(Rmom,'Rmom')), 'eval_divergence')
+   end subroutine eval_divergence
+
+   ! mod_operators.f90:684-775.  dNgp, leviCivi (CFDSolverBase.f90:1605-1611's
+   ! Levi-Civita symbol) and atoIJK are part of the reference's interface and
+   ! are not read.
+   subroutine compute_vorticity(nelem,npoin,npoin_w,lpoin_w,connec,lelpn,He,dNgp,leviCivi, &
+         dlxigp_ip,atoIJK,invAtoIJK,gmshAtoI,gmshAtoJ,gmshAtoK,u,curlU,makeConti)
+      integer(4), intent(in)  :: nelem, npoin, npoin_w, lpoin_w(npoin), connec(nelem,nnode)
+      integer(4), intent(in)  :: lelpn(npoin)
+      real(rp),   intent(in)  :: He(ndime,ndime,ngaus,nelem), dNgp(ndime,nnode,ngaus)
+      real(rp),   intent(in)  :: u(npoin,ndime)
+      real(rp),   intent(in)  :: leviCivi(ndime,ndime,ndime)
+      real(rp),   intent(in)  :: dlxigp_ip(ngaus,ndime,porder+1)
+      integer(4), intent(in)  :: atoIJK(nnode),invAtoIJK(porder+1,porder+1,porder+1)
+      integer(4), intent(in)  :: gmshAtoI(nnode), gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(out) :: curlU(npoin,ndime)
+      logical,    intent(in)  :: makeConti
+      type(c_ptr) :: plel
+      plel = c_null_ptr
+      if (makeConti) plel = dev(lelpn,'lelpn')
+      call check(c_vorticity(ctx, porder, nelem, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), &
+         dev(connec,'connec'), plel, dev(He,'He'), dev(dlxigp_ip,'dlxigp_ip'), &
+         dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), &
+         dev(gmshAtoK,'gmshAtoK'), dev(u,'u'), dev(curlU,'curlU')), 'compute_vorticity')
+   end subroutine compute_vorticity
+
+   ! mod_entropy_viscosity.f90:277-314.  Ngp, coord, dNgp, gpvol, wgp, rho,
+   ! csound, helem_k, invAtoIJK, gmshAtoI/J/K and mue_l are part of the
+   ! reference's interface and do not enter the result.
+   subroutine ducros_visc_limit(nelem,npoin,npoin_w,connec,lpoin_w,Ngp,coord,dNgp,gpvol,wgp, &
+         rho,u,divU,vorti,csound,helem,helem_k,Ml,mu_e,invAtoIJK,gmshAtoI,gmshAtoJ,gmshAtoK,mue_l)
+      integer(4), intent(in)  :: nelem, npoin,npoin_w, connec(nelem,nnode),lpoin_w(npoin_w)
+      real(rp),   intent(in)  :: Ngp(ngaus,nnode),helem(nelem,nnode)
+      real(rp),   intent(in)  :: rho(npoin), u(npoin,ndime),csound(npoin),helem_k(nelem)
+      real(rp),   intent(in)  :: Ml(npoin),divU(npoin),vorti(npoin,ndime)
+      real(rp),   intent(inout) :: mu_e(nelem,ngaus)
+      real(rp),   intent(in)  :: coord(npoin,ndime), dNgp(ndime,nnode,ngaus), wgp(ngaus)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode),gmshAtoK(nnode)
+      real(rp),intent(inout)  :: mue_l(nelem,nnode)
+      call check(c_ducros(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(u,'u'), &
+         dev(divU,'divU'), dev(vorti,'vorti'), dev(Ml,'Ml'), dev(helem,'helem'), &
+         ducros_min_val, dev(mu_e,'mu_e')), 'ducros_visc_limit')
+   end subroutine ducros_visc_limit
 
    ! mod_solver.f90:40-59
    subroutine lumped_solver_scal_opt(npoin,npoin_w,lpoin_w,invMl,R)

@@ -17,6 +17,9 @@ program shim_driver
    real(8), allocatable :: mu_sgs(:,:), Ml(:), invMl(:), Reta(:), eta(:), helem_l(:,:), helem(:)
    real(8), allocatable :: csound(:), Rmass(:), Rmom(:,:), Rener(:), Rconv(:), mue_out(:,:)
    real(8), allocatable :: mue_l(:,:), rho1(:), q1(:,:), E1(:), rhoS(:), qS(:,:), ES(:)
+   real(8), allocatable :: mu_imex(:,:), mue_l2(:,:), divU(:), curlU(:,:)
+   integer(4), allocatable :: lelpn(:), atoIJK(:)
+   real(8) :: leviCivi(3,3,3)
    real(8) :: Cp, Prt, gamma_gas, ce, cmax, cglob, Rgas, dt, lam1, lams, gam
    integer(4) :: flag_norm
 
@@ -37,7 +40,9 @@ program shim_driver
    allocate(eta(npoin), helem_l(nelem,nnode), helem(nelem), csound(npoin))
    allocate(Rmass(npoin), Rmom(npoin,3), Rener(npoin), Rconv(npoin), mue_out(nelem,ngaus))
    allocate(mue_l(nelem,nnode), rho1(npoin), q1(npoin,3), E1(npoin), rhoS(npoin), qS(npoin,3))
-   allocate(ES(npoin))
+   allocate(ES(npoin), mu_imex(nelem,ngaus), mue_l2(nelem,nnode), divU(npoin), curlU(npoin,3))
+   allocate(lelpn(npoin), atoIJK(nnode))
+   leviCivi = 0.0d0; atoIJK = 0; mu_imex = 0.0d0; mue_l2 = 0.0d0
    Ngp = 0.0d0; dNgp = 0.0d0; xgp = 0.0d0; wgp = 0.0d0; coord = 0.0d0; Ml = 0.0d0
    csound = 0.0d0; mu_sgs = 0.0d0; mue_out = 0.0d0; mue_l = 0.0d0
 
@@ -49,6 +54,7 @@ program shim_driver
    call rd_r(mu_e, 'mu_e'); call rd_r(invMl, 'invMl'); call rd_r(Reta, 'Reta')
    call rd_r(eta, 'eta'); call rd_r(helem_l, 'helem_l'); call rd_r(helem, 'helem')
    call rd_r(rho1, 'rho1'); call rd_r(q1, 'q1'); call rd_r(E1, 'E1')
+   call rd_i(lelpn, 'lelpn'); call rd_r(csound, 'csound'); call rd_r(Ml, 'Ml')
    rhoS = rho; qS = q; ES = h
 
    call sod2d_hip_init(0)
@@ -68,6 +74,10 @@ program shim_driver
    call sod2d_hip_enter_data(mue_out, .false.); call sod2d_hip_enter_data(mue_l)
    call sod2d_hip_enter_data(rho1); call sod2d_hip_enter_data(q1); call sod2d_hip_enter_data(E1)
    call sod2d_hip_enter_data(rhoS); call sod2d_hip_enter_data(qS); call sod2d_hip_enter_data(ES)
+   call sod2d_hip_enter_data(lelpn); call sod2d_hip_enter_data(csound)
+   call sod2d_hip_enter_data(Ml); call sod2d_hip_enter_data(mu_imex)
+   call sod2d_hip_enter_data(mue_l2); call sod2d_hip_enter_data(divU, .false.)
+   call sod2d_hip_enter_data(curlU, .false.)
 
    ! the RHS of updateF: convection (initialising) then diffusion (accumulating),
    ! both with fact = -1 (time_integ_ls.f90:605-640)
@@ -100,6 +110,22 @@ program shim_driver
                             mue_l)
    call sod2d_hip_update_host(mue_out)
    call wr(mue_out, 'out_mu_e')
+
+   ! the live path's sensor and limiter (time_integ_ls.f90:455-463)
+   call smart_visc_spectral_imex(nelem,npoin,npoin_w,connec,lpoin_w,Reta,Ngp,coord,dNgp,gpvol, &
+                                 wgp,gamma_gas,rho,u,csound,Tem,eta,helem_l,helem,Ml,mu_imex,inv, &
+                                 gI,gJ,gK,mue_l2)
+   call sod2d_hip_update_host(mu_imex)
+   call wr(mu_imex, 'out_mu_imex')
+   call eval_divergence(nelem,npoin,connec,He,gpvol,dl,inv,gI,gJ,gK,u,divU)
+   call compute_vorticity(nelem,npoin,npoin_w,lpoin_w,connec,lelpn,He,dNgp,leviCivi,dl,atoIJK, &
+                          inv,gI,gJ,gK,u,curlU,.true.)
+   call sod2d_hip_update_host(divU); call sod2d_hip_update_host(curlU)
+   call wr(divU, 'out_divU'); call wr(curlU, 'out_curlU')
+   call ducros_visc_limit(nelem,npoin,npoin_w,connec,lpoin_w,Ngp,coord,dNgp,gpvol,wgp,rho,u, &
+                          divU,curlU,csound,helem_l,helem,Ml,mu_imex,inv,gI,gJ,gK,mue_l2)
+   call sod2d_hip_update_host(mu_imex)
+   call wr(mu_imex, 'out_mu_ducros')
 
    call sod2d_hip_synchronize()
    call sod2d_hip_finalize()

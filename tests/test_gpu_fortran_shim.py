@@ -1,7 +1,8 @@
 """The Fortran layer (sod2d_amd/fortran/mod_sod2d_hip.f90) keeps the
 reference's subroutine interfaces of full_convec_ijk, full_diffusion_ijk,
-generic_scalar_convec_ijk, smart_visc_spectral and lumped_solver_scal_opt and
-adds the fused stage combine.  The test program tests/fortran/shim_driver.f90
+generic_scalar_convec_ijk, smart_visc_spectral, lumped_solver_scal_opt and the
+live path's smart_visc_spectral_imex, eval_divergence, compute_vorticity and
+ducros_visc_limit, and adds the fused stage combine.  The test program tests/fortran/shim_driver.f90
 calls them, through the C-ABI, on a case written here; its results are checked
 against the oracle: the residuals, the stage combine and the sensor to 1e-13
 normwise relative (the device arithmetic is reordered, so not bit-exact)."""
@@ -44,6 +45,7 @@ def test_fortran_shim(oracle, tmp_path, p, n, periodic, jitter):
     q1 = np.asfortranarray(s["q"] * 0.99)
     E1 = s["h"] * 1.02
     dt, lam1, lams, gam = 1e-3, 0.3, 0.7, 0.45
+    csound = r.uniform(0.9, 1.1, npoin) * 300.0
     prm = oracle.visc_params(Rgas)
     d = str(tmp_path)
     with open(os.path.join(d, "meta.txt"), "w") as f:
@@ -51,13 +53,15 @@ def test_fortran_shim(oracle, tmp_path, p, n, periodic, jitter):
         f.write(f"{prm.ce!r} {prm.cmax!r} {prm.cglob!r} {Rgas!r} 1\n")
         f.write(f"{dt!r} {lam1!r} {lams!r} {gam!r}\n")
     for name, a in [("connec", m.connec), ("lpoin_w", m.lpoin_w), ("invAtoIJK", T.invAtoIJK),
-                    ("gI", T.gI), ("gJ", T.gJ), ("gK", T.gK)]:
+                    ("gI", T.gI), ("gJ", T.gJ), ("gK", T.gK),
+                    ("lelpn", oracle.elem_per_node(m.connec, npoin))]:
         _w(d, name, np.asarray(a, dtype=np.int32))
     for name, a in [("He", g.He), ("gpvol", g.gpvol), ("dlxigp_ip", T.dlxigp_ip), ("u", s["u"]),
                     ("q", s["q"]), ("rho", s["rho"]), ("pr", s["pr"]), ("h", s["h"]),
                     ("Tem", s["Tem"]), ("mu_fluid", s["mu_fluid"]), ("mu_e", s["mu_e"]),
                     ("invMl", g.invMl), ("Reta", Reta), ("eta", eta), ("helem_l", g.helem_l),
-                    ("helem", g.helem), ("rho1", rho1), ("q1", q1), ("E1", E1)]:
+                    ("helem", g.helem), ("rho1", rho1), ("q1", q1), ("E1", E1),
+                    ("csound", csound), ("Ml", g.Ml)]:
         _w(d, name, np.asarray(a, dtype=np.float64))
     print(f"fortran shim case p={p} n={n} periodic={periodic}", flush=True)
     res = subprocess.run([exe, d], capture_output=True, text=True, timeout=120)
@@ -90,3 +94,15 @@ def test_fortran_shim(oracle, tmp_path, p, n, periodic, jitter):
     oracle.smart_visc_spectral(T, m.connec, m.lpoin_w, Reta, gamma, s["rho"], s["u"], s["Tem"],
                                eta, g.helem_l, g.helem, prm, mu_e)
     assert relerr(_r(d, "out_mu_e", (nelem, nn)), mu_e) < 1e-13
+    # live path: imex sensor, divergence, vorticity, Ducros limit
+    mu_i, _ = oracle.smart_visc_spectral_imex(T, m.connec, m.lpoin_w, Reta, s["rho"], s["u"],
+                                              csound, eta, g.helem, oracle.visc_params(Rgas),
+                                              np.zeros((nelem, nn), order="F"))
+    assert relerr(_r(d, "out_mu_imex", (nelem, nn)), mu_i) < 1e-13
+    divU = oracle.eval_divergence(T, m.connec, g.He, g.gpvol, s["u"])
+    curl = oracle.compute_vorticity(T, m.connec, m.lpoin_w, oracle.elem_per_node(m.connec, npoin),
+                                    g.He, s["u"])
+    assert relerr(_r(d, "out_divU", npoin), divU) < 1e-13
+    assert relerr(_r(d, "out_curlU", (npoin, 3)), curl) < 1e-13
+    oracle.ducros_visc_limit(T, m.connec, s["u"], divU, curl, g.Ml, g.helem_l, mu_i, 0.1)
+    assert relerr(_r(d, "out_mu_ducros", (nelem, nn)), mu_i) < 1e-13
