@@ -19,7 +19,10 @@ __global__ __launch_bounds__(256) void k_lumped(int npoin, int npoin_w,
 }
 
 // R <- R*invMl (stored back), y <- lam1*y1 + lams*y + gdt*R over working nodes
-// for the five conserved fields (time_integ_ls.f90:323-331,350-358).
+// for the five conserved fields (time_integ_ls.f90:323-331,350-358).  RD1 / RDY:
+// y1 / y enters (non-zero coefficient); otherwise the term is 0 * 0, the same
+// value as 0 * (finite y), without reading the field.
+template <bool RD1, bool RDY>
 __global__ __launch_bounds__(256) void k_rk_update(int npoin, int npoin_w,
                                                    const int *__restrict__ lpoin_w,
                                                    const double *__restrict__ invMl, double lam1,
@@ -40,24 +43,24 @@ __global__ __launch_bounds__(256) void k_rk_update(int npoin, int npoin_w,
         const double m = invMl[n];
         const double rm = Rmass[n] * m;
         Rmass[n] = rm;
-        rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * rm;
+        rho[n] = lam1 * (RD1 ? rho1[n] : 0.0) + lams * (RDY ? rho[n] : 0.0) + gdt * rm;
         const double re = Rener[n] * m;
         Rener[n] = re;
-        E[n] = lam1 * E1[n] + lams * E[n] + gdt * re;
+        E[n] = lam1 * (RD1 ? E1[n] : 0.0) + lams * (RDY ? E[n] : 0.0) + gdt * re;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t k = n + (size_t)npoin * d;
             const double rq = Rmom[k] * m;
             Rmom[k] = rq;
-            q[k] = lam1 * q1[k] + lams * q[k] + gdt * rq;
+            q[k] = lam1 * (RD1 ? q1[k] : 0.0) + lams * (RDY ? q[k] : 0.0) + gdt * rq;
         }
     } else { // R already carries the mass inverse (the residual kept from updateF)
-        rho[n] = lam1 * rho1[n] + lams * rho[n] + gdt * Rmass[n];
-        E[n] = lam1 * E1[n] + lams * E[n] + gdt * Rener[n];
+        rho[n] = lam1 * (RD1 ? rho1[n] : 0.0) + lams * (RDY ? rho[n] : 0.0) + gdt * Rmass[n];
+        E[n] = lam1 * (RD1 ? E1[n] : 0.0) + lams * (RDY ? E[n] : 0.0) + gdt * Rener[n];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t k = n + (size_t)npoin * d;
-            q[k] = lam1 * q1[k] + lams * q[k] + gdt * Rmom[k];
+            q[k] = lam1 * (RD1 ? q1[k] : 0.0) + lams * (RDY ? q[k] : 0.0) + gdt * Rmom[k];
         }
     }
 }
@@ -106,9 +109,19 @@ extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w
     if (npoin_w == 0) return SODHIP_OK;
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     int tok = prof_begin(ctx, 3);
-    k_rk_update<<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(
-        npoin, npoin_w, lpoin_w, invMl, lam1, lams, gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom,
-        Rener);
+#define SODHIP_RKU(R1, RY)                                                                     \
+    k_rk_update<R1, RY><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(                       \
+        npoin, npoin_w, lpoin_w, invMl, lam1, lams, gam * dt, rho1, q1, E1, rho, q, E, Rmass,  \
+        Rmom, Rener)
+    if (lam1 != 0.0 && lams != 0.0)
+        SODHIP_RKU(true, true);
+    else if (lam1 != 0.0)
+        SODHIP_RKU(true, false);
+    else if (lams != 0.0)
+        SODHIP_RKU(false, true);
+    else
+        SODHIP_RKU(false, false);
+#undef SODHIP_RKU
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;

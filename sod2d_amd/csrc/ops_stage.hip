@@ -27,7 +27,10 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
                        double mu_factor, double c_rho, double c_ener, double p_ref, T *out);
 
-template <typename TS>
+// RD1 / RDY: the base state y1 / the current state y enters the combine (its
+// coefficient lam1 / lams is non-zero); otherwise the term is 0 * 0, the same
+// value as 0 * (finite y), without reading the field.
+template <typename TS, bool RD1, bool RDY>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
     const int *__restrict__ wsrc, const TS *__restrict__ scr, size_t EN,
@@ -49,8 +52,10 @@ __global__ __launch_bounds__(256) void k_gather_update(
     // the slot-index -> partial chain
     const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
     const double m = invMl[n];
-    const double b0 = rho1[n], b1 = q1[n], b2 = q1[n1], b3 = q1[n2], b4 = E1[n];
-    const double y0 = rho[n], y1 = q[n], y2 = q[n1], y3 = q[n2], y4 = E[n];
+    const double b0 = RD1 ? rho1[n] : 0.0, b1 = RD1 ? q1[n] : 0.0, b2 = RD1 ? q1[n1] : 0.0,
+                 b3 = RD1 ? q1[n2] : 0.0, b4 = RD1 ? E1[n] : 0.0;
+    const double y0 = RDY ? rho[n] : 0.0, y1 = RDY ? q[n] : 0.0, y2 = RDY ? q[n1] : 0.0,
+                 y3 = RDY ? q[n2] : 0.0, y4 = RDY ? E[n] : 0.0;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     // four slots per step, all loads issued before the (in-order) additions
     for (int j = j0; j < j1; j += 4) {
@@ -107,9 +112,20 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
     int tok = prof_begin(ctx, 1);
-    k_gather_update<T><<<((npoin_w + 255) / 256 + 7) / 8 * 8, 256, 0, ctx->stream>>>(
-        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, prm->fact, lam1, lams, gam * dt,
-        rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener);
+    const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
+#define SODHIP_GU(R1, RY)                                                                      \
+    k_gather_update<T, R1, RY><<<grid, 256, 0, ctx->stream>>>(                                 \
+        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, prm->fact, lam1, lams,      \
+        gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener)
+    if (lam1 != 0.0 && lams != 0.0)
+        SODHIP_GU(true, true);
+    else if (lam1 != 0.0)
+        SODHIP_GU(true, false);
+    else if (lams != 0.0)
+        SODHIP_GU(false, true);
+    else
+        SODHIP_GU(false, false);
+#undef SODHIP_GU
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
