@@ -195,14 +195,14 @@ def full_diffusion_ijk(T, connec, He, gpvol, Cp, Pr, rho_n, rho, u, Tem, mu_flui
 
 
 def generic_scalar_convec_ijk(T, connec, He, gpvol, q, eta, u, Rconvec=None, initialize=True,
-                              fact=1.0):
+                              fact=1.0, nthreads=1):
     npoin = eta.shape[0]
     if Rconvec is None:
         Rconvec = np.zeros(npoin)
     lib().or_generic_scalar_convec_ijk(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
                                        _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ),
                                        _i(T.gK), _d(q), _d(eta), _d(u), _d(Rconvec),
-                                       int(bool(initialize)), C.c_double(fact))
+                                       int(bool(initialize)), C.c_double(fact), int(nthreads))
     return Rconvec
 
 
@@ -393,11 +393,15 @@ class TGVOracle:
 
     ``geo`` provides the mesh and geometry (connec_orig/connec/lpoin_w/coord,
     He, gpvol, invMl, helem, helem_l, vol_tot) and ``ic`` the initial fields;
-    ``t`` the TGV constants (an oracle TGVParams); ``cfl`` (conv, diff)."""
+    ``t`` the TGV constants (an oracle TGVParams); ``cfl`` (conv, diff);
+    ``nthreads`` the OpenMP threads of the three element operators (their
+    assembly then adds under atomics, so the summation order, and with it the
+    last bits, depend on the thread schedule)."""
 
     def __init__(self, p, geo, ic, t, stages=5, variant=1, parity=True, dt_mode="adaptive",
-                 cfl=(0.5, 0.5), sensor=True, sensor_path="spectral"):
+                 cfl=(0.5, 0.5), sensor=True, sensor_path="spectral", nthreads=1):
         self.p = p
+        self.nthreads = nthreads
         self.T = Tables(p)
         self.geo = geo
         m = geo.mesh
@@ -455,10 +459,12 @@ class TGVOracle:
                   self.e_int[1], self.pr[1], self.aux_h, self.Tem[1])
         sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
         full_convec_ijk(self.T, self.connec, g.He, g.gpvol, self.u[1], self.q[1], self.rho[1],
-                        self.pr[1], self.aux_h, self.Rmass, self.Rmom, self.Rener, True, -1.0)
+                        self.pr[1], self.aux_h, self.Rmass, self.Rmom, self.Rener, True, -1.0,
+                        nthreads=self.nthreads)
         full_diffusion_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, self.rho[1],
                            self.rho[1], self.u[1], self.Tem[1], self.mu_fluid, self.mu_e,
-                           self.mu_sgs, self.Rmass, self.Rmom, self.Rener, False, -1.0)
+                           self.mu_sgs, self.Rmass, self.Rmom, self.Rener, False, -1.0,
+                           nthreads=self.nthreads)
         lumped_solver_scal(lw, g.invMl, self.Rmass)
         lumped_solver_scal(lw, g.invMl, self.Rener)
         lumped_solver_vect(lw, g.invMl, self.Rmom)
@@ -466,7 +472,8 @@ class TGVOracle:
     def _scalar_convec(self, col):
         out = np.zeros(self.npoin)
         generic_scalar_convec_ijk(self.T, self.connec, self.geo.He, self.geo.gpvol, self.f_eta,
-                                  np.ascontiguousarray(self.eta[:, 0]), self.u[0], out)
+                                  np.ascontiguousarray(self.eta[:, 0]), self.u[0], out,
+                                  nthreads=self.nthreads)
         lumped_solver_scal(self.lpoin_w, self.geo.invMl, out)
         self.Reta[:, col] = out
 

@@ -754,12 +754,15 @@ void or_generic_scalar_convec_ijk(int p, int nelem, int npoin, const int *connec
                                   const double *gpvol, const double *dlxigp_ip,
                                   const int *invAtoIJK, const int *gI, const int *gJ,
                                   const int *gK, const double *q, const double *eta,
-                                  const double *u, double *Rconvec, int initialize, double fact)
+                                  const double *u, double *Rconvec, int initialize, double fact,
+                                  int nthreads)
 {
     const int n1 = p + 1, ngaus = n1 * n1 * n1, nnode = ngaus;
     if (initialize)
         for (int n = 0; n < npoin; ++n) Rconvec[n] = 0.0;
     if (nnode > 128) return;
+    const int par = nthreads > 1;
+#pragma omp parallel for schedule(static) num_threads(par ? nthreads : 1) if (par)
     for (int e = 0; e < nelem; ++e) {
         int ip[128];
         double ul[128][3], fel[128][3], etal[128], Re[128];
@@ -798,7 +801,7 @@ void or_generic_scalar_convec_ijk(int p, int nelem, int npoin, const int *connec
             for (int i = 0; i < 3; ++i) Re[g] = Re[g] + 0.5 * (ul[g][i] * gradE[i]);
             Re[g] = GPVOL(g, e) * Re[g];
         }
-        for (int a = 0; a < nnode; ++a) Rconvec[ip[a]] = Rconvec[ip[a]] + fact * Re[a];
+        for (int a = 0; a < nnode; ++a) atomic_add(&Rconvec[ip[a]], fact * Re[a], par);
     }
 }
 

@@ -69,7 +69,8 @@ void or_generic_scalar_convec_ijk(int p, int nelem, int npoin, const int *connec
                                   const double *gpvol, const double *dlxigp_ip,
                                   const int *invAtoIJK, const int *gI, const int *gJ,
                                   const int *gK, const double *q, const double *eta,
-                                  const double *u, double *Rconvec, int initialize, double fact);
+                                  const double *u, double *Rconvec, int initialize, double fact,
+                                  int nthreads);
 
 /* ---- entropy viscosity (mod_entropy_viscosity.f90, mod_filters.f90) ---- */
 void or_convert_ijk(int p, int *convertIJK /* size p+3, index 0..p+2 */);
