@@ -164,6 +164,47 @@ int sodhip_full_diffusion_ijk_f32(sodhip_ctx ctx, int porder, int nelem, int npo
                                   float c_ener, float *Rmass, float *Rmom, float *Rener,
                                   int initialize, float fact);
 
+/* LPS stabilisation (elem_stab.f90), the reference's default viscous path
+ * (flag_lps_stab, time_integ_ls.f90:266-268,607-611).
+ *
+ * comp_tau (elem_stab.f90:589-616): tau(e) = max over the element's nodes of
+ * helem_k(e) * (c_species_stab/p) * (c + |u|); the sound speed is dropped when
+ * flag_drop_c_in_envit is set.  c_species_stab is the module constant of
+ * mod_numerical_params.f90 (0.5).  The reference's dt argument is unused. */
+int sodhip_comp_tau_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                        const double *csound, const double *u, const double *helem_k,
+                        double c_species_stab, int flag_drop_c_in_envit, double *tau);
+
+/* full_proj_ijk (elem_stab.f90:441-586): lumped L2 projections of grad rho
+ * (ProjMass), grad T (ProjEner) and of the rows of 2 S - 2/3 div u I
+ * (ProjMX/MY/MZ), each (npoin,3); overwritten, then multiplied by invMl over
+ * the working nodes.  The reference's Ngp, Cp, Pr and Ml are unused. */
+int sodhip_full_proj_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                             const int *connec, const int *lpoin_w, const double *He,
+                             const double *gpvol, const double *dlxigp_ip,
+                             const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                             const int *gmshAtoK, const double *rho, const double *u,
+                             const double *Tem, const double *invMl, double *ProjMass,
+                             double *ProjEner, double *ProjMX, double *ProjMY, double *ProjMZ);
+
+/* full_diff_stab_ijk (elem_stab.f90:207-438): the viscous operator with the
+ * entropy-viscosity coefficients applied per quadrature point and the LPS
+ * fluxes -c_lps tau_stab(e) (Proj - grad); c_lps is c_lps_comp
+ * (mod_numerical_params.f90:31, a default-real 0.1).  initialize/fact as
+ * full_diffusion_ijk; mu_sgs may be NULL (zero). */
+int sodhip_full_diff_stab_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                  const int *connec, const double *He, const double *gpvol,
+                                  const double *dlxigp_ip, const int *invAtoIJK,
+                                  const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                                  double Cp, double Pr, const double *rho_n, const double *rho,
+                                  const double *u, const double *Tem, const double *mu_fluid,
+                                  const double *mu_e, const double *mu_sgs,
+                                  const double *ProjMass, const double *ProjEner,
+                                  const double *ProjMX, const double *ProjMY,
+                                  const double *ProjMZ, const double *tau_stab, double c_rho,
+                                  double c_ener, double c_lps, double *Rmass, double *Rmom,
+                                  double *Rener, int initialize, double fact);
+
 /* generic_scalar_convec_ijk (elem_convec.f90:472-594): q is the flux f(npoin,3),
  * eta the transported scalar, u the velocity; output Rconvec(npoin). */
 int sodhip_generic_scalar_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,

@@ -194,6 +194,49 @@ def full_diffusion_ijk(T, connec, He, gpvol, Cp, Pr, rho_n, rho, u, Tem, mu_flui
     return Rmass, Rmom, Rener
 
 
+C_LPS_COMP = float(np.float32(0.1))  # c_lps_comp is a default-real constant
+C_SPECIES_STAB = 0.5
+
+
+def comp_tau(p, connec, csound, u, helem_k, c_species_stab=C_SPECIES_STAB, drop_c=False):
+    """comp_tau (elem_stab.f90:589-616): the LPS time scale per element."""
+    tau = np.zeros(connec.shape[0])
+    lib().or_comp_tau(p, connec.shape[0], csound.shape[0], _i(connec), _d(csound), _d(u),
+                      _d(helem_k), C.c_double(c_species_stab), int(bool(drop_c)), _d(tau))
+    return tau
+
+
+def full_proj_ijk(T, connec, lpoin_w, He, gpvol, rho, u, Tem, invMl, nthreads=1):
+    """full_proj_ijk (elem_stab.f90:441-586): returns ProjMass, ProjEner, ProjMX,
+    ProjMY, ProjMZ, each (npoin,3)."""
+    npoin = rho.shape[0]
+    P = [fzeros((npoin, 3)) for _ in range(5)]
+    lib().or_full_proj_ijk(T.p, connec.shape[0], npoin, lpoin_w.shape[0], _i(connec),
+                           _i(lpoin_w), _d(He), _d(gpvol), _d(T.dlxigp_ip), _i(T.invAtoIJK),
+                           _i(T.gI), _i(T.gJ), _i(T.gK), _d(rho), _d(u), _d(Tem), _d(invMl),
+                           *[_d(x) for x in P], int(nthreads))
+    return P
+
+
+def full_diff_stab_ijk(T, connec, He, gpvol, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs,
+                       proj, tau_stab, Rmass=None, Rmom=None, Rener=None, initialize=True,
+                       fact=1.0, c_rho=1.0, c_ener=1.0, c_lps=C_LPS_COMP, nthreads=1):
+    """full_diff_stab_ijk (elem_stab.f90:207-438); ``proj`` is full_proj_ijk's
+    (ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ)."""
+    npoin = rho.shape[0]
+    if Rmass is None:
+        Rmass, Rmom, Rener = np.zeros(npoin), fzeros((npoin, 3)), np.zeros(npoin)
+    lib().or_full_diff_stab_ijk(T.p, connec.shape[0], npoin, _i(connec), _d(He), _d(gpvol),
+                                _d(T.dlxigp_ip), _i(T.invAtoIJK), _i(T.gI), _i(T.gJ), _i(T.gK),
+                                C.c_double(Cp), C.c_double(Pr), _d(rho_n), _d(rho), _d(u),
+                                _d(Tem), _d(mu_fluid), _d(mu_e), _d(mu_sgs),
+                                *[_d(x) for x in proj], _d(tau_stab), C.c_double(c_rho),
+                                C.c_double(c_ener), C.c_double(c_lps), _d(Rmass), _d(Rmom),
+                                _d(Rener), int(bool(initialize)), C.c_double(fact),
+                                int(nthreads))
+    return Rmass, Rmom, Rener
+
+
 def generic_scalar_convec_ijk(T, connec, He, gpvol, q, eta, u, Rconvec=None, initialize=True,
                               fact=1.0, nthreads=1):
     npoin = eta.shape[0]
@@ -389,7 +432,12 @@ class TGVOracle:
     or the reference's live path (``sensor_path`` "live"):
     smart_visc_spectral_imex at the first step (time_integ_ls.f90:292) and,
     after each step, smart_visc_spectral_imex, eval_divergence,
-    compute_vorticity and ducros_visc_limit (:455-463).
+    compute_vorticity and ducros_visc_limit (:455-463).  ``high_mach`` selects
+    the high-Mach normalisation of smart_visc_spectral_imex (flag_high_mach,
+    which TGVCompSolver sets, TGVCompSolver.f90:128) and ``lps`` the LPS
+    stabilisation of updateF (flag_lps_stab, :607-611): comp_tau at the start
+    of each step (:266-268), then full_proj_ijk and full_diff_stab_ijk in
+    place of full_diffusion_ijk.
 
     ``geo`` provides the mesh and geometry (connec_orig/connec/lpoin_w/coord,
     He, gpvol, invMl, helem, helem_l, vol_tot) and ``ic`` the initial fields;
@@ -399,9 +447,12 @@ class TGVOracle:
     last bits, depend on the thread schedule)."""
 
     def __init__(self, p, geo, ic, t, stages=5, variant=1, parity=True, dt_mode="adaptive",
-                 cfl=(0.5, 0.5), sensor=True, sensor_path="spectral", nthreads=1):
+                 cfl=(0.5, 0.5), sensor=True, sensor_path="spectral", nthreads=1, lps=False,
+                 high_mach=False):
         self.p = p
         self.nthreads = nthreads
+        self.lps = lps
+        self.high_mach = high_mach
         self.T = Tables(p)
         self.geo = geo
         m = geo.mesh
@@ -442,6 +493,7 @@ class TGVOracle:
         self.f_eta = fzeros((self.npoin, 3))
         self.aux_h = np.zeros(self.npoin)
         self.prm = visc_params(t.Rgas)
+        self.tau_stab = np.zeros(self.nelem)
         self.first = True
         self.time = 0.0
         sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
@@ -453,7 +505,7 @@ class TGVOracle:
                             self.rho[1])
 
     def update_f(self):
-        """updateF (time_integ_ls.f90:489-691) with flag_lps_stab = .false."""
+        """updateF (time_integ_ls.f90:489-691)."""
         lw, t, g = self.lpoin_w, self.t, self.geo
         eos_stage(lw, t.gamma_gas, t.Rgas, self.rho[1], self.q[1], self.E[1], self.u[1],
                   self.e_int[1], self.pr[1], self.aux_h, self.Tem[1])
@@ -461,10 +513,18 @@ class TGVOracle:
         full_convec_ijk(self.T, self.connec, g.He, g.gpvol, self.u[1], self.q[1], self.rho[1],
                         self.pr[1], self.aux_h, self.Rmass, self.Rmom, self.Rener, True, -1.0,
                         nthreads=self.nthreads)
-        full_diffusion_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, self.rho[1],
-                           self.rho[1], self.u[1], self.Tem[1], self.mu_fluid, self.mu_e,
-                           self.mu_sgs, self.Rmass, self.Rmom, self.Rener, False, -1.0,
-                           nthreads=self.nthreads)
+        if self.lps:
+            proj = full_proj_ijk(self.T, self.connec, lw, g.He, g.gpvol, self.rho[1], self.u[1],
+                                 self.Tem[1], g.invMl, nthreads=self.nthreads)
+            full_diff_stab_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, self.rho[1],
+                               self.rho[1], self.u[1], self.Tem[1], self.mu_fluid, self.mu_e,
+                               self.mu_sgs, proj, self.tau_stab, self.Rmass, self.Rmom,
+                               self.Rener, False, -1.0, nthreads=self.nthreads)
+        else:
+            full_diffusion_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, self.rho[1],
+                               self.rho[1], self.u[1], self.Tem[1], self.mu_fluid, self.mu_e,
+                               self.mu_sgs, self.Rmass, self.Rmom, self.Rener, False, -1.0,
+                               nthreads=self.nthreads)
         lumped_solver_scal(lw, g.invMl, self.Rmass)
         lumped_solver_scal(lw, g.invMl, self.Rener)
         lumped_solver_vect(lw, g.invMl, self.Rmom)
@@ -486,7 +546,8 @@ class TGVOracle:
         g = self.geo
         smart_visc_spectral_imex(self.T, self.connec, self.lpoin_w, np.ascontiguousarray(Reta),
                                  rho, u, self.csound, np.ascontiguousarray(eta), g.helem,
-                                 self.prm, self.mu_e, self.mue_l)
+                                 self.prm, self.mu_e, self.mue_l,
+                                 flag_high_mach=int(self.high_mach))
         if ducros:
             divU = eval_divergence(self.T, self.connec, g.He, g.gpvol, u)
             vorti = compute_vorticity(self.T, self.connec, self.lpoin_w, self.lelpn, g.He, u)
@@ -499,6 +560,8 @@ class TGVOracle:
         self.eta[:, 0] = self.eta[:, 1]
         lw = self.lpoin_w - 1
         L, G = self.L, self.G
+        if self.lps:
+            self.tau_stab = comp_tau(self.p, self.connec, self.csound, self.u[1], self.geo.helem)
         if self.first:
             self.first = False
             self.f_eta[lw, :] = self.u[0][lw, :] * self.eta[lw, 0][:, None]

@@ -749,6 +749,243 @@ void or_full_diffusion_ijk(int p, int nelem, int npoin, const int *connec, const
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* LPS stabilisation                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Reference-space gradients of rho, T and u at GP g of element e from the
+ * gathered nodal values (the first loops of elem_stab.f90:300-335 and
+ * :497-531), then physical ones through He. */
+static void lps_gradients(int p, int nelem, int e, int g, const double *He,
+                          const double *dlxigp_ip, const int *invAtoIJK, const int *gI,
+                          const int *gJ, const int *gK, const double *rhol, const double *Teml,
+                          const double (*ul)[3], double gradRho[3], double gradT[3],
+                          double gradU[3][3])
+{
+    const int n1 = p + 1, ngaus = n1 * n1 * n1;
+    int I = gI[g] - 1, J = gJ[g] - 1, K = gK[g] - 1;
+    double gRho[3] = {0}, gT[3] = {0}, gU[3][3] = {{0}};
+    for (int ii = 0; ii < n1; ++ii) {
+        int a1 = INV(ii, J, K), a2 = INV(I, ii, K), a3 = INV(I, J, ii);
+        gRho[0] = gRho[0] + DL(g, 0, ii) * rhol[a1];
+        gRho[1] = gRho[1] + DL(g, 1, ii) * rhol[a2];
+        gRho[2] = gRho[2] + DL(g, 2, ii) * rhol[a3];
+        gT[0] = gT[0] + DL(g, 0, ii) * Teml[a1];
+        gT[1] = gT[1] + DL(g, 1, ii) * Teml[a2];
+        gT[2] = gT[2] + DL(g, 2, ii) * Teml[a3];
+        for (int i = 0; i < 3; ++i) {
+            gU[i][0] = gU[i][0] + DL(g, 0, ii) * ul[a1][i];
+            gU[i][1] = gU[i][1] + DL(g, 1, ii) * ul[a2][i];
+            gU[i][2] = gU[i][2] + DL(g, 2, ii) * ul[a3][i];
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        gradRho[i] = 0.0;
+        gradT[i] = 0.0;
+        for (int j = 0; j < 3; ++j) gradU[i][j] = 0.0;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            gradRho[i] = gradRho[i] + HE(i, j, g, e) * gRho[j];
+            gradT[i] = gradT[i] + HE(i, j, g, e) * gT[j];
+            for (int k = 0; k < 3; ++k) gradU[i][j] = gradU[i][j] + HE(j, k, g, e) * gU[i][k];
+        }
+}
+
+/* comp_tau (elem_stab.f90:589-616): tau(e) = max over the element's nodes of
+ * helem_k(e) * (c_species_stab / p) * (f * c + |u|), f = 0 when the sound
+ * speed is dropped (flag_drop_c_in_envit), else 1. */
+void or_comp_tau(int p, int nelem, int npoin, const int *connec, const double *csound,
+                 const double *u, const double *helem_k, double c_species_stab, int drop_c,
+                 double *tau)
+{
+    const int nnode = (p + 1) * (p + 1) * (p + 1);
+    const double fact_low_mach = drop_c ? 0.0 : 1.0;
+    for (int e = 0; e < nelem; ++e) {
+        double taul = 0.0;
+        for (int a = 0; a < nnode; ++a) {
+            int n = CONNEC(e, a);
+            double uu = 0.0;
+            for (int d = 0; d < 3; ++d) uu = uu + NODAL(u, n, d) * NODAL(u, n, d);
+            double aux1 = fact_low_mach * csound[n] + sqrt(uu);
+            taul = fmax(taul, helem_k[e] * (c_species_stab / (double)p) * aux1);
+        }
+        tau[e] = taul;
+    }
+}
+
+/* full_proj_ijk (elem_stab.f90:441-586): lumped L2 projections of grad rho
+ * (ProjMass), grad T (ProjEner) and the rows of the traceless strain
+ * 2 S - 2/3 div u I (ProjMX/MY/MZ), each (npoin,3). */
+void or_full_proj_ijk(int p, int nelem, int npoin, int npoin_w, const int *connec,
+                      const int *lpoin_w, const double *He, const double *gpvol,
+                      const double *dlxigp_ip, const int *invAtoIJK, const int *gI, const int *gJ,
+                      const int *gK, const double *rho, const double *u, const double *Tem,
+                      const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
+                      double *ProjMY, double *ProjMZ, int nthreads)
+{
+    const int n1 = p + 1, ngaus = n1 * n1 * n1, nnode = ngaus;
+    const double twoThirds = 2.0 / 3.0;
+    for (size_t n = 0; n < (size_t)npoin * 3; ++n) {
+        ProjMass[n] = 0.0;
+        ProjEner[n] = 0.0;
+        ProjMX[n] = 0.0;
+        ProjMY[n] = 0.0;
+        ProjMZ[n] = 0.0;
+    }
+    if (nnode > 128) return;
+    const int par = nthreads > 1;
+#pragma omp parallel for schedule(static) num_threads(par ? nthreads : 1) if (par)
+    for (int e = 0; e < nelem; ++e) {
+        int ip[128];
+        double ul[128][3], rhol[128], Teml[128];
+        for (int a = 0; a < nnode; ++a) {
+            ip[a] = CONNEC(e, a);
+            rhol[a] = rho[ip[a]];
+            Teml[a] = Tem[ip[a]];
+            for (int d = 0; d < 3; ++d) ul[a][d] = NODAL(u, ip[a], d);
+        }
+        for (int g = 0; g < ngaus; ++g) {
+            double gradRho[3], gradT[3], gradU[3][3], tau[3][3];
+            lps_gradients(p, nelem, e, g, He, dlxigp_ip, invAtoIJK, gI, gJ, gK, rhol, Teml,
+                          (const double(*)[3])ul, gradRho, gradT, gradU);
+            double divU = gradU[0][0] + gradU[1][1] + gradU[2][2];
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j) tau[i][j] = (gradU[i][j] + gradU[j][i]);
+                tau[i][i] = tau[i][i] - twoThirds * divU;
+            }
+            double w = GPVOL(g, e);
+            for (int d = 0; d < 3; ++d) {
+                atomic_add(&NODAL(ProjMX, ip[g], d), w * tau[0][d], par);
+                atomic_add(&NODAL(ProjMY, ip[g], d), w * tau[1][d], par);
+                atomic_add(&NODAL(ProjMZ, ip[g], d), w * tau[2][d], par);
+                atomic_add(&NODAL(ProjMass, ip[g], d), w * gradRho[d], par);
+                atomic_add(&NODAL(ProjEner, ip[g], d), w * gradT[d], par);
+            }
+        }
+    }
+    or_lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, ProjMX);
+    or_lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, ProjMY);
+    or_lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, ProjMZ);
+    or_lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, ProjMass);
+    or_lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, ProjEner);
+}
+
+/* full_diff_stab_ijk (elem_stab.f90:207-438): the viscous fluxes of
+ * full_diffusion_ijk with the entropy-viscosity coefficients applied at each
+ * GP (nu_e grad rho, kappa_e grad T), plus the LPS fluxes
+ * -c_lps tau_e (Proj - grad) scaled by rho_n (momentum), rho_n Cp/Pr
+ * (energy) and 1 (mass); c_lps is c_lps_comp, a default-real constant. */
+void or_full_diff_stab_ijk(int p, int nelem, int npoin, const int *connec, const double *He,
+                           const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                           const int *gI, const int *gJ, const int *gK, double Cp, double Pr,
+                           const double *rho_n, const double *rho, const double *u,
+                           const double *Tem, const double *mu_fluid, const double *mu_e,
+                           const double *mu_sgs, const double *ProjMass, const double *ProjEner,
+                           const double *ProjMX, const double *ProjMY, const double *ProjMZ,
+                           const double *tau_stab, double c_rho, double c_ener, double c_lps,
+                           double *Rmass, double *Rmom, double *Rener, int initialize,
+                           double fact, int nthreads)
+{
+    const int n1 = p + 1, ngaus = n1 * n1 * n1, nnode = ngaus;
+    const double twoThirds = 2.0 / 3.0;
+    if (initialize) {
+        for (int n = 0; n < npoin; ++n) {
+            Rmass[n] = 0.0;
+            Rener[n] = 0.0;
+            for (int d = 0; d < 3; ++d) NODAL(Rmom, n, d) = 0.0;
+        }
+    }
+    const double aux_fact = fact;
+    if (nnode > 128) return;
+    const int par = nthreads > 1;
+#pragma omp parallel for schedule(static) num_threads(par ? nthreads : 1) if (par)
+    for (int e = 0; e < nelem; ++e) {
+        int ip[128];
+        double ul[128][3], rhol[128], Teml[128], mufluidl[128], rhonl[128];
+        double tauXl[128][3], tauYl[128][3], tauZl[128][3], gradTl[128][3], gradRhol[128][3];
+        for (int a = 0; a < nnode; ++a) {
+            ip[a] = CONNEC(e, a);
+            rhol[a] = rho[ip[a]];
+            rhonl[a] = rho_n[ip[a]];
+            Teml[a] = Tem[ip[a]];
+            mufluidl[a] = mu_fluid[ip[a]];
+            for (int d = 0; d < 3; ++d) ul[a][d] = NODAL(u, ip[a], d);
+        }
+        const double taustabl = tau_stab[e];
+        for (int g = 0; g < ngaus; ++g) {
+            double nu_e = c_rho * ELGP(mu_e, e, g) / rhonl[g];
+            double mu_fgp = mufluidl[g] + rhonl[g] * ELGP(mu_sgs, e, g);
+            double mu_egp = ELGP(mu_e, e, g);
+            double kappa_e = mufluidl[g] * Cp / Pr + c_ener * ELGP(mu_e, e, g) * Cp / Pr +
+                             Cp * rhonl[g] * ELGP(mu_sgs, e, g) / 0.9;
+            double gradRho[3], gradT[3], gradU[3][3], tau[3][3];
+            lps_gradients(p, nelem, e, g, He, dlxigp_ip, invAtoIJK, gI, gJ, gK, rhol, Teml,
+                          (const double(*)[3])ul, gradRho, gradT, gradU);
+            double divU = gradU[0][0] + gradU[1][1] + gradU[2][2];
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j) tau[i][j] = (gradU[i][j] + gradU[j][i]);
+                tau[i][i] = tau[i][i] - twoThirds * divU;
+            }
+            const int n = ip[g];
+            for (int d = 0; d < 3; ++d) {
+                tauXl[g][d] = -c_lps * taustabl * (NODAL(ProjMX, n, d) - tau[0][d]) * rhonl[g];
+                tauYl[g][d] = -c_lps * taustabl * (NODAL(ProjMY, n, d) - tau[1][d]) * rhonl[g];
+                tauZl[g][d] = -c_lps * taustabl * (NODAL(ProjMZ, n, d) - tau[2][d]) * rhonl[g];
+                gradTl[g][d] =
+                    -c_lps * taustabl * (NODAL(ProjEner, n, d) - gradT[d]) * rhonl[g] * Cp / Pr;
+                gradRhol[g][d] = -c_lps * taustabl * (NODAL(ProjMass, n, d) - gradRho[d]);
+            }
+            double tauU[3] = {0, 0, 0};
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j) {
+                    tauU[i] = tauU[i] + (mu_fgp + mu_egp) * (gradU[i][j] + gradU[j][i]) * ul[g][j];
+                    tau[i][j] = (mu_fgp + mu_egp) * (gradU[i][j] + gradU[j][i]);
+                }
+                tauU[i] = tauU[i] - (mu_fgp)*twoThirds * divU * ul[g][i];
+                tau[i][i] = tau[i][i] - (mu_fgp)*twoThirds * divU;
+            }
+            for (int d = 0; d < 3; ++d) {
+                tauXl[g][d] = tauXl[g][d] + tau[0][d];
+                tauYl[g][d] = tauYl[g][d] + tau[1][d];
+                tauZl[g][d] = tauZl[g][d] + tau[2][d];
+                gradTl[g][d] = gradTl[g][d] + kappa_e * gradT[d] + tauU[d];
+                gradRhol[g][d] = gradRhol[g][d] + nu_e * gradRho[d];
+            }
+        }
+        for (int g = 0; g < ngaus; ++g) {
+            int I = gI[g] - 1, J = gJ[g] - 1, K = gK[g] - 1;
+            double divDe = 0.0, divDr = 0.0, divDm[3] = {0, 0, 0};
+            for (int ii = 0; ii < n1; ++ii) {
+                int b1 = INV(ii, J, K), b2 = INV(I, ii, K), b3 = INV(I, J, ii);
+                for (int d = 0; d < 3; ++d) {
+                    double w1 = HE(d, 0, b1, e) * GPVOL(b1, e) * DL(b1, 0, I);
+                    double w2 = HE(d, 1, b2, e) * GPVOL(b2, e) * DL(b2, 1, J);
+                    double w3 = HE(d, 2, b3, e) * GPVOL(b3, e) * DL(b3, 2, K);
+                    divDr = divDr + w1 * gradRhol[b1][d];
+                    divDr = divDr + w2 * gradRhol[b2][d];
+                    divDr = divDr + w3 * gradRhol[b3][d];
+                    divDe = divDe + w1 * gradTl[b1][d];
+                    divDe = divDe + w2 * gradTl[b2][d];
+                    divDe = divDe + w3 * gradTl[b3][d];
+                    divDm[0] = divDm[0] + w1 * tauXl[b1][d];
+                    divDm[0] = divDm[0] + w2 * tauXl[b2][d];
+                    divDm[0] = divDm[0] + w3 * tauXl[b3][d];
+                    divDm[1] = divDm[1] + w1 * tauYl[b1][d];
+                    divDm[1] = divDm[1] + w2 * tauYl[b2][d];
+                    divDm[1] = divDm[1] + w3 * tauYl[b3][d];
+                    divDm[2] = divDm[2] + w1 * tauZl[b1][d];
+                    divDm[2] = divDm[2] + w2 * tauZl[b2][d];
+                    divDm[2] = divDm[2] + w3 * tauZl[b3][d];
+                }
+            }
+            atomic_add(&Rmass[ip[g]], aux_fact * divDr, par);
+            atomic_add(&Rener[ip[g]], aux_fact * divDe, par);
+            for (int d = 0; d < 3; ++d) atomic_add(&NODAL(Rmom, ip[g], d), aux_fact * divDm[d], par);
+        }
+    }
+}
+
 /* generic_scalar_convec_ijk (elem_convec.f90:472-594) */
 void or_generic_scalar_convec_ijk(int p, int nelem, int npoin, const int *connec, const double *He,
                                   const double *gpvol, const double *dlxigp_ip,

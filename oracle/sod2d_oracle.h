@@ -65,6 +65,25 @@ void or_full_diffusion_ijk(int p, int nelem, int npoin, const int *connec, const
                            const double *Tem, const double *mu_fluid, const double *mu_e,
                            const double *mu_sgs, double c_rho, double c_ener, double *Rmass,
                            double *Rmom, double *Rener, int initialize, double fact, int nthreads);
+void or_comp_tau(int p, int nelem, int npoin, const int *connec, const double *csound,
+                 const double *u, const double *helem_k, double c_species_stab, int drop_c,
+                 double *tau);
+void or_full_proj_ijk(int p, int nelem, int npoin, int npoin_w, const int *connec,
+                      const int *lpoin_w, const double *He, const double *gpvol,
+                      const double *dlxigp_ip, const int *invAtoIJK, const int *gI, const int *gJ,
+                      const int *gK, const double *rho, const double *u, const double *Tem,
+                      const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
+                      double *ProjMY, double *ProjMZ, int nthreads);
+void or_full_diff_stab_ijk(int p, int nelem, int npoin, const int *connec, const double *He,
+                           const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                           const int *gI, const int *gJ, const int *gK, double Cp, double Pr,
+                           const double *rho_n, const double *rho, const double *u,
+                           const double *Tem, const double *mu_fluid, const double *mu_e,
+                           const double *mu_sgs, const double *ProjMass, const double *ProjEner,
+                           const double *ProjMX, const double *ProjMY, const double *ProjMZ,
+                           const double *tau_stab, double c_rho, double c_ener, double c_lps,
+                           double *Rmass, double *Rmom, double *Rener, int initialize,
+                           double fact, int nthreads);
 void or_generic_scalar_convec_ijk(int p, int nelem, int npoin, const int *connec, const double *He,
                                   const double *gpvol, const double *dlxigp_ip,
                                   const int *invAtoIJK, const int *gI, const int *gJ,

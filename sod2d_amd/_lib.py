@@ -62,6 +62,10 @@ _SIGS = {
                                       + [P] * 3 + [I, D]),
     "sodhip_full_diffusion_ijk_f32": (I, [P, I, I, I] + [P] * 8 + [F, F] + [P] * 7 + [F, F]
                                       + [P] * 3 + [I, F]),
+    "sodhip_comp_tau_f64": (I, [P, I, I, I, P, P, P, P, D, I, P]),
+    "sodhip_full_proj_ijk_f64": (I, [P, I, I, I, I] + [P] * 18),
+    "sodhip_full_diff_stab_ijk_f64": (I, [P, I, I, I] + [P] * 8 + [D, D] + [P] * 13
+                                      + [D, D, D] + [P] * 3 + [I, D]),
     "sodhip_generic_scalar_convec_ijk_f64": (I, [P, I, I, I] + [P] * 11 + [P, I, D]),
     "sodhip_smart_visc_spectral_f64": (I, [P, I, I, I, I, P, P, P, D] + [P] * 7
                                        + [C.POINTER(ViscParams), P, P]),

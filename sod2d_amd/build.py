@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("SODHIP_ARCH", "gfx950")
 
-SOURCES = ["runtime.hip", "ops_element.hip", "ops_node.hip", "ops_sensor.hip", "ops_stage.hip",
+SOURCES = ["runtime.hip", "ops_element.hip", "ops_lps.hip", "ops_node.hip", "ops_sensor.hip", "ops_stage.hip",
            "setup.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]

@@ -1,0 +1,443 @@
+// ops_lps.hip -- local-projection stabilisation of the viscous operator
+// (elem_stab.f90): the element time scale comp_tau, the lumped L2 projections
+// of the gradients full_proj_ijk, and the stabilised viscous operator
+// full_diff_stab_ijk, with their drop-in C-ABI entry points (FP64).
+//
+// Mapping as the other element kernels: one element per workgroup of
+// NT = roundup((p+1)^3, 64) lanes, lane a owns element-local node /
+// quadrature point a, nodal fields staged field-major in LDS, line sums from
+// the per-lane GLL line tables, element records gathered per node in the
+// reference's (element, node) order by a second kernel (deterministic, no
+// atomics).
+#include "common.hpp"
+#include "device.hpp"
+#include "element.hpp"
+
+namespace sodhip {
+
+struct LpsArgs {
+    int nelem, npoin;
+    const int *connecT; // element-major, 0-based
+    const int *perm;    // record position of each element-local node
+    const double *He, *gpvol, *dl;
+    const int *inv, *gI, *gJ, *gK;
+    const double *rho_n, *rho, *u, *Tem, *mu_fluid, *mu_e, *mu_sgs;
+    const double *pMass, *pEner, *pMX, *pMY, *pMZ; // projections (npoin,3)
+    const double *tau;                             // tau_stab(nelem)
+    double Cp, Pr, c_rho, c_ener, c_lps;
+    double *out;
+};
+
+// Physical gradients of rho, T and u at the lane's point from the staged
+// element fields s (rho, T, u1, u2, u3, each NT long):
+//   gR[i] = sum_j He(i,j) dR/dxi_j,  gU[i][j] = sum_k He(j,k) du_i/dxi_k
+// (elem_stab.f90:300-335, :497-531).
+template <int P>
+__device__ __forceinline__ void lps_gradients(const LineTab<P, double> &lt, const double *s,
+                                              int a, const double *hp, double gR[3],
+                                              double gT[3], double gU[3][3])
+{
+    constexpr int NT = EG<P>::NT;
+    double g[5][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double c0 = lt.ds[k];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) g[f][k] = c0 * s[f * NT + a];
+#pragma unroll
+        for (int o = 0; o < P; ++o) {
+            const int b = lt.off[k][o];
+            const double c = lt.d[k][o];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) g[f][k] += c * s[f * NT + b];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        gR[i] = hp[i] * g[0][0] + hp[i + 3] * g[0][1] + hp[i + 6] * g[0][2];
+        gT[i] = hp[i] * g[1][0] + hp[i + 3] * g[1][1] + hp[i + 6] * g[1][2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            gU[i][j] = hp[j] * g[2 + i][0] + hp[j + 3] * g[2 + i][1] + hp[j + 6] * g[2 + i][2];
+}
+
+// full_proj_ijk's element part (elem_stab.f90:489-563): records of 15 values
+// per element-node, gpvol * (rows 1..3 of 2 S - 2/3 div u I, grad rho, grad T).
+template <int P>
+__global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
+{
+    constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
+    __shared__ double s[5 * NT];
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    LineTab<P, double> lt;
+    build_linetab<P, double>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1);
+    const ElemRange er = elem_range(A.nelem);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? A.connecT[ea] : 0;
+        if (act) {
+            s[a] = A.rho[n];
+            s[NT + a] = A.Tem[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + a] = A.u[n + (size_t)A.npoin * d];
+        }
+        double hp[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) hp[i] = A.He[9 * ea + i];
+        const double w = A.gpvol[ea];
+        lds_barrier();
+        double gR[3], gT[3], gU[3][3];
+        lps_gradients<P>(lt, s, a, hp, gR, gT, gU);
+        const double divU = gU[0][0] + gU[1][1] + gU[2][2];
+        if (act) {
+            double *o = A.out + 15 * ((size_t)e * NN + pa);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double t = gU[i][d] + gU[d][i];
+                    if (i == d) t = t - (2.0 / 3.0) * divU;
+                    o[3 * i + d] = w * t;
+                }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                o[9 + d] = w * gR[d];
+                o[12 + d] = w * gT[d];
+            }
+        }
+        lds_barrier();
+    }
+}
+
+// full_diff_stab_ijk (elem_stab.f90:207-438): per quadrature point the viscous
+// fluxes with the LPS terms -c_lps tau_e (Proj - grad) (scaled by rho_n for
+// momentum, rho_n Cp/Pr for energy), mapped to contravariant components
+// G_k(F) = gpvol sum_d He(d,k) F_d and staged in LDS; then the weak divergence
+// sum_k sum_{b on the k-line} dl(b,k,I_k(a)) G_k(F)(b).  Records of 5 values
+// (mass, momentum 1..3, energy) per element-node.
+template <int P>
+__global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
+{
+    constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
+    __shared__ double s[5 * NT];
+    __shared__ double G[15 * NT]; // flux f (mass, mom x, y, z, energy), direction k
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const int a = act ? lane : 0;
+    LineTab<P, double> lt;
+    build_linetab<P, double>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
+    const int pa = min(max(A.perm[a], 0), NN - 1);
+    const double twoThirds = 2.0 / 3.0;
+    const ElemRange er = elem_range(A.nelem);
+    for (int e = er.begin; e < er.end; e += er.stride) {
+        const size_t ea = (size_t)e * NN + a;
+        const int n = act ? A.connecT[ea] : 0;
+        const size_t np = (size_t)A.npoin;
+        double ul[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ul[d] = A.u[n + np * d];
+        if (act) {
+            s[a] = A.rho[n];
+            s[NT + a] = A.Tem[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + a] = ul[d];
+        }
+        double hp[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) hp[i] = A.He[9 * ea + i];
+        const double w = A.gpvol[ea];
+        const double rhon = A.rho_n[n], mufl = A.mu_fluid[n];
+        const size_t eg = (size_t)e + (size_t)A.nelem * a; // (nelem,ngaus) element-fastest
+        const double mue = A.mu_e[eg];
+        const double musgs = A.mu_sgs ? A.mu_sgs[eg] : 0.0;
+        const double taus = A.tau[e];
+        lds_barrier();
+        double gR[3], gT[3], gU[3][3];
+        lps_gradients<P>(lt, s, a, hp, gR, gT, gU);
+        const double divU = gU[0][0] + gU[1][1] + gU[2][2];
+        const double nu_e = A.c_rho * mue / rhon;
+        const double mu_fgp = mufl + rhon * musgs;
+        const double kappa_e =
+            mufl * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr + A.Cp * rhon * musgs / 0.9;
+        const double cl = -A.c_lps * taus;
+        double F[5][3];
+        const double *pm[3] = {A.pMX, A.pMY, A.pMZ};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double t = gU[i][d] + gU[d][i];
+                if (i == d) t = t - twoThirds * divU;
+                F[1 + i][d] = cl * (pm[i][n + np * d] - t) * rhon;
+            }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            F[4][d] = cl * (A.pEner[n + np * d] - gT[d]) * rhon * A.Cp / A.Pr;
+            F[0][d] = cl * (A.pMass[n + np * d] - gR[d]);
+        }
+        double tauU[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                tauU[i] = tauU[i] + (mu_fgp + mue) * (gU[i][j] + gU[j][i]) * ul[j];
+                double t = (mu_fgp + mue) * (gU[i][j] + gU[j][i]);
+                if (i == j) t = t - mu_fgp * twoThirds * divU;
+                F[1 + i][j] = F[1 + i][j] + t;
+            }
+            tauU[i] = tauU[i] - mu_fgp * twoThirds * divU * ul[i];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            F[4][d] = F[4][d] + kappa_e * gT[d] + tauU[d];
+            F[0][d] = F[0][d] + nu_e * gR[d];
+        }
+        if (act) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    G[(3 * f + k) * NT + a] =
+                        w * (hp[3 * k] * F[f][0] + hp[1 + 3 * k] * F[f][1] +
+                             hp[2 + 3 * k] * F[f][2]);
+        }
+        lds_barrier();
+        double R[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double *gk = G + (3 * f + k) * NT;
+                acc += lt.ds[k] * gk[a];
+#pragma unroll
+                for (int o = 0; o < P; ++o) acc += lt.t[k][o] * gk[lt.off[k][o]];
+            }
+            R[f] = acc;
+        }
+        if (act) {
+            double *o = A.out + 5 * ((size_t)e * NN + pa);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) o[f] = R[f];
+        }
+        lds_barrier();
+    }
+}
+
+// comp_tau (elem_stab.f90:589-616): one thread per element over the
+// reference's element-fastest connec.
+__global__ __launch_bounds__(256) void k_comp_tau(int nelem, int nnode, int porder,
+                                                  const int *__restrict__ connec,
+                                                  const double *__restrict__ csound,
+                                                  const double *__restrict__ u, int npoin,
+                                                  const double *__restrict__ helem_k,
+                                                  double c_species_stab, double fact_c,
+                                                  double *__restrict__ tau)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    const double h = helem_k[e] * (c_species_stab / (double)porder);
+    double taul = 0.0;
+    for (int a = 0; a < nnode; ++a) {
+        const int n = connec[e + (size_t)nelem * a] - 1;
+        const double u0 = u[n], u1 = u[n + (size_t)npoin], u2 = u[n + 2 * (size_t)npoin];
+        const double aux1 = fact_c * csound[n] + sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        taul = fmax(taul, h * aux1);
+    }
+    tau[e] = taul;
+}
+
+// Node-ordered assembly of NF-value records into NF nodal arrays:
+// R_f(n) = sum_j fact * scr[src[j]*NF + f] over the node's incidence slots in
+// (element, node) order.
+template <int NF> struct NodalOut {
+    double *R[NF];
+};
+template <int NF>
+__global__ __launch_bounds__(256) void k_gather_nf(int npoin, const int *__restrict__ row,
+                                                   const int *__restrict__ src,
+                                                   const double *__restrict__ scr,
+                                                   NodalOut<NF> out, int init, double fact)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npoin) return;
+    const int j0 = row[n], j1 = row[n + 1];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        double acc = init ? 0.0 : out.R[f][n];
+        for (int j = j0; j < j1; ++j) acc += fact * scr[(size_t)src[j] * NF + f];
+        out.R[f][n] = acc;
+    }
+}
+
+static int lps_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
+                      const double *He, const double *gpvol, const double *dl, const int *inv,
+                      const int *gI, const int *gJ, const int *gK, MeshCache **m)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && connec, "invalid mesh arguments");
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK, "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    return get_mesh(ctx, porder, nelem, npoin, connec, m);
+}
+
+#define SODHIP_LPS_LAUNCH(KERNEL, GRID, ARGS)                                              \
+    switch (porder) {                                                                      \
+    case 2: KERNEL<2><<<GRID, EG<2>::NT, 0, ctx->stream>>>(ARGS); break;                   \
+    case 3: KERNEL<3><<<GRID, EG<3>::NT, 0, ctx->stream>>>(ARGS); break;                   \
+    case 4: KERNEL<4><<<GRID, EG<4>::NT, 0, ctx->stream>>>(ARGS); break;                   \
+    default: return set_error(SODHIP_ERR_UNSUPPORTED, "porder %d not supported", porder); \
+    }
+
+} // namespace sodhip
+
+using namespace sodhip;
+
+extern "C" int sodhip_comp_tau_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                   const int *connec, const double *csound, const double *u,
+                                   const double *helem_k, double c_species_stab,
+                                   int flag_drop_c_in_envit, double *tau)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_REQUIRE(porder >= 1 && porder <= kMaxP, "porder %d not supported", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0, "invalid mesh arguments");
+    SODHIP_REQUIRE(connec && csound && u && helem_k && tau, "null array argument");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    const int nn = nnode_of(porder);
+    k_comp_tau<<<(nelem + 255) / 256, 256, 0, ctx->stream>>>(
+        nelem, nn, porder, connec, csound, u, npoin, helem_k, c_species_stab,
+        flag_drop_c_in_envit ? 0.0 : 1.0, tau);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_full_proj_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                        int npoin_w, const int *connec, const int *lpoin_w,
+                                        const double *He, const double *gpvol,
+                                        const double *dlxigp_ip, const int *invAtoIJK,
+                                        const int *gmshAtoI, const int *gmshAtoJ,
+                                        const int *gmshAtoK, const double *rho, const double *u,
+                                        const double *Tem, const double *invMl,
+                                        double *ProjMass, double *ProjEner, double *ProjMX,
+                                        double *ProjMY, double *ProjMZ)
+{
+    MeshCache *m = nullptr;
+    int rc = lps_common(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                        gmshAtoI, gmshAtoJ, gmshAtoK, &m);
+    if (rc) return rc;
+    SODHIP_REQUIRE(rho && u && Tem && invMl && lpoin_w && ProjMass && ProjEner && ProjMX &&
+                       ProjMY && ProjMZ,
+                   "null array argument");
+    SODHIP_REQUIRE(npoin_w > 0 && npoin_w <= npoin, "invalid npoin_w");
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 15 * EN * sizeof(double)))) return rc;
+    LpsArgs A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dlxigp_ip;
+    A.inv = invAtoIJK;
+    A.gI = gmshAtoI;
+    A.gJ = gmshAtoJ;
+    A.gK = gmshAtoK;
+    A.rho = rho;
+    A.u = u;
+    A.Tem = Tem;
+    A.out = (double *)ctx->scratch.ptr;
+    const int grid = elem_grid(ctx, nelem, 16);
+    int tok = prof_begin(ctx, 3);
+    SODHIP_LPS_LAUNCH(k_lps_proj, grid, A)
+    SODHIP_TRY_HIP(hipGetLastError());
+    NodalOut<15> o;
+    double *P[5] = {ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner};
+    for (int q = 0; q < 5; ++q)
+        for (int d = 0; d < 3; ++d) o.R[3 * q + d] = P[q] + (size_t)npoin * d;
+    k_gather_nf<15><<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(npoin, m->row, m->src, A.out, o,
+                                                                  1, 1.0);
+    SODHIP_TRY_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+    // lumped solve of each projection over the working nodes
+    for (int q = 0; q < 5; ++q)
+        if ((rc = sodhip_lumped_solver_vect_f64(ctx, npoin, npoin_w, lpoin_w, invMl, P[q])))
+            return rc;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_full_diff_stab_ijk_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, const double *He,
+    const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+    const int *gmshAtoJ, const int *gmshAtoK, double Cp, double Pr, const double *rho_n,
+    const double *rho, const double *u, const double *Tem, const double *mu_fluid,
+    const double *mu_e, const double *mu_sgs, const double *ProjMass, const double *ProjEner,
+    const double *ProjMX, const double *ProjMY, const double *ProjMZ, const double *tau_stab,
+    double c_rho, double c_ener, double c_lps, double *Rmass, double *Rmom, double *Rener,
+    int initialize, double fact)
+{
+    MeshCache *m = nullptr;
+    int rc = lps_common(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                        gmshAtoI, gmshAtoJ, gmshAtoK, &m);
+    if (rc) return rc;
+    SODHIP_REQUIRE(rho_n && rho && u && Tem && mu_fluid && mu_e && ProjMass && ProjEner &&
+                       ProjMX && ProjMY && ProjMZ && tau_stab && Rmass && Rmom && Rener,
+                   "null array argument");
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(double)))) return rc;
+    LpsArgs A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dlxigp_ip;
+    A.inv = invAtoIJK;
+    A.gI = gmshAtoI;
+    A.gJ = gmshAtoJ;
+    A.gK = gmshAtoK;
+    A.rho_n = rho_n;
+    A.rho = rho;
+    A.u = u;
+    A.Tem = Tem;
+    A.mu_fluid = mu_fluid;
+    A.mu_e = mu_e;
+    A.mu_sgs = mu_sgs;
+    A.pMass = ProjMass;
+    A.pEner = ProjEner;
+    A.pMX = ProjMX;
+    A.pMY = ProjMY;
+    A.pMZ = ProjMZ;
+    A.tau = tau_stab;
+    A.Cp = Cp;
+    A.Pr = Pr;
+    A.c_rho = c_rho;
+    A.c_ener = c_ener;
+    A.c_lps = c_lps;
+    A.out = (double *)ctx->scratch.ptr;
+    const int grid = elem_grid(ctx, nelem, 16);
+    int tok = prof_begin(ctx, 0);
+    SODHIP_LPS_LAUNCH(k_diff_stab, grid, A)
+    SODHIP_TRY_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+    NodalOut<5> o;
+    o.R[0] = Rmass;
+    o.R[1] = Rmom;
+    o.R[2] = Rmom + npoin;
+    o.R[3] = Rmom + 2 * (size_t)npoin;
+    o.R[4] = Rener;
+    tok = prof_begin(ctx, 1);
+    k_gather_nf<5><<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(npoin, m->row, m->src, A.out, o,
+                                                                 initialize, fact);
+    SODHIP_TRY_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+    return SODHIP_OK;
+}

@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from ._lib import SodhipError, ViscParams, check, lib
@@ -98,7 +99,7 @@ def _porder(nelem, connec):
     return p
 
 
-_saved_fact = {"convec": 1.0, "diffu": 1.0, "scalar": 1.0}
+_saved_fact = {"convec": 1.0, "diffu": 1.0, "scalar": 1.0, "diffstab": 1.0}
 
 
 def _resolve(name, initialze, fact):
@@ -164,6 +165,75 @@ def full_diffusion_ijk(nelem, npoin, connec, Ngp, He, gpvol, dlxigp_ip, invAtoIJ
             _ptr(Rmass, dt, npoin, "Rmass"), _ptr(Rmom, dt, 3 * npoin, "Rmom"),
             _ptr(Rener, dt, npoin, "Rener"), init, fc(f))
     check(rc, "full_diffusion_ijk")
+
+
+C_LPS_COMP = float(np.float32(0.1))  # c_lps_comp: a default-real constant
+C_SPECIES_STAB = 0.5
+
+
+def comp_tau(nelem, npoin, connec, csound, u, helem_k, dt, tau, c_species_stab=C_SPECIES_STAB,
+             flag_drop_c_in_envit=0, ctx: Context | None = None):
+    """comp_tau (elem_stab.f90:589-616) on the device (dt is unused there too)."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    F64, I32 = torch.float64, torch.int32
+    check(lib().sodhip_comp_tau_f64(ctx.h, p, nelem, npoin, _ptr(connec, I32, nelem * nn, "connec"),
+                                    _ptr(csound, F64, npoin, "csound"), _ptr(u, F64, 3 * npoin, "u"),
+                                    _ptr(helem_k, F64, nelem, "helem_k"), float(c_species_stab),
+                                    int(bool(flag_drop_c_in_envit)), _ptr(tau, F64, nelem, "tau")),
+          "comp_tau")
+
+
+def full_proj_ijk(nelem, npoin, npoin_w, connec, lpoin_w, Ngp, He, gpvol, dlxigp_ip, invAtoIJK,
+                  gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, Ml, invMl, ProjMass, ProjEner,
+                  ProjMX, ProjMY, ProjMZ, ctx: Context | None = None):
+    """full_proj_ijk (elem_stab.f90:441-586) on the device."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    nn = (p + 1) ** 3
+    F64, I32 = torch.float64, torch.int32
+    out = [_ptr(x, F64, 3 * npoin, nm) for x, nm in
+           ((ProjMass, "ProjMass"), (ProjEner, "ProjEner"), (ProjMX, "ProjMX"), (ProjMY, "ProjMY"),
+            (ProjMZ, "ProjMZ"))]
+    check(lib().sodhip_full_proj_ijk_f64(
+        ctx.h, p, nelem, npoin, npoin_w, _ptr(connec, I32, nelem * nn, "connec"),
+        _ptr(lpoin_w, I32, npoin_w, "lpoin_w"), _ptr(He, F64, 9 * nn * nelem, "He"),
+        _ptr(gpvol, F64, nn * nelem, "gpvol"), _ptr(dlxigp_ip, F64, 3 * nn * (p + 1), "dlxigp_ip"),
+        _ptr(invAtoIJK, I32, nn), _ptr(gmshAtoI, I32, nn), _ptr(gmshAtoJ, I32, nn),
+        _ptr(gmshAtoK, I32, nn), _ptr(rho, F64, npoin, "rho"), _ptr(u, F64, 3 * npoin, "u"),
+        _ptr(Tem, F64, npoin, "Tem"), _ptr(invMl, F64, npoin, "invMl"), *out), "full_proj_ijk")
+
+
+def full_diff_stab_ijk(nelem, npoin, connec, Ngp, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI,
+                       gmshAtoJ, gmshAtoK, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs, Ml,
+                       ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ, tau_stab, Rmass, Rmom, Rener,
+                       initialze=None, fact=None, c_rho=1.0, c_ener=1.0, c_lps=C_LPS_COMP,
+                       ctx: Context | None = None):
+    """full_diff_stab_ijk (elem_stab.f90:207-438) on the device."""
+    ctx = ctx or default_context()
+    ctx.sync_stream()
+    p = _porder(nelem, connec)
+    init, f = _resolve("diffstab", initialze, fact)
+    nn = (p + 1) ** 3
+    F64, I32 = torch.float64, torch.int32
+    proj = [_ptr(x, F64, 3 * npoin, nm) for x, nm in
+            ((ProjMass, "ProjMass"), (ProjEner, "ProjEner"), (ProjMX, "ProjMX"),
+             (ProjMY, "ProjMY"), (ProjMZ, "ProjMZ"))]
+    check(lib().sodhip_full_diff_stab_ijk_f64(
+        ctx.h, p, nelem, npoin, _ptr(connec, I32, nelem * nn, "connec"),
+        _ptr(He, F64, 9 * nn * nelem, "He"), _ptr(gpvol, F64, nn * nelem, "gpvol"),
+        _ptr(dlxigp_ip, F64, 3 * nn * (p + 1), "dlxigp_ip"), _ptr(invAtoIJK, I32, nn),
+        _ptr(gmshAtoI, I32, nn), _ptr(gmshAtoJ, I32, nn), _ptr(gmshAtoK, I32, nn), float(Cp),
+        float(Pr), _ptr(rho_n, F64, npoin, "rho_n"), _ptr(rho, F64, npoin, "rho"),
+        _ptr(u, F64, 3 * npoin, "u"), _ptr(Tem, F64, npoin, "Tem"),
+        _ptr(mu_fluid, F64, npoin, "mu_fluid"), _ptr(mu_e, F64, nn * nelem, "mu_e"),
+        _ptr(mu_sgs, F64, nn * nelem, "mu_sgs"), *proj, _ptr(tau_stab, F64, nelem, "tau_stab"),
+        float(c_rho), float(c_ener), float(c_lps), _ptr(Rmass, F64, npoin, "Rmass"),
+        _ptr(Rmom, F64, 3 * npoin, "Rmom"), _ptr(Rener, F64, npoin, "Rener"), init, float(f)),
+        "full_diff_stab_ijk")
 
 
 def generic_scalar_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp,

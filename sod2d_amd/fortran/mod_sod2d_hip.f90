@@ -37,6 +37,7 @@ module mod_sod2d_hip
    public :: rk_stage_update
    public :: smart_visc_spectral_imex, eval_divergence, compute_vorticity, ducros_visc_limit
    public :: sod2d_hip_set_live_params
+   public :: comp_tau, full_proj_ijk, full_diff_stab_ijk
 
    type, bind(C) :: visc_params_t
       real(c_double) :: ce, cmax, cglob, Rgas
@@ -57,6 +58,10 @@ module mod_sod2d_hip
    ! normalisation sets the module's ce to ce_comp for every later call
    real(rp), save :: ce_comp = 1.0_rp, ducros_min_val = 0.1_rp
    logical, save :: flag_high_mach = .false.
+   ! LPS constants (mod_numerical_params.f90:31,46): c_lps_comp is default real
+   real, save :: c_lps_comp = 0.1
+   real(rp), save :: c_species_stab = 0.5_rp
+   real(rp), save :: fact_diffstab = 1.0_rp
 
    interface sod2d_hip_enter_data
       module procedure enter_r8, enter_i4
@@ -133,6 +138,32 @@ module mod_sod2d_hip
          type(c_ptr), value :: mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener
          integer(c_int), value :: p, nelem, npoin, init
          real(c_double), value :: Cp, Pr, c_rho, c_ener, fact
+      end function
+      integer(c_int) function c_comp_tau(c, p, nelem, npoin, connec, csound, u, helem_k, &
+            c_species, drop_c, tau) bind(C, name='sodhip_comp_tau_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, csound, u, helem_k, tau
+         integer(c_int), value :: p, nelem, npoin, drop_c
+         real(c_double), value :: c_species
+      end function
+      integer(c_int) function c_full_proj(c, p, nelem, npoin, npoin_w, connec, lpoin_w, He, &
+            gpvol, dl, inv, gI, gJ, gK, rho, u, Tem, invMl, PMass, PEner, PMX, PMY, PMZ) &
+            bind(C, name='sodhip_full_proj_ijk_f64')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, rho, u, Tem
+         type(c_ptr), value :: invMl, PMass, PEner, PMX, PMY, PMZ
+         integer(c_int), value :: p, nelem, npoin, npoin_w
+      end function
+      integer(c_int) function c_diff_stab(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
+            gJ, gK, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs, PMass, PEner, PMX, PMY, &
+            PMZ, tau, c_rho, c_ener, c_lps, Rmass, Rmom, Rener, init, fact) &
+            bind(C, name='sodhip_full_diff_stab_ijk_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, rho_n, rho, u, Tem
+         type(c_ptr), value :: mu_fluid, mu_e, mu_sgs, PMass, PEner, PMX, PMY, PMZ, tau
+         type(c_ptr), value :: Rmass, Rmom, Rener
+         integer(c_int), value :: p, nelem, npoin, init
+         real(c_double), value :: Cp, Pr, c_rho, c_ener, c_lps, fact
       end function
       integer(c_int) function c_scalar_convec(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
             gJ, gK, q, eta, u, Rconvec, init, fact) &
@@ -417,6 +448,75 @@ contains
          dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener'), init_flag(initialze), &
          fact_diffu), 'full_diffusion_ijk')
    end subroutine full_diffusion_ijk
+
+   ! elem_stab.f90:589-616.  dt is part of the reference's interface and is not
+   ! read; c_species_stab and flag_drop_c_in_envit are module globals.
+   subroutine comp_tau(nelem,npoin,connec,csound,u,helem_k,dt,tau)
+      integer(4), intent(in)   :: nelem, npoin,connec(nelem,nnode)
+      real(rp),    intent(in)  :: u(npoin,ndime),helem_k(nelem),dt,csound(npoin)
+      real(rp),    intent(out) :: tau(nelem)
+      call check(c_comp_tau(ctx, porder, nelem, npoin, dev(connec,'connec'), &
+         dev(csound,'csound'), dev(u,'u'), dev(helem_k,'helem_k'), c_species_stab, &
+         vprm%flag_drop_c_in_envit, dev(tau,'tau')), 'comp_tau')
+   end subroutine comp_tau
+
+   ! elem_stab.f90:441-586.  Ngp, Cp, Pr and Ml are part of the reference's
+   ! interface and are not read.
+   subroutine full_proj_ijk(nelem,npoin,npoin_w,connec,lpoin_w,Ngp,He,gpvol,dlxigp_ip,invAtoIJK, &
+         gmshAtoI,gmshAtoJ,gmshAtoK,Cp,Pr,rho,u,Tem,Ml,invMl,ProjMass,ProjEner,ProjMX,ProjMY, &
+         ProjMZ)
+      integer(4), intent(in)  :: nelem, npoin,npoin_w
+      integer(4), intent(in)  :: connec(nelem,nnode),lpoin_w(npoin_w)
+      real(rp),   intent(in)  :: Ngp(ngaus,nnode)
+      real(rp),   intent(in)  :: He(ndime,ndime,ngaus,nelem),dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)  :: Cp,Pr,rho(npoin),u(npoin,ndime),Tem(npoin),Ml(npoin)
+      real(rp),   intent(in)  :: invMl(npoin)
+      real(rp),   intent(inout) :: ProjMass(npoin,ndime),ProjEner(npoin,ndime)
+      real(rp),   intent(inout) :: ProjMX(npoin,ndime),ProjMY(npoin,ndime),ProjMZ(npoin,ndime)
+      call check(c_full_proj(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+         dev(lpoin_w,'lpoin_w'), dev(He,'He'), dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), &
+         dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), &
+         dev(gmshAtoK,'gmshAtoK'), dev(rho,'rho'), dev(u,'u'), dev(Tem,'Tem'), &
+         dev(invMl,'invMl'), dev(ProjMass,'ProjMass'), dev(ProjEner,'ProjEner'), &
+         dev(ProjMX,'ProjMX'), dev(ProjMY,'ProjMY'), dev(ProjMZ,'ProjMZ')), 'full_proj_ijk')
+   end subroutine full_proj_ijk
+
+   ! elem_stab.f90:207-438.  Ngp and Ml are part of the reference's interface
+   ! and are not read; c_rho and c_ener take the reference's defaults (1).
+   subroutine full_diff_stab_ijk(nelem,npoin,connec,Ngp,He,gpvol,dlxigp_ip,invAtoIJK,gmshAtoI, &
+         gmshAtoJ,gmshAtoK,Cp,Pr,rho_n,rho,u,Tem,mu_fluid,mu_e,mu_sgs,Ml,ProjMass,ProjEner, &
+         ProjMX,ProjMY,ProjMZ,tau_stab,Rmass,Rmom,Rener,initialze,fact)
+      integer(4), intent(in)  :: nelem, npoin
+      integer(4), intent(in)  :: connec(nelem,nnode)
+      real(rp),   intent(in)  :: Ngp(ngaus,nnode)
+      real(rp),   intent(in)  :: He(ndime,ndime,ngaus,nelem),dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp),   intent(in)  :: gpvol(1,ngaus,nelem)
+      integer(4), intent(in)  :: invAtoIJK(porder+1,porder+1,porder+1),gmshAtoI(nnode)
+      integer(4), intent(in)  :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)  :: Cp,Pr,rho_n(npoin),rho(npoin),u(npoin,ndime), Tem(npoin)
+      real(rp),   intent(in)  :: Ml(npoin), mu_e(nelem,ngaus), mu_sgs(nelem,ngaus)
+      real(rp),   intent(in)  :: tau_stab(nelem),ProjMass(npoin,ndime),ProjEner(npoin,ndime)
+      real(rp),   intent(in)  :: ProjMX(npoin,ndime),ProjMY(npoin,ndime),ProjMZ(npoin,ndime)
+      real(rp),   intent(in)  :: mu_fluid(npoin)
+      real(rp),   intent(inout) :: Rmass(npoin)
+      real(rp),   intent(inout) :: Rmom(npoin,ndime)
+      real(rp),   intent(inout) :: Rener(npoin)
+      logical, optional, intent(in)    :: initialze
+      real(rp), optional, intent(in)  :: fact
+      if (present(fact)) fact_diffstab = fact
+      call check(c_diff_stab(ctx, porder, nelem, npoin, dev(connec,'connec'), dev(He,'He'), &
+         dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), &
+         dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), Cp, Pr, &
+         dev(rho_n,'rho_n'), dev(rho,'rho'), dev(u,'u'), dev(Tem,'Tem'), &
+         dev(mu_fluid,'mu_fluid'), dev(mu_e,'mu_e'), dev(mu_sgs,'mu_sgs'), &
+         dev(ProjMass,'ProjMass'), dev(ProjEner,'ProjEner'), dev(ProjMX,'ProjMX'), &
+         dev(ProjMY,'ProjMY'), dev(ProjMZ,'ProjMZ'), dev(tau_stab,'tau_stab'), 1.0_rp, 1.0_rp, &
+         real(c_lps_comp, rp), dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener'), &
+         init_flag(initialze), fact_diffstab), 'full_diff_stab_ijk')
+   end subroutine full_diff_stab_ijk
 
    ! elem_convec.f90:472-594
    subroutine generic_scalar_convec_ijk(nelem,npoin,connec,Ngp,dNgp,He,gpvol,dlxigp_ip,xgp, &

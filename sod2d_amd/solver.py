@@ -23,7 +23,16 @@ The entropy-viscosity sensor (``sensor_path``) is either smart_visc_spectral
 reference's live path calls its _imex variant, or that live path itself
 ("live"): smart_visc_spectral_imex at the first step (time_integ_ls.f90:292)
 and, after every step, smart_visc_spectral_imex, eval_divergence,
-compute_vorticity and ducros_visc_limit (time_integ_ls.f90:455-463).
+compute_vorticity and ducros_visc_limit (time_integ_ls.f90:455-463);
+``high_mach`` selects that sensor's high-Mach normalisation (flag_high_mach,
+set by the reference's compressible TGV solver).
+
+``lps=True`` turns on the LPS stabilisation of the viscous operator
+(flag_lps_stab, the reference default): comp_tau at the start of every step
+and, in every updateF, full_proj_ijk and full_diff_stab_ijk in place of
+full_diffusion_ijk.  That path runs through the drop-in operators
+(sodhip_full_convec_ijk_f64, _full_proj_ijk_f64, _full_diff_stab_ijk_f64) and
+the stage update, not the fused stage kernel.
 """
 from __future__ import annotations
 
@@ -36,7 +45,7 @@ import torch
 
 from . import case as case_mod
 from ._lib import SodhipError, StageParams, ViscParams, check, lib
-from .device import Context, _ptr
+from .device import C_LPS_COMP, C_SPECIES_STAB, Context, _ptr
 
 F64, F32, I32 = torch.float64, torch.float32, torch.int32
 
@@ -104,7 +113,8 @@ class TGVRun:
                  length: float = 2.0 * math.pi, stages: int = 5, variant: int = 1,
                  op_precision: int = 8, parity: bool = True, dt_mode: str = "adaptive",
                  uniform_rho: bool = False, sensor: bool = True, ctx: Context | None = None,
-                 geometry=None, initial=None, sensor_path: str = "spectral"):
+                 geometry=None, initial=None, sensor_path: str = "spectral", lps: bool = False,
+                 high_mach: bool = False):
         self.t = params or TGVParams()
         self.ctx = ctx or Context(torch.cuda.current_device())
         self.p = porder
@@ -115,6 +125,12 @@ class TGVRun:
         if sensor_path not in ("spectral", "live"):
             raise SodhipError(f"unknown sensor_path {sensor_path!r}")
         self.sensor_path = sensor_path
+        # LPS stabilisation (flag_lps_stab) runs updateF through the drop-in
+        # operators (FP64): convection, full_proj_ijk and full_diff_stab_ijk
+        if lps and op_precision != 8:
+            raise SodhipError("LPS stabilisation runs with FP64 operators only")
+        self.lps = lps
+        self.high_mach = high_mach
         g = geometry or case_mod.build_case(porder, nelem_dir, periodic=True,
                                              lengths=(length,) * 3)
         self.geo = g
@@ -164,6 +180,10 @@ class TGVRun:
         self.f_eta = z(3 * n)
         if sensor_path == "live":
             self.divU, self.vorti = z(n), z(3 * n)
+        if lps:
+            self.proj = [z(3 * n) for _ in range(5)]  # ProjMass, ProjEner, ProjMX, MY, MZ
+            self.tau_stab = z(ne)
+            self.aux_h = z(n)
         self.first = True
         self.time = 0.0
         self.istep = 0
@@ -213,7 +233,7 @@ class TGVRun:
             check(lib().sodhip_smart_visc_spectral_imex_f64(
                 h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta), _ptr(rho),
                 _ptr(u), _ptr(self.csound), _ptr(eta), _ptr(self.helem), C.byref(self.vp),
-                C.c_double(1.0), 0, _ptr(self.mu_e), _ptr(self.mue_l)),
+                C.c_double(1.0), int(self.high_mach), _ptr(self.mu_e), _ptr(self.mue_l)),
                 "smart_visc_spectral_imex")
             if ducros:
                 check(lib().sodhip_div_curl_f64(
@@ -228,7 +248,42 @@ class TGVRun:
         if self.op_precision == 4:
             self.mu_e_op.copy_(self.mu_e)
 
+    def _update_f(self):
+        """updateF (time_integ_ls.f90:489-691) through the drop-in operators, with
+        LPS stabilisation: EOS, Sutherland, full_convec_ijk, full_proj_ijk and
+        full_diff_stab_ijk; R is left before the lumped solve."""
+        h, p, ne, n, nw, t = self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, self.t
+        tabs = [_ptr(x) for x in self.tabs]
+        geo = (_ptr(self.He), _ptr(self.gpvol), _ptr(self.dl))
+        R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener))
+        rho, q, u, Tem = self.rho[1], self.q[1], self.u[1], self.Tem[1]
+        check(lib().sodhip_eos_stage_f64(h, n, nw, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas,
+                                         _ptr(rho), _ptr(q), _ptr(self.E[1]), _ptr(u),
+                                         _ptr(self.e_int[1]), _ptr(self.pr[1]), _ptr(self.aux_h),
+                                         _ptr(Tem)), "eos_stage")
+        self._sutherland()
+        check(lib().sodhip_full_convec_ijk_f64(h, p, ne, n, _ptr(self.connec), *geo, *tabs,
+                                               _ptr(u), _ptr(q), _ptr(rho), _ptr(self.pr[1]),
+                                               _ptr(self.aux_h), *R, 1, -1.0), "full_convec_ijk")
+        check(lib().sodhip_full_proj_ijk_f64(h, p, ne, n, nw, _ptr(self.connec),
+                                             _ptr(self.lpoin_w), *geo, *tabs, _ptr(rho), _ptr(u),
+                                             _ptr(Tem), _ptr(self.invMl),
+                                             *[_ptr(x) for x in self.proj]), "full_proj_ijk")
+        check(lib().sodhip_full_diff_stab_ijk_f64(
+            h, p, ne, n, _ptr(self.connec), *geo, *tabs, t.Cp, t.Prt, _ptr(rho), _ptr(rho),
+            _ptr(u), _ptr(Tem), _ptr(self.mu_fluid), _ptr(self.mu_e), None,
+            *[_ptr(x) for x in self.proj], _ptr(self.tau_stab), 1.0, 1.0, C_LPS_COMP, *R, 0,
+            -1.0), "full_diff_stab_ijk")
+
     def _stage(self, lam1, lams, gam, store_R):
+        if self.lps:
+            self._update_f()
+            check(lib().sodhip_rk_stage_update_f64(
+                self.ctx.h, self.npoin, self.npoin_w, _ptr(self.lpoin_w), _ptr(self.invMl),
+                self.dt, lam1, lams, gam, _ptr(self.rho[0]), _ptr(self.q[0]), _ptr(self.E[0]),
+                _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.Rmass),
+                _ptr(self.Rmom), _ptr(self.Rener)), "rk_stage_update")
+            return
         R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener)) if store_R else (None,) * 3
         check(lib().sodhip_rk_stage_f64(
             self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
@@ -261,6 +316,13 @@ class TGVRun:
         n = self.npoin
         self.eta[:n].copy_(self.eta[n:])
         h = self.ctx.h
+        if self.lps:
+            # comp_tau at the start of rk_4_ls_main (time_integ_ls.f90:266-268) on
+            # the level-1 velocity of this step
+            check(lib().sodhip_comp_tau_f64(h, self.p, self.nelem, n, _ptr(self.connec),
+                                            _ptr(self.csound), _ptr(self.u[0]),
+                                            _ptr(self.helem), C_SPECIES_STAB, 0,
+                                            _ptr(self.tau_stab)), "comp_tau")
         if self.first:
             self.first = False
             # f_eta = u(:,:,1)*eta(:,1) over working nodes; Reta(:,1); sensor; updateF

@@ -19,6 +19,8 @@ program shim_driver
    real(8), allocatable :: mue_l(:,:), rho1(:), q1(:,:), E1(:), rhoS(:), qS(:,:), ES(:)
    real(8), allocatable :: mu_imex(:,:), mue_l2(:,:), divU(:), curlU(:,:)
    integer(4), allocatable :: lelpn(:), atoIJK(:)
+   real(8), allocatable :: tau(:), PMass(:,:), PEner(:,:), PMX(:,:), PMY(:,:), PMZ(:,:)
+   real(8), allocatable :: Rsm(:), Rsq(:,:), Rse(:)
    real(8) :: leviCivi(3,3,3)
    real(8) :: Cp, Prt, gamma_gas, ce, cmax, cglob, Rgas, dt, lam1, lams, gam
    integer(4) :: flag_norm
@@ -42,6 +44,8 @@ program shim_driver
    allocate(mue_l(nelem,nnode), rho1(npoin), q1(npoin,3), E1(npoin), rhoS(npoin), qS(npoin,3))
    allocate(ES(npoin), mu_imex(nelem,ngaus), mue_l2(nelem,nnode), divU(npoin), curlU(npoin,3))
    allocate(lelpn(npoin), atoIJK(nnode))
+   allocate(tau(nelem), PMass(npoin,3), PEner(npoin,3), PMX(npoin,3), PMY(npoin,3), PMZ(npoin,3))
+   allocate(Rsm(npoin), Rsq(npoin,3), Rse(npoin))
    leviCivi = 0.0d0; atoIJK = 0; mu_imex = 0.0d0; mue_l2 = 0.0d0
    Ngp = 0.0d0; dNgp = 0.0d0; xgp = 0.0d0; wgp = 0.0d0; coord = 0.0d0; Ml = 0.0d0
    csound = 0.0d0; mu_sgs = 0.0d0; mue_out = 0.0d0; mue_l = 0.0d0
@@ -78,6 +82,11 @@ program shim_driver
    call sod2d_hip_enter_data(Ml); call sod2d_hip_enter_data(mu_imex)
    call sod2d_hip_enter_data(mue_l2); call sod2d_hip_enter_data(divU, .false.)
    call sod2d_hip_enter_data(curlU, .false.)
+   call sod2d_hip_enter_data(tau, .false.); call sod2d_hip_enter_data(PMass, .false.)
+   call sod2d_hip_enter_data(PEner, .false.); call sod2d_hip_enter_data(PMX, .false.)
+   call sod2d_hip_enter_data(PMY, .false.); call sod2d_hip_enter_data(PMZ, .false.)
+   call sod2d_hip_enter_data(Rsm, .false.); call sod2d_hip_enter_data(Rsq, .false.)
+   call sod2d_hip_enter_data(Rse, .false.)
 
    ! the RHS of updateF: convection (initialising) then diffusion (accumulating),
    ! both with fact = -1 (time_integ_ls.f90:605-640)
@@ -126,6 +135,19 @@ program shim_driver
                           divU,curlU,csound,helem_l,helem,Ml,mu_imex,inv,gI,gJ,gK,mue_l2)
    call sod2d_hip_update_host(mu_imex)
    call wr(mu_imex, 'out_mu_ducros')
+
+   ! LPS stabilisation (time_integ_ls.f90:266-268,607-611)
+   call comp_tau(nelem,npoin,connec,csound,u,helem,dt,tau)
+   call full_proj_ijk(nelem,npoin,npoin_w,connec,lpoin_w,Ngp,He,gpvol,dl,inv,gI,gJ,gK,Cp,Prt, &
+                      rho,u,Tem,Ml,invMl,PMass,PEner,PMX,PMY,PMZ)
+   call full_diff_stab_ijk(nelem,npoin,connec,Ngp,He,gpvol,dl,inv,gI,gJ,gK,Cp,Prt,rho,rho,u,Tem, &
+                           mu_fluid,mu_e,mu_sgs,Ml,PMass,PEner,PMX,PMY,PMZ,tau,Rsm,Rsq,Rse, &
+                           .true.,-1.0d0)
+   call sod2d_hip_update_host(tau); call sod2d_hip_update_host(PMass)
+   call sod2d_hip_update_host(PMX); call sod2d_hip_update_host(Rsm)
+   call sod2d_hip_update_host(Rsq); call sod2d_hip_update_host(Rse)
+   call wr(tau, 'out_tau'); call wr(PMass, 'out_PMass'); call wr(PMX, 'out_PMX')
+   call wr(Rsm, 'out_Rsm'); call wr(Rsq, 'out_Rsq'); call wr(Rse, 'out_Rse')
 
    call sod2d_hip_synchronize()
    call sod2d_hip_finalize()

@@ -2,7 +2,8 @@
 reference's subroutine interfaces of full_convec_ijk, full_diffusion_ijk,
 generic_scalar_convec_ijk, smart_visc_spectral, lumped_solver_scal_opt and the
 live path's smart_visc_spectral_imex, eval_divergence, compute_vorticity and
-ducros_visc_limit, and adds the fused stage combine.  The test program tests/fortran/shim_driver.f90
+ducros_visc_limit, the LPS routines comp_tau, full_proj_ijk and
+full_diff_stab_ijk, and adds the fused stage combine.  The test program tests/fortran/shim_driver.f90
 calls them, through the C-ABI, on a case written here; its results are checked
 against the oracle: the residuals, the stage combine and the sensor to 1e-13
 normwise relative (the device arithmetic is reordered, so not bit-exact)."""
@@ -106,3 +107,16 @@ def test_fortran_shim(oracle, tmp_path, p, n, periodic, jitter):
     assert relerr(_r(d, "out_curlU", (npoin, 3)), curl) < 1e-13
     oracle.ducros_visc_limit(T, m.connec, s["u"], divU, curl, g.Ml, g.helem_l, mu_i, 0.1)
     assert relerr(_r(d, "out_mu_ducros", (nelem, nn)), mu_i) < 1e-13
+    # LPS stabilisation
+    tau = oracle.comp_tau(p, m.connec, csound, s["u"], g.helem)
+    assert relerr(_r(d, "out_tau", nelem), tau) < 1e-15
+    P = oracle.full_proj_ijk(T, m.connec, m.lpoin_w, g.He, g.gpvol, s["rho"], s["u"], s["Tem"],
+                             g.invMl)
+    assert relerr(_r(d, "out_PMass", (npoin, 3)), P[0]) < 1e-13
+    assert relerr(_r(d, "out_PMX", (npoin, 3)), P[2]) < 1e-13
+    Rs = oracle.full_diff_stab_ijk(T, m.connec, g.He, g.gpvol, Cp, Pr, s["rho"], s["rho"], s["u"],
+                                   s["Tem"], s["mu_fluid"], s["mu_e"], np.zeros_like(s["mu_e"]),
+                                   P, tau, initialize=True, fact=-1.0)
+    assert relerr(_r(d, "out_Rsm", npoin), Rs[0]) < 1e-13
+    assert relerr(_r(d, "out_Rsq", (npoin, 3)), Rs[1]) < 1e-13
+    assert relerr(_r(d, "out_Rse", npoin), Rs[2]) < 1e-13
