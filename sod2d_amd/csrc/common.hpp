@@ -63,6 +63,7 @@ struct MeshCache {
     const int *lpw_key = nullptr; // device pointer of the caller's lpoin_w
     int npoin_w = 0;
     int *wrow = nullptr, *wsrc = nullptr;
+    unsigned char *wflag = nullptr; // npoin: 1 where the node is in the list
 };
 
 struct PresentEntry {

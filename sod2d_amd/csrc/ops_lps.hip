@@ -64,8 +64,11 @@ __device__ __forceinline__ void lps_gradients(const LineTab<P, double> &lt, cons
             gU[i][j] = hp[j] * g[2 + i][0] + hp[j + 3] * g[2 + i][1] + hp[j + 6] * g[2 + i][2];
 }
 
-// full_proj_ijk's element part (elem_stab.f90:489-563): records of 15 values
-// per element-node, gpvol * (rows 1..3 of 2 S - 2/3 div u I, grad rho, grad T).
+// full_proj_ijk's element part (elem_stab.f90:489-563): records of 12 values
+// per element-node, gpvol * (the upper triangle t11 t12 t13 t22 t23 t33 of the
+// symmetric tensor 2 S - 2/3 div u I, grad rho, grad T).  The tensor rows the
+// reference projects separately are rebuilt from the triangle by the gather:
+// tau(i,j) and tau(j,i) are the same sum gradU(i,j) + gradU(j,i).
 template <int P>
 __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
 {
@@ -96,20 +99,24 @@ __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
         lps_gradients<P>(lt, s, a, hp, gR, gT, gU);
         const double divU = gU[0][0] + gU[1][1] + gU[2][2];
         if (act) {
-            double *o = A.out + 15 * ((size_t)e * NN + pa);
+            double o[12];
+            int f = 0;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
+                for (int d = i; d < 3; ++d) {
                     double t = gU[i][d] + gU[d][i];
                     if (i == d) t = t - (2.0 / 3.0) * divU;
-                    o[3 * i + d] = w * t;
+                    o[f++] = w * t;
                 }
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                o[9 + d] = w * gR[d];
-                o[12 + d] = w * gT[d];
+                o[6 + d] = w * gR[d];
+                o[9 + d] = w * gT[d];
             }
+            double2 *dst = (double2 *)(A.out + 12 * ((size_t)e * NN + pa));
+#pragma unroll
+            for (int v = 0; v < 6; ++v) dst[v] = make_double2(o[2 * v], o[2 * v + 1]);
         }
         lds_barrier();
     }
@@ -253,28 +260,117 @@ __global__ __launch_bounds__(256) void k_comp_tau(int nelem, int nnode, int pord
     tau[e] = taul;
 }
 
-// Node-ordered assembly of NF-value records into NF nodal arrays:
-// R_f(n) = sum_j fact * scr[src[j]*NF + f] over the node's incidence slots in
-// (element, node) order.
-template <int NF> struct NodalOut {
-    double *R[NF];
-};
-template <int NF>
-__global__ __launch_bounds__(256) void k_gather_nf(int npoin, const int *__restrict__ row,
-                                                   const int *__restrict__ src,
-                                                   const double *__restrict__ scr,
-                                                   NodalOut<NF> out, int init, double fact)
+// Node-ordered assemblies.  Node n's slots row[n]..row[n+1]-1 are taken four
+// at a time with every record load issued before the additions, which stay in
+// slot order: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * rec_j[f].  Blocks are
+// mapped XCD-aware (the nodes of one XCD are one contiguous range).
+__device__ __forceinline__ int xcd_node(int nthr)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= npoin) return;
-    const int j0 = row[n], j1 = row[n + 1];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        double acc = init ? 0.0 : out.R[f][n];
-        for (int j = j0; j < j1; ++j) acc += fact * scr[(size_t)src[j] * NF + f];
-        out.R[f][n] = acc;
-    }
+    const int per = gridDim.x >> 3; // grid is a multiple of 8
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int n = blk * blockDim.x + threadIdx.x;
+    return n < nthr ? n : -1;
 }
+
+struct Rec5Out {
+    double *R[5];
+};
+__global__ __launch_bounds__(256) void k_gather5(int npoin, const int *__restrict__ row,
+                                                 const int *__restrict__ src,
+                                                 const double *__restrict__ scr, Rec5Out out,
+                                                 int init, double fact)
+{
+    const int n = xcd_node(npoin);
+    if (n < 0) return;
+    const int j0 = row[n], j1 = row[n + 1];
+    double acc[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) acc[f] = init ? 0.0 : out.R[f][n];
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
+        double v[4][5];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? scr[5 * (size_t)sj[u] + f] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) acc[f] += fact * v[u][f];
+            }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) out.R[f][n] = acc[f];
+}
+
+// full_proj_ijk's assembly and lumped solve: the 12-value records summed per
+// node (the projections start from zero, elem_stab.f90:466-472), scaled by
+// invMl on the working nodes (lumped_solver_vect_opt, :576-581), and written
+// as ProjMX/MY/MZ (rows of the symmetric tensor), ProjMass, ProjEner.
+struct ProjOut {
+    double *MX, *MY, *MZ, *Mass, *Ener;
+};
+__global__ __launch_bounds__(256) void k_gather_proj(int npoin, const int *__restrict__ row,
+                                                     const int *__restrict__ src,
+                                                     const double *__restrict__ scr,
+                                                     const unsigned char *__restrict__ wflag,
+                                                     const double *__restrict__ invMl,
+                                                     ProjOut out)
+{
+    const int n = xcd_node(npoin);
+    if (n < 0) return;
+    const int j0 = row[n], j1 = row[n + 1];
+    double acc[12];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) acc[f] = 0.0;
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
+        double2 v[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double2 *r = (const double2 *)(scr + 12 * (size_t)max(sj[u], 0));
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[u][q] = sj[u] >= 0 ? r[q] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    acc[2 * q] += v[u][q].x;
+                    acc[2 * q + 1] += v[u][q].y;
+                }
+            }
+    }
+    if (wflag[n]) {
+        const double m = invMl[n];
+#pragma unroll
+        for (int f = 0; f < 12; ++f) acc[f] *= m;
+    }
+    const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
+    out.MX[n] = acc[0];
+    out.MX[n1] = acc[1];
+    out.MX[n2] = acc[2];
+    out.MY[n] = acc[1];
+    out.MY[n1] = acc[3];
+    out.MY[n2] = acc[4];
+    out.MZ[n] = acc[2];
+    out.MZ[n1] = acc[4];
+    out.MZ[n2] = acc[5];
+    out.Mass[n] = acc[6];
+    out.Mass[n1] = acc[7];
+    out.Mass[n2] = acc[8];
+    out.Ener[n] = acc[9];
+    out.Ener[n1] = acc[10];
+    out.Ener[n2] = acc[11];
+}
+
+static int node_grid(int npoin) { return ((npoin + 255) / 256 + 7) / 8 * 8; }
 
 static int lps_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
                       const double *He, const double *gpvol, const double *dl, const int *inv,
@@ -337,7 +433,8 @@ extern "C" int sodhip_full_proj_ijk_f64(sodhip_ctx ctx, int porder, int nelem, i
                    "null array argument");
     SODHIP_REQUIRE(npoin_w > 0 && npoin_w <= npoin, "invalid npoin_w");
     const size_t EN = (size_t)nelem * nnode_of(porder);
-    if ((rc = ensure_buf(ctx->scratch, 15 * EN * sizeof(double)))) return rc;
+    if ((rc = ensure_buf(ctx->scratch, 12 * EN * sizeof(double)))) return rc;
+    if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
     LpsArgs A{};
     A.nelem = nelem;
     A.npoin = npoin;
@@ -358,18 +455,11 @@ extern "C" int sodhip_full_proj_ijk_f64(sodhip_ctx ctx, int porder, int nelem, i
     int tok = prof_begin(ctx, 3);
     SODHIP_LPS_LAUNCH(k_lps_proj, grid, A)
     SODHIP_TRY_HIP(hipGetLastError());
-    NodalOut<15> o;
-    double *P[5] = {ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner};
-    for (int q = 0; q < 5; ++q)
-        for (int d = 0; d < 3; ++d) o.R[3 * q + d] = P[q] + (size_t)npoin * d;
-    k_gather_nf<15><<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(npoin, m->row, m->src, A.out, o,
-                                                                  1, 1.0);
+    k_gather_proj<<<node_grid(npoin), 256, 0, ctx->stream>>>(
+        npoin, m->row, m->src, A.out, m->wflag, invMl,
+        ProjOut{ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner});
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
-    // lumped solve of each projection over the working nodes
-    for (int q = 0; q < 5; ++q)
-        if ((rc = sodhip_lumped_solver_vect_f64(ctx, npoin, npoin_w, lpoin_w, invMl, P[q])))
-            return rc;
     return SODHIP_OK;
 }
 
@@ -428,15 +518,10 @@ extern "C" int sodhip_full_diff_stab_ijk_f64(
     SODHIP_LPS_LAUNCH(k_diff_stab, grid, A)
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
-    NodalOut<5> o;
-    o.R[0] = Rmass;
-    o.R[1] = Rmom;
-    o.R[2] = Rmom + npoin;
-    o.R[3] = Rmom + 2 * (size_t)npoin;
-    o.R[4] = Rener;
+    Rec5Out o{{Rmass, Rmom, Rmom + npoin, Rmom + 2 * (size_t)npoin, Rener}};
     tok = prof_begin(ctx, 1);
-    k_gather_nf<5><<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(npoin, m->row, m->src, A.out, o,
-                                                                 initialize, fact);
+    k_gather5<<<node_grid(npoin), 256, 0, ctx->stream>>>(npoin, m->row, m->src, A.out, o,
+                                                          initialize, fact);
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
     return SODHIP_OK;

@@ -59,6 +59,7 @@ static void free_mesh(MeshCache *m)
     (void)hipFree(m->perm);
     (void)hipFree(m->wrow);
     (void)hipFree(m->wsrc);
+    (void)hipFree(m->wflag);
 }
 
 int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
@@ -149,6 +150,8 @@ int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
             return set_error(SODHIP_ERR_ARG, "lpoin_w entry %d outside 1..npoin=%d", lw[i], m->npoin);
         wrow[(size_t)i + 1] = wrow[i] + (m->h_row[(size_t)n + 1] - m->h_row[n]);
     }
+    std::vector<unsigned char> wflag((size_t)m->npoin, 0);
+    for (int i = 0; i < npoin_w; ++i) wflag[(size_t)lw[i] - 1] = 1;
     std::vector<int> wsrc(wrow[npoin_w]);
     for (int i = 0; i < npoin_w; ++i) {
         const int n = lw[i] - 1;
@@ -157,11 +160,16 @@ int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
     }
     (void)hipFree(m->wrow);
     (void)hipFree(m->wsrc);
+    (void)hipFree(m->wflag);
     m->wrow = m->wsrc = nullptr;
+    m->wflag = nullptr;
     m->lpw_key = nullptr;
     if (hipMalloc(&m->wrow, wrow.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&m->wsrc, std::max<size_t>(1, wsrc.size()) * sizeof(int)) != hipSuccess)
+        hipMalloc(&m->wsrc, std::max<size_t>(1, wsrc.size()) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&m->wflag, wflag.size()) != hipSuccess)
         return set_error(SODHIP_ERR_ALLOC, "working incidence allocation failed");
+    SODHIP_TRY_HIP(hipMemcpyAsync(m->wflag, wflag.data(), wflag.size(), hipMemcpyHostToDevice,
+                                  ctx->stream));
     SODHIP_TRY_HIP(hipMemcpyAsync(m->wrow, wrow.data(), wrow.size() * sizeof(int),
                                   hipMemcpyHostToDevice, ctx->stream));
     if (!wsrc.empty())
