@@ -206,7 +206,9 @@ int sodhip_full_diff_stab_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npo
                                   double *Rener, int initialize, double fact);
 
 /* generic_scalar_convec_ijk (elem_convec.f90:472-594): q is the flux f(npoin,3),
- * eta the transported scalar, u the velocity; output Rconvec(npoin). */
+ * eta the transported scalar, u the velocity; output Rconvec(npoin).  q may be
+ * NULL: the flux is then f = u * eta, formed in the kernel (the time loop's
+ * f_eta = u(:,:,1) * eta(:,1), time_integ_ls.f90:277-281 and thermo_entropy). */
 int sodhip_generic_scalar_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
                                          const int *connec, const double *He, const double *gpvol,
                                          const double *dlxigp_ip, const int *invAtoIJK,
@@ -327,7 +329,9 @@ int sodhip_eos_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin
  * may be NULL, then the scalar mu_factor_scalar applies to every node. */
 int sodhip_sutherland_f64(sodhip_ctx ctx, int npoin, const double *Tem, const double *mu_factor,
                           double mu_factor_scalar, double *mu_fluid);
-/* thermo_entropy (mod_entropy_viscosity.f90:411-433); eta(npoin,2), feta(npoin,3) */
+/* thermo_entropy (mod_entropy_viscosity.f90:411-433); eta(npoin,2), feta(npoin,3).
+ * feta may be NULL (then u is not read): the flux is not stored, for callers
+ * that pass q = NULL to sodhip_generic_scalar_convec_ijk_f64. */
 int sodhip_thermo_entropy_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
                               double gamma_gas, double p_inf, double rho_inf, const double *rho,
                               const double *pr, const double *u, double *eta, double *feta);
@@ -368,6 +372,25 @@ int sodhip_rk_stage_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int np
                         double dt, double lam1, double lams, double gam, const double *rho1,
                         const double *q1, const double *E1, double *rho, double *q, double *E,
                         double *Rmass, double *Rmom, double *Rener);
+
+/* sodhip_rk_stage_f64 for the last stage of a step at fixed dt, fused with the
+ * first stage of the next step of the reference sequencing: besides the new
+ * state y (rho, q, E), stores y_next = 1 * y + 0 * 0 + gam_next * dt * R over
+ * the working nodes, the stage update of time_integ_ls.f90:323-331 with the
+ * residual this stage keeps (R as stored by sodhip_rk_stage_f64).  No
+ * counterpart in the reference, which evaluates that update at the start of
+ * the next step; valid while dt is unchanged between the two steps.  y_next
+ * must not alias y; it may alias y1 (each node reads y1 before writing). */
+int sodhip_rk_stage_next_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                             const int *connec, const int *lpoin_w, int op_precision,
+                             const void *He, const void *gpvol, const void *dlxigp_ip,
+                             const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                             const int *gmshAtoK, const void *mu_e, const double *invMl,
+                             const sodhip_stage_params *prm, double dt, double lam1, double lams,
+                             double gam, const double *rho1, const double *q1, const double *E1,
+                             double *rho, double *q, double *E, double *Rmass, double *Rmom,
+                             double *Rener, double gam_next, double *rho_next, double *q_next,
+                             double *E_next);
 
 /* ------------------------------------------------------------------------ */
 /* Time step and diagnostics (results returned to the host; synchronising)   */

@@ -562,7 +562,9 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
 
 // generic_scalar_convec_ijk (elem_convec.f90:472-594):
 // Re = gpvol [ 1/2 (div f + eta div u) + 1/2 u.grad(eta) ]
-template <int P, typename T>
+// FUSEF: the flux is f = u * eta, formed in registers instead of read (the
+// time loop's f_eta = u(:,:,1) * eta(:,1) of the same two arrays)
+template <int P, typename T, bool FUSEF>
 __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const T *__restrict__ f,
                                                              const T *__restrict__ eta)
 {
@@ -584,7 +586,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_scalar_convec(RhsArgs<T> A, const
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             x.u[d] = A.u[n + (size_t)A.npoin * d];
-            x.fv[d] = f[n + (size_t)A.npoin * d];
+            x.fv[d] = FUSEF ? x.u[d] * x.et : f[n + (size_t)A.npoin * d];
         }
         const T *hp = A.He + 9 * ea;
 #pragma unroll
@@ -1653,7 +1655,7 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
 {
     int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
     if (rc) return rc;
-    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && q && eta && u && Rconvec,
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && eta && u && Rconvec,
                    "null array argument");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     MeshCache *m = nullptr;
@@ -1677,11 +1679,17 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
     A.out = (T *)ctx->scratch.ptr;
     const int grid = elem_grid(ctx, nelem, 16);
     int tok = prof_begin(ctx, 3);
+#define SODHIP_SC(PP)                                                                       \
+    if (q)                                                                                  \
+        k_scalar_convec<PP, T, false><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A, q, eta);     \
+    else                                                                                    \
+        k_scalar_convec<PP, T, true><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A, q, eta);
     switch (porder) {
-    case 2: k_scalar_convec<2, T><<<grid, EG<2>::NT, 0, ctx->stream>>>(A, q, eta); break;
-    case 3: k_scalar_convec<3, T><<<grid, EG<3>::NT, 0, ctx->stream>>>(A, q, eta); break;
-    case 4: k_scalar_convec<4, T><<<grid, EG<4>::NT, 0, ctx->stream>>>(A, q, eta); break;
+    case 2: SODHIP_SC(2) break;
+    case 3: SODHIP_SC(3) break;
+    case 4: SODHIP_SC(4) break;
     }
+#undef SODHIP_SC
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return launch_gather<T, T>(ctx, m, A.out, EN, 1, Rconvec, nullptr, nullptr, nullptr, nullptr,

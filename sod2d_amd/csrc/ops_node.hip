@@ -224,8 +224,10 @@ __global__ __launch_bounds__(256) void k_thermo_entropy(int npoin, int npoin_w,
     const double r = rho[n];
     eta[n + (size_t)npoin] =
         (r / (g - 1.0)) * log(fmax(pr[n] / p_inf, 2.220446049250313e-16) / pow(r / rho_inf, g));
+    if (feta) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) feta[n + (size_t)npoin * d] = u[n + (size_t)npoin * d] * e1;
+        for (int d = 0; d < 3; ++d) feta[n + (size_t)npoin * d] = u[n + (size_t)npoin * d] * e1;
+    }
 }
 
 // entropy residual of rk_4_ls_main (time_integ_ls.f90:437-443); Reta and eta are (npoin,2)
@@ -298,7 +300,7 @@ extern "C" int sodhip_thermo_entropy_f64(sodhip_ctx ctx, int npoin, int npoin_w,
                                          double rho_inf, const double *rho, const double *pr,
                                          const double *u, double *eta, double *feta)
 {
-    SODHIP_REQUIRE(ctx && lpoin_w && rho && pr && u && eta && feta, "null argument");
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && pr && eta && (u || !feta), "null argument");
     SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     SODHIP_NODE_LAUNCH(3, npoin_w, k_thermo_entropy, npoin, npoin_w, lpoin_w, gamma_gas, p_inf,

@@ -30,15 +30,25 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 // RD1 / RDY: the base state y1 / the current state y enters the combine (its
 // coefficient lam1 / lams is non-zero); otherwise the term is 0 * 0, the same
 // value as 0 * (finite y), without reading the field.
-template <typename TS, bool RD1, bool RDY>
+// NX: the new y also yields the first stage of the next step of the
+// reference sequencing, yn = nl1 * y + 0 * 0 + ngdt * R (the stage update with
+// lam1 = 1, lams = 0 and the kept residual, time_integ_ls.f90:323-331), which
+// is stored to rhoN/qN/EN.
+struct NextStage {
+    double nl1, ngdt;
+    double *rho, *q, *E;
+};
+template <typename TS, bool RD1, bool RDY, bool NX>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
     const int *__restrict__ wsrc, const TS *__restrict__ scr, size_t EN,
     const double *__restrict__ invMl, double fact, double lam1, double lams, double gdt,
-    const double *__restrict__ rho1, const double *__restrict__ q1, const double *__restrict__ E1,
-    double *__restrict__ rho, double *__restrict__ q, double *__restrict__ E,
-    double *__restrict__ Rmass, double *__restrict__ Rmom, double *__restrict__ Rener)
+    const double *rho1, const double *q1, const double *E1, double *__restrict__ rho,
+    double *__restrict__ q, double *__restrict__ E, double *__restrict__ Rmass,
+    double *__restrict__ Rmom, double *__restrict__ Rener, NextStage nx)
 {
+    // y1 may share memory with the next-stage output (each node's y1 is read
+    // before its next-stage value is written), so y1 carries no __restrict__
     // XCD-aware: the blocks of one XCD (blockIdx % 8) take one contiguous
     // eighth of the nodes, whose partials share that XCD's L2 lines
     const int per = gridDim.x >> 3; // grid is a multiple of 8
@@ -76,11 +86,24 @@ __global__ __launch_bounds__(256) void k_gather_update(
     }
     const double rm = acc[0] * m, r0 = acc[1] * m, r1 = acc[2] * m, r2 = acc[3] * m,
                  re = acc[4] * m;
-    rho[n] = lam1 * b0 + lams * y0 + gdt * rm;
-    q[n] = lam1 * b1 + lams * y1 + gdt * r0;
-    q[n1] = lam1 * b2 + lams * y2 + gdt * r1;
-    q[n2] = lam1 * b3 + lams * y3 + gdt * r2;
-    E[n] = lam1 * b4 + lams * y4 + gdt * re;
+    const double z0 = lam1 * b0 + lams * y0 + gdt * rm;
+    const double z1 = lam1 * b1 + lams * y1 + gdt * r0;
+    const double z2 = lam1 * b2 + lams * y2 + gdt * r1;
+    const double z3 = lam1 * b3 + lams * y3 + gdt * r2;
+    const double z4 = lam1 * b4 + lams * y4 + gdt * re;
+    rho[n] = z0;
+    q[n] = z1;
+    q[n1] = z2;
+    q[n2] = z3;
+    E[n] = z4;
+    if constexpr (NX) {
+        const double zero = 0.0, nls = 0.0;
+        nx.rho[n] = nx.nl1 * z0 + nls * zero + nx.ngdt * rm;
+        nx.q[n] = nx.nl1 * z1 + nls * zero + nx.ngdt * r0;
+        nx.q[n1] = nx.nl1 * z2 + nls * zero + nx.ngdt * r1;
+        nx.q[n2] = nx.nl1 * z3 + nls * zero + nx.ngdt * r2;
+        nx.E[n] = nx.nl1 * z4 + nls * zero + nx.ngdt * re;
+    }
     if (Rmass) {
         Rmass[n] = rm;
         Rmom[n] = r0;
@@ -97,7 +120,7 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
                     const T *mu_e, const double *invMl, const sodhip_stage_params *prm, double dt,
                     double lam1, double lams, double gam, const double *rho1, const double *q1,
                     const double *E1, double *rho, double *q, double *E, double *Rmass,
-                    double *Rmom, double *Rener)
+                    double *Rmom, double *Rener, const NextStage *next = nullptr)
 {
     MeshCache *m = nullptr;
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
@@ -113,10 +136,16 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     SODHIP_TRY_HIP(hipGetLastError());
     int tok = prof_begin(ctx, 1);
     const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
-#define SODHIP_GU(R1, RY)                                                                      \
-    k_gather_update<T, R1, RY><<<grid, 256, 0, ctx->stream>>>(                                 \
+    const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
+#define SODHIP_GU2(R1, RY, NX)                                                                 \
+    k_gather_update<T, R1, RY, NX><<<grid, 256, 0, ctx->stream>>>(                             \
         npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, prm->fact, lam1, lams,      \
-        gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener)
+        gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
+#define SODHIP_GU(R1, RY)                                                                      \
+    if (next)                                                                                  \
+        SODHIP_GU2(R1, RY, true);                                                              \
+    else                                                                                       \
+        SODHIP_GU2(R1, RY, false)
     if (lam1 != 0.0 && lams != 0.0)
         SODHIP_GU(true, true);
     else if (lam1 != 0.0)
@@ -126,6 +155,7 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     else
         SODHIP_GU(false, false);
 #undef SODHIP_GU
+#undef SODHIP_GU2
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
@@ -342,6 +372,43 @@ extern "C" int sodhip_rk_stage_f64(sodhip_ctx ctx, int porder, int nelem, int np
                                invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e, invMl,
                                prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom,
                                Rener);
+    return set_error(SODHIP_ERR_ARG, "op_precision must be 8 or 4 (got %d)", op_precision);
+}
+
+extern "C" int sodhip_rk_stage_next_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, int op_precision, const void *He, const void *gpvol,
+    const void *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+    const int *gmshAtoK, const void *mu_e, const double *invMl, const sodhip_stage_params *prm,
+    double dt, double lam1, double lams, double gam, const double *rho1, const double *q1,
+    const double *E1, double *rho, double *q, double *E, double *Rmass, double *Rmom,
+    double *Rener, double gam_next, double *rho_next, double *q_next, double *E_next)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && He && gpvol && dlxigp_ip && invAtoIJK && gmshAtoI &&
+                       gmshAtoJ && gmshAtoK && mu_e && invMl && rho1 && q1 && E1 && rho && q && E,
+                   "null array argument");
+    SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
+    SODHIP_REQUIRE(rho_next && q_next && E_next, "null next-stage array");
+    SODHIP_REQUIRE(rho_next != rho && q_next != q && E_next != E,
+                   "the next stage must not overwrite the new state");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    const NextStage nx{1.0, gam_next * dt, rho_next, q_next, E_next};
+    if (op_precision == 8)
+        return rk_stage<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                                (const double *)He, (const double *)gpvol,
+                                (const double *)dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
+                                (const double *)mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1,
+                                E1, rho, q, E, Rmass, Rmom, Rener, &nx);
+    if (op_precision == 4)
+        return rk_stage<float>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                               (const float *)He, (const float *)gpvol, (const float *)dlxigp_ip,
+                               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e, invMl,
+                               prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom,
+                               Rener, &nx);
     return set_error(SODHIP_ERR_ARG, "op_precision must be 8 or 4 (got %d)", op_precision);
 }
 

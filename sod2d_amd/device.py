@@ -239,7 +239,8 @@ def full_diff_stab_ijk(nelem, npoin, connec, Ngp, He, gpvol, dlxigp_ip, invAtoIJ
 def generic_scalar_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp,
                               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, q, eta, u, Rconvec,
                               initialze=None, fact=None, ctx: Context | None = None):
-    """generic_scalar_convec_ijk (elem_convec.f90:472-594) on the device (FP64)."""
+    """generic_scalar_convec_ijk (elem_convec.f90:472-594) on the device (FP64); q may
+    be None: the flux is then u * eta, formed in the kernel."""
     ctx = ctx or default_context()
     ctx.sync_stream()
     p = _porder(nelem, connec)

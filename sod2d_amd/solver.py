@@ -16,7 +16,10 @@ Modes:
   * ``op_precision`` 8 (FP64 operators) or 4 (FP32 operator arithmetic, FP64
     state, gather, mass inverse, stage combine and reductions).
   * ``dt_mode`` "adaptive" (adapt_dt_cfl after every step, as the reference) or
-    "fixed" (the CFL time step of the initial state, held).
+    "fixed" (the CFL time step of the initial state, held).  At fixed dt with
+    the reference sequencing the last stage of a step also stores the next
+    step's stage 1 (sodhip_rk_stage_next_f64, the same arithmetic), so that
+    stage costs no launch of its own.
 
 The entropy-viscosity sensor (``sensor_path``) is either smart_visc_spectral
 (mod_entropy_viscosity.f90:14-111, "spectral") at the points where the
@@ -177,7 +180,6 @@ class TGVRun:
         self.Rmass, self.Rmom, self.Rener = z(n), z(3 * n), z(n)
         self.Reta = z(2 * n)
         self.auxReta = z(n)
-        self.f_eta = z(3 * n)
         if sensor_path == "live":
             self.divU, self.vorti = z(n), z(3 * n)
         if lps:
@@ -185,6 +187,8 @@ class TGVRun:
             self.tau_stab = z(ne)
             self.aux_h = z(n)
         self.first = True
+        self._next_dt = None  # dt with which the next step's stage 1 was stored
+        self.fuse_next_stage = True  # at fixed dt, see the module docstring
         self.time = 0.0
         self.istep = 0
         # FP32 operators: the uniform reference pressure is removed before the
@@ -213,9 +217,10 @@ class TGVRun:
         return dt.value
 
     def _scalar_convec(self, out):
+        # the flux f_eta = u(:,:,1) * eta(:,1) is formed in the kernel (q = NULL)
         check(lib().sodhip_generic_scalar_convec_ijk_f64(
             self.ctx.h, self.p, self.nelem, self.npoin, _ptr(self.connec), _ptr(self.He),
-            _ptr(self.gpvol), _ptr(self.dl), *[_ptr(x) for x in self.tabs], _ptr(self.f_eta),
+            _ptr(self.gpvol), _ptr(self.dl), *[_ptr(x) for x in self.tabs], None,
             _ptr(self.eta), _ptr(self.u[0]), _ptr(out), 1, 1.0), "generic_scalar_convec_ijk")
         check(lib().sodhip_lumped_solver_scal_f64(self.ctx.h, self.npoin, self.npoin_w,
                                                   _ptr(self.lpoin_w), _ptr(self.invMl), _ptr(out)),
@@ -275,7 +280,7 @@ class TGVRun:
             *[_ptr(x) for x in self.proj], _ptr(self.tau_stab), 1.0, 1.0, C_LPS_COMP, *R, 0,
             -1.0), "full_diff_stab_ijk")
 
-    def _stage(self, lam1, lams, gam, store_R):
+    def _stage(self, lam1, lams, gam, store_R, next_gam=None):
         if self.lps:
             self._update_f()
             check(lib().sodhip_rk_stage_update_f64(
@@ -285,13 +290,21 @@ class TGVRun:
                 _ptr(self.Rmom), _ptr(self.Rener)), "rk_stage_update")
             return
         R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener)) if store_R else (None,) * 3
-        check(lib().sodhip_rk_stage_f64(
-            self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
-            _ptr(self.lpoin_w), self.op_precision, _ptr(self.He_op), _ptr(self.gpvol_op),
-            _ptr(self.dl_op), *[_ptr(x) for x in self.tabs], _ptr(self.mu_e_op), _ptr(self.invMl),
-            C.byref(self.sp), self.dt, lam1, lams, gam, _ptr(self.rho[0]), _ptr(self.q[0]),
-            _ptr(self.E[0]), _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]), *R),
-            "rk_stage")
+        args = (self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
+                _ptr(self.lpoin_w), self.op_precision, _ptr(self.He_op), _ptr(self.gpvol_op),
+                _ptr(self.dl_op), *[_ptr(x) for x in self.tabs], _ptr(self.mu_e_op),
+                _ptr(self.invMl), C.byref(self.sp), self.dt, lam1, lams, gam, _ptr(self.rho[0]),
+                _ptr(self.q[0]), _ptr(self.E[0]), _ptr(self.rho[1]), _ptr(self.q[1]),
+                _ptr(self.E[1]), *R)
+        if next_gam is None:
+            check(lib().sodhip_rk_stage_f64(*args), "rk_stage")
+        else:
+            # the next step's stage 1 goes to level 1, which becomes level 2 at
+            # that step's exchange
+            check(lib().sodhip_rk_stage_next_f64(*args, next_gam, _ptr(self.rho[0]),
+                                                 _ptr(self.q[0]), _ptr(self.E[0])),
+                  "rk_stage_next")
+            self._next_dt = self.dt
 
     # ------------------------------------------------------------ time step
     def step(self):
@@ -325,15 +338,14 @@ class TGVRun:
                                             _ptr(self.tau_stab)), "comp_tau")
         if self.first:
             self.first = False
-            # f_eta = u(:,:,1)*eta(:,1) over working nodes; Reta(:,1); sensor; updateF
-            w = self.lpoin_w.long() - 1
-            for d in range(3):
-                self.f_eta[d * n + w] = self.u[0][d * n + w] * self.eta[w]
+            # Reta(:,1) of f_eta = u(:,:,1)*eta(:,1); sensor; updateF
             self._scalar_convec(self.Reta[:n])
             if self.sensor:
                 self._smart_visc(self.Reta[:n], self.rho[0], self.u[0], self.Tem[0], self.eta[:n])
             self._stage(0.0, 1.0, 0.0, True)  # updateF at y^n: keeps R, y unchanged
-        if self.parity:
+        if self.parity and self._next_dt is not None and self._next_dt == self.dt:
+            pass  # stage 1 was stored by the previous step's last stage
+        elif self.parity:
             # stage 1 with the kept residual (time_integ_ls.f90:323-331)
             check(lib().sodhip_rk_stage_update_f64(
                 h, n, self.npoin_w, _ptr(self.lpoin_w), None, self.dt, 1.0, 0.0, G[1, 0],
@@ -342,8 +354,13 @@ class TGVRun:
                 _ptr(self.Rener)), "rk_stage_update")
         else:
             self._stage(0.0, 1.0, G[1, 0], False)
+        self._next_dt = None
+        # at fixed dt the last stage also stores the next step's stage 1
+        fuse_next = (self.fuse_next_stage and self.parity and self.dt_mode == "fixed"
+                     and not self.lps)
         for s in range(2, S + 1):
-            self._stage(L[s, 0], L[s, s - 1], G[s, s - 1], store_R=(s == S and self.parity))
+            self._stage(L[s, 0], L[s, s - 1], G[s, s - 1], store_R=(s == S and self.parity),
+                        next_gam=G[1, 0] if (s == S and fuse_next) else None)
         self.end_of_step()
         self.time += self.dt
         self.istep += 1
@@ -360,7 +377,7 @@ class TGVRun:
             _ptr(self.pr[1]), _ptr(self.csound), _ptr(self.machno), _ptr(self.Tem[1])), "eos_step")
         check(lib().sodhip_thermo_entropy_f64(
             h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.po, t.rho0, _ptr(self.rho[1]),
-            _ptr(self.pr[1]), _ptr(self.u[0]), _ptr(self.eta), _ptr(self.f_eta)), "thermo_entropy")
+            _ptr(self.pr[1]), None, _ptr(self.eta), None), "thermo_entropy")
         self._scalar_convec(self.Reta[n:])
         check(lib().sodhip_entropy_residual_f64(
             h, n, self.npoin_w, _ptr(self.lpoin_w), 1.0, self.dt, _ptr(self.Reta), _ptr(self.eta),

@@ -116,6 +116,14 @@ def test_generic_scalar_convec_f64(oracle):
                                 _dev(eta), _dev(u), dR)
     torch.cuda.synchronize()
     assert relerr(dR.cpu().numpy(), R) < 1e-13
+    # q = NULL: the flux u * eta formed in the kernel gives the same bits
+    dR2 = torch.zeros_like(dR)
+    D.generic_scalar_convec_ijk(m.nelem, m.npoin, _dev(m.connec), None, None, _dev(g.He),
+                                _dev(g.gpvol), _dev(T.dlxigp_ip), None, _dev(T.invAtoIJK),
+                                _dev(T.gmshAtoI), _dev(T.gmshAtoJ), _dev(T.gmshAtoK), None,
+                                _dev(eta), _dev(u), dR2)
+    torch.cuda.synchronize()
+    assert torch.equal(dR, dR2)
 
 
 def test_f32_entry_points(oracle):

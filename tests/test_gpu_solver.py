@@ -79,3 +79,25 @@ def test_time_loop_fp32_operators(oracle):
     w = orc.lpoin_w - 1
     e = {k: relerr(s[k][w], getattr(orc, k)[1][w]) for k in ("rho", "q", "E")}
     assert e["rho"] < 2e-7 and e["q"] < 1e-5 and e["E"] < 2e-7, e
+
+
+@pytest.mark.parametrize("op_precision", [8, 4])
+def test_time_loop_fixed_dt_next_stage(oracle, op_precision):
+    """At fixed dt the last stage of a step also stores the next step's stage 1
+    (sodhip_rk_stage_next_f64): the same per-step bars against the oracle, and
+    the same bits as the separate stage-1 update."""
+    dev, orc = _setup(oracle, 3, 4, op_precision=op_precision, dt_mode="fixed")
+    ref, _ = _setup(oracle, 3, 4, op_precision=op_precision, dt_mode="fixed")
+    ref.fuse_next_stage = False  # stage 1 evaluated at the start of each step
+    for step in range(6):
+        dev.step()
+        ref.step()
+        orc.step()
+        s, r = dev.state(), ref.state()
+        for k in ("rho", "q", "E", "mu_e"):
+            assert np.array_equal(s[k], r[k]), (step, k)
+        if op_precision == 8:
+            w = orc.lpoin_w - 1
+            assert relerr(s["rho"][w], orc.rho[1][w]) < 1e-12, step
+            assert relerr(s["q"][w], orc.q[1][w]) < 1e-12, step
+            assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
