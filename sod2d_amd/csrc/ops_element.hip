@@ -740,27 +740,48 @@ __global__ __launch_bounds__(256) void k_per_elem_count(int npoin, int npoin_w,
     for (int d = 0; d < 3; ++d) curl[n + (size_t)npoin * d] /= c;
 }
 
-// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * scr[src[j]*nf + f], j
+// Node-ordered assembly: R_f(n) = (init ? 0 : R_f(n)) + sum_j fact * scr[src[j]*NF + f], j
 // over the node's slots row[n]..row[n+1]-1 in ascending (element, node) order
-// (records of nf fields per element-node).
-template <typename TS, typename TR>
+// (records of NF fields per element-node).  Slots are taken four at a time
+// with every record load issued before the additions, which stay in slot
+// order; blocks are mapped XCD-aware (the nodes of one XCD are one contiguous
+// range, whose records share that XCD's L2).
+template <typename TS, typename TR, int NF>
 __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict__ row,
                                                 const int *__restrict__ src,
-                                                const TS *__restrict__ scr, size_t EN, int nf,
-                                                TR *__restrict__ R0, TR *__restrict__ R1,
-                                                TR *__restrict__ R2, TR *__restrict__ R3,
-                                                TR *__restrict__ R4, int init, TR fact)
+                                                const TS *__restrict__ scr, TR *__restrict__ R0,
+                                                TR *__restrict__ R1, TR *__restrict__ R2,
+                                                TR *__restrict__ R3, TR *__restrict__ R4, int init,
+                                                TR fact)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = gridDim.x >> 3; // grid is a multiple of 8
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int n = blk * blockDim.x + threadIdx.x;
     if (n >= npoin) return;
     const int j0 = row[n], j1 = row[n + 1];
     TR *R[5] = {R0, R1, R2, R3, R4};
-    (void)EN;
-    for (int f = 0; f < nf; ++f) {
-        TR acc = init ? TR(0) : R[f][n];
-        for (int j = j0; j < j1; ++j) acc += fact * TR(scr[(size_t)src[j] * nf + f]);
-        R[f][n] = acc;
+    TR acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = init ? TR(0) : R[f][n];
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
+        TR v[4][NF];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                v[u][f] = sj[u] >= 0 ? TR(scr[(size_t)sj[u] * NF + f]) : TR(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f) acc[f] += fact * v[u][f];
+            }
     }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) R[f][n] = acc[f];
 }
 
 // ---------------------------------------------------------------- launch helpers
@@ -1455,10 +1476,20 @@ template <typename TS, typename TR>
 static int launch_gather(sodhip_ctx ctx, const MeshCache *m, const TS *scr, size_t EN, int nf,
                          TR *R0, TR *R1, TR *R2, TR *R3, TR *R4, int init, TR fact)
 {
+    (void)EN;
     const int bs = 256;
-    const int grid = (m->npoin + bs - 1) / bs;
-    k_gather<TS, TR><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, m->src, scr, EN, nf, R0, R1,
-                                                   R2, R3, R4, init, fact);
+    const int grid = ((m->npoin + bs - 1) / bs + 7) / 8 * 8;
+#define SODHIP_GATHER(NFV)                                                                     \
+    k_gather<TS, TR, NFV><<<grid, bs, 0, ctx->stream>>>(m->npoin, m->row, m->src, scr, R0, R1,  \
+                                                        R2, R3, R4, init, fact)
+    switch (nf) {
+    case 1: SODHIP_GATHER(1); break;
+    case 3: SODHIP_GATHER(3); break;
+    case 4: SODHIP_GATHER(4); break;
+    case 5: SODHIP_GATHER(5); break;
+    default: return set_error(SODHIP_ERR_UNSUPPORTED, "gather of %d fields", nf);
+    }
+#undef SODHIP_GATHER
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
 }
