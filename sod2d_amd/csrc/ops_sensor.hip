@@ -220,135 +220,26 @@ __device__ __forceinline__ int convert_ijk(int g, int P)
     return g + 1; // 2..P -> 3..P+1
 }
 
-template <int P>
-__global__ __launch_bounds__(SG<P>::NT) void k_smart_visc(SensorArgs A)
-{
-    constexpr int N1 = SG<P>::N1, NN = SG<P>::NN, NT = SG<P>::NT;
-    __shared__ double sm[NT];
-    __shared__ double red[NT];
-    __shared__ double s_norm;
-    const int lane = threadIdx.x;
-    const bool act = lane < NN;
-    const int a = act ? lane : 0;
-    if (lane == 0) {
-        double nrm = 1.0;
-        if (A.pmax) {
-            nrm = 0.0;
-            for (int i = 0; i < A.nparts; ++i) nrm = fmax(nrm, A.pmax[i]);
-        }
-        s_norm = nrm;
-    }
-    // the lane's sod2d (I,J,K): position of a in invAtoIJK (a permutation)
-    int I = 1, J = 1, K = 1;
-    for (int k = 0; k < N1; ++k)
-        for (int j = 0; j < N1; ++j)
-            for (int i = 0; i < N1; ++i)
-                if (A.inv[i + N1 * (j + N1 * k)] - 1 == a) {
-                    I = i + 1;
-                    J = j + 1;
-                    K = k + 1;
-                }
-    // the 27 stencil source nodes of the lane's output point
-    const int ii = geo_of(I, N1), jj = geo_of(J, N1), kk = geo_of(K, N1);
-    int src[27];
-#pragma unroll
-    for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const int ci = convert_ijk(ii + l - 1, P) - 1, cj = convert_ijk(jj + m - 1, P) - 1,
-                          ck = convert_ijk(kk + n - 1, P) - 1;
-                src[9 * l + 3 * m + n] = min(max(A.inv[ci + N1 * (cj + N1 * ck)] - 1, 0), NN - 1);
-            }
-    __syncthreads();
-    const double norm = s_norm;
-    const double w[3] = {0.25, 0.5, 0.25};
-    // XCD-aware element ranges: blocks of one XCD (blockIdx % 8) walk one
-    // contiguous eighth of the elements, so the element-fastest mu_e / mue_l /
-    // helem lines are filled by one L2
-    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7, rk = blockIdx.x >> 3;
-    const int chunk = (A.nelem + 7) >> 3;
-    const int eb = xcd * chunk, ee = min(A.nelem, eb + chunk);
-    struct In {
-        double u0, u1, u2, T, rho, Reta, h, hk;
-    };
-    auto load = [&](In &x, int e) {
-        const int n = act ? A.connecT[(size_t)e * NN + a] : 0;
-        x.u0 = A.u[n];
-        x.u1 = A.u[n + A.npoin];
-        x.u2 = A.u[n + 2 * (size_t)A.npoin];
-        x.T = A.Tem[n];
-        x.rho = A.rho[n];
-        x.Reta = A.Reta[n];
-        x.h = act ? A.helem[e + (size_t)A.nelem * a] : 0.0;
-        x.hk = A.helem_k[e];
-    };
-    if (eb + rk >= ee) return;
-    // elements past the block's range are clamped to its last one: the extra
-    // loads keep the prefetch unconditional
-    const int elast = eb + rk + (ee - 1 - eb - rk) / per * per;
-    In cur, nxt;
-    load(cur, eb + rk);
-    for (int e = eb + rk; e < ee; e += per) {
-        load(nxt, min(e + per, elast));
-        const double aux2 = sqrt(cur.u0 * cur.u0 + cur.u1 * cur.u1 + cur.u2 * cur.u2);
-        const double aux3 = A.fact_low_mach * sqrt(A.Rgas * A.gamma * cur.T);
-        double be = act ? (cur.rho * cur.hk) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
-        if constexpr (NT == 64) {
-#pragma unroll
-            for (int s = 32; s > 0; s >>= 1) be = fmax(be, __shfl_xor(be, s, 64));
-        } else {
-            red[lane] = be;
-            lds_barrier();
-            for (int s = NT / 2; s > 0; s >>= 1) {
-                if (lane < s) red[lane] = fmax(red[lane], red[lane + s]);
-                lds_barrier();
-            }
-            be = red[0];
-        }
-        const double betae = be;
-        const double R1 = cur.rho * fabs(cur.Reta) / norm;
-        const double Ve = A.ce * R1 * (cur.h * cur.h);
-        const double ml = A.cglob * fmin(Ve, betae);
-        if (act) {
-            sm[a] = ml;
-            A.mue_l[e + (size_t)A.nelem * a] = ml;
-        }
-        lds_barrier();
-        double acc = 0.0;
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-#pragma unroll
-            for (int m = 0; m < 3; ++m)
-#pragma unroll
-                for (int nn = 0; nn < 3; ++nn) acc += w[l] * w[m] * w[nn] * sm[src[9 * l + 3 * m + nn]];
-        if (act) A.mu_e[e + (size_t)A.nelem * a] = acc;
-        cur = nxt;
-        lds_barrier();
-    }
-}
-
-// Chunked variant for (p+1)^3 <= 64 (one wave per workgroup): the wave walks
-// CH consecutive elements at a time so that the element-fastest arrays
-// helem(nelem,nnode), mue_l and mu_e move as whole 128-byte lines: the CH x NN
-// block is read and written through LDS with lane t holding (node t/CH,
-// element t%CH), while the per-element arithmetic keeps lane = node.
+// Chunked variant: the workgroup (NT = roundup((p+1)^3, 64) lanes, one or two
+// waves) walks CH consecutive elements at a time so that the element-fastest
+// arrays helem(nelem,nnode), mue_l and mu_e move as whole 128-byte lines: the
+// CH x NN block is read and written through LDS with thread t holding (node
+// t/CH, element t%CH), while the per-element arithmetic keeps lane = node.
 template <int P, int CH>
-__global__ __launch_bounds__(64) void k_smart_visc_ch(SensorArgs A)
+__global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgs A)
 {
-    constexpr int N1 = SG<P>::N1, NN = SG<P>::NN;
-    static_assert(NN <= 64, "one wave per element");
-    constexpr int BLK = CH * NN, ROWS = (BLK + 63) / 64;
+    constexpr int N1 = SG<P>::N1, NN = SG<P>::NN, NT = SG<P>::NT, NW = NT / 64;
+    constexpr int BLK = CH * NN, ROWS = (BLK + NT - 1) / NT;
     __shared__ double lb[BLK]; // helem, then mue_l, of the chunk: [c][a]
     __shared__ double ob[BLK]; // mu_e of the chunk
-    const int lane = threadIdx.x;
+    __shared__ double wmax[NW];
+    const int lane = threadIdx.x, wl = lane & 63, wv = lane >> 6;
     const bool act = lane < NN;
     const int a = act ? lane : 0;
     double norm = 1.0;
-    if (A.pmax) { // fold the partial maxima across the wave
+    if (A.pmax) { // every wave folds all partial maxima itself
         norm = 0.0;
-        for (int i = lane; i < A.nparts; i += 64) norm = fmax(norm, A.pmax[i]);
+        for (int i = wl; i < A.nparts; i += 64) norm = fmax(norm, A.pmax[i]);
 #pragma unroll
         for (int s = 32; s > 0; s >>= 1) norm = fmax(norm, __shfl_xor(norm, s, 64));
     }
@@ -401,7 +292,7 @@ __global__ __launch_bounds__(64) void k_smart_visc_ch(SensorArgs A)
         // helem of the chunk, whole lines (elements past the range re-read the last)
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const int t = r * 64 + lane;
+            const int t = r * NT + lane;
             if (t < BLK) {
                 const int an = t / CH, c = t % CH;
                 lb[c * NN + an] = A.helem[(c0 + min(c, cn - 1)) + (size_t)A.nelem * an];
@@ -416,6 +307,13 @@ __global__ __launch_bounds__(64) void k_smart_visc_ch(SensorArgs A)
             double be = act ? (cur.rho * cur.hk) * (A.cmax / (double)P) * (aux2 + aux3) : 0.0;
 #pragma unroll
             for (int s = 32; s > 0; s >>= 1) be = fmax(be, __shfl_xor(be, s, 64));
+            if constexpr (NW > 1) {
+                if (wl == 0) wmax[wv] = be;
+                lds_barrier();
+                be = wmax[0];
+#pragma unroll
+                for (int w2 = 1; w2 < NW; ++w2) be = fmax(be, wmax[w2]);
+            }
             const double R1 = cur.rho * fabs(cur.Reta) / norm;
             const double h = lb[c * NN + a];
             const double Ve = A.ce * R1 * (h * h);
@@ -436,7 +334,7 @@ __global__ __launch_bounds__(64) void k_smart_visc_ch(SensorArgs A)
         lds_barrier();
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const int t = r * 64 + lane;
+            const int t = r * NT + lane;
             if (t < BLK) {
                 const int an = t / CH, c = t % CH;
                 if (c < cn) {
@@ -508,7 +406,7 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     switch (porder) {
     case 2: k_smart_visc_ch<2, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
     case 3: k_smart_visc_ch<3, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
-    case 4: k_smart_visc<4><<<elem_grid(ctx, nelem, 16), SG<4>::NT, 0, ctx->stream>>>(A); break;
+    case 4: k_smart_visc_ch<4, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), SG<4>::NT, 0, ctx->stream>>>(A); break;
     }
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
