@@ -392,6 +392,31 @@ int sodhip_rk_stage_next_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, i
                              double *Rener, double gam_next, double *rho_next, double *q_next,
                              double *E_next);
 
+/* One LS-RK stage of the reference's default viscous path (flag_lps_stab):
+ * updateF (time_integ_ls.f90:489-691) with full_proj_ijk and
+ * full_diff_stab_ijk in place of full_diffusion_ijk, the lumped solve and the
+ * stage combine y <- lam1*y1 + lams*y + gam*dt*R, as sodhip_rk_stage_f64 does
+ * for the plain path (FP64).  u, e_int, pr, aux_h, Tem receive the EOS of the
+ * current state (working nodes), mu_fluid Sutherland's viscosity (all nodes),
+ * the Proj arrays the projections; tau_stab is comp_tau's.  mu_sgs is zero.
+ * The convection and the stabilised viscous terms are summed per element
+ * before the node-ordered gather.  Rmass/Rmom/Rener (all or none) receive
+ * R*invMl; rho_next/q_next/E_next (all or none) the next step's first stage
+ * as in sodhip_rk_stage_next_f64. */
+int sodhip_rk_stage_lps_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                            const int *connec, const int *lpoin_w, const double *He,
+                            const double *gpvol, const double *dlxigp_ip, const int *invAtoIJK,
+                            const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                            const double *mu_e, const double *invMl, const double *tau_stab,
+                            const sodhip_stage_params *prm, double c_lps, double dt, double lam1,
+                            double lams, double gam, const double *rho1, const double *q1,
+                            const double *E1, double *rho, double *q, double *E, double *u,
+                            double *e_int, double *pr, double *aux_h, double *Tem,
+                            double *mu_fluid, double *ProjMass, double *ProjEner, double *ProjMX,
+                            double *ProjMY, double *ProjMZ, double *Rmass, double *Rmom,
+                            double *Rener, double gam_next, double *rho_next, double *q_next,
+                            double *E_next);
+
 /* ------------------------------------------------------------------------ */
 /* Time step and diagnostics (results returned to the host; synchronising)   */
 /* ------------------------------------------------------------------------ */

@@ -116,4 +116,23 @@ void prof_end(sodhip_ctx ctx, int token);
 // XCD-aware grid: number of blocks for a grid-stride element loop
 int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu);
 
+
+// first stage of the next step stored by a stage's gather/update (ops_stage.hip):
+// y_next = nl1 * y + 0 * 0 + ngdt * R
+struct NextStage {
+    double nl1, ngdt;
+    double *rho, *q, *E;
+};
+// gather/update of one stage from 5-value element records (ops_stage.hip)
+template <typename TS>
+int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
+                  const int *lpoin_w, const TS *scr, size_t EN, const double *invMl, double fact,
+                  double dt, double lam1, double lams, double gam, const double *rho1,
+                  const double *q1, const double *E1, double *rho, double *q, double *E,
+                  double *Rmass, double *Rmom, double *Rener, const NextStage *next);
+// full_convec_ijk element records (ops_element.hip)
+template <typename T>
+int conv_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He, const T *gpvol,
+                  const T *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                  const T *u, const T *q, const T *rho, const T *pr, const T *E, T *out);
 } // namespace sodhip

@@ -1503,6 +1503,43 @@ static int check_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const 
     return SODHIP_OK;
 }
 
+// full_convec_ijk's element records only (5 values per element-node, unscaled,
+// at e*nnode + perm[a] of out), for stage drivers that add more terms to the
+// same records before one gather
+template <typename T>
+int conv_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He, const T *gpvol,
+                  const T *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                  const T *u, const T *q, const T *rho, const T *pr, const T *E, T *out)
+{
+    RhsArgs<T> A{};
+    A.nelem = m->nelem;
+    A.npoin = m->npoin;
+    A.EN = (size_t)m->nelem * nnode_of(porder);
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.u = u;
+    A.q = q;
+    A.rho = rho;
+    A.pr = pr;
+    A.E = E;
+    A.out = out;
+    int tok = prof_begin(ctx, 0);
+    const int rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, m->nelem, 16));
+    prof_end(ctx, tok);
+    return rc;
+}
+template int conv_elements<double>(sodhip_ctx, int, const MeshCache *, const double *,
+                                   const double *, const double *, const int *, const int *,
+                                   const int *, const int *, const double *, const double *,
+                                   const double *, const double *, const double *, double *);
+
 template <typename T>
 static int full_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
                        const T *He, const T *gpvol, const T *dl, const int *inv, const int *gI,

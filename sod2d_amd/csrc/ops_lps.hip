@@ -127,8 +127,9 @@ __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
 // momentum, rho_n Cp/Pr for energy), mapped to contravariant components
 // G_k(F) = gpvol sum_d He(d,k) F_d and staged in LDS; then the weak divergence
 // sum_k sum_{b on the k-line} dl(b,k,I_k(a)) G_k(F)(b).  Records of 5 values
-// (mass, momentum 1..3, energy) per element-node.
-template <int P>
+// (mass, momentum 1..3, energy) per element-node; ACC adds them to the records
+// already there (the convection of the same stage).
+template <int P, bool ACC = false>
 __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
 {
     constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
         if (act) {
             double *o = A.out + 5 * ((size_t)e * NN + pa);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) o[f] = R[f];
+            for (int f = 0; f < 5; ++f) o[f] = ACC ? o[f] + R[f] : R[f];
         }
         lds_barrier();
     }
@@ -384,6 +385,11 @@ static int lps_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const in
     return get_mesh(ctx, porder, nelem, npoin, connec, m);
 }
 
+// full_proj_ijk into the caller's projections, records through out (12 x EN)
+static int proj_run(sodhip_ctx ctx, int porder, const MeshCache *m, LpsArgs A,
+                    const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
+                    double *ProjMY, double *ProjMZ);
+
 #define SODHIP_LPS_LAUNCH(KERNEL, GRID, ARGS)                                              \
     switch (porder) {                                                                      \
     case 2: KERNEL<2><<<GRID, EG<2>::NT, 0, ctx->stream>>>(ARGS); break;                   \
@@ -391,6 +397,22 @@ static int lps_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const in
     case 4: KERNEL<4><<<GRID, EG<4>::NT, 0, ctx->stream>>>(ARGS); break;                   \
     default: return set_error(SODHIP_ERR_UNSUPPORTED, "porder %d not supported", porder); \
     }
+
+static int proj_run(sodhip_ctx ctx, int porder, const MeshCache *m, LpsArgs A,
+                    const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
+                    double *ProjMY, double *ProjMZ)
+{
+    const int grid = elem_grid(ctx, m->nelem, 16);
+    int tok = prof_begin(ctx, 3);
+    SODHIP_LPS_LAUNCH(k_lps_proj, grid, A)
+    SODHIP_TRY_HIP(hipGetLastError());
+    k_gather_proj<<<node_grid(m->npoin), 256, 0, ctx->stream>>>(
+        m->npoin, m->row, m->src, A.out, m->wflag, invMl,
+        ProjOut{ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner});
+    SODHIP_TRY_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+    return SODHIP_OK;
+}
 
 } // namespace sodhip
 
@@ -451,16 +473,7 @@ extern "C" int sodhip_full_proj_ijk_f64(sodhip_ctx ctx, int porder, int nelem, i
     A.u = u;
     A.Tem = Tem;
     A.out = (double *)ctx->scratch.ptr;
-    const int grid = elem_grid(ctx, nelem, 16);
-    int tok = prof_begin(ctx, 3);
-    SODHIP_LPS_LAUNCH(k_lps_proj, grid, A)
-    SODHIP_TRY_HIP(hipGetLastError());
-    k_gather_proj<<<node_grid(npoin), 256, 0, ctx->stream>>>(
-        npoin, m->row, m->src, A.out, m->wflag, invMl,
-        ProjOut{ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner});
-    SODHIP_TRY_HIP(hipGetLastError());
-    prof_end(ctx, tok);
-    return SODHIP_OK;
+    return proj_run(ctx, porder, m, A, invMl, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ);
 }
 
 extern "C" int sodhip_full_diff_stab_ijk_f64(
@@ -525,4 +538,95 @@ extern "C" int sodhip_full_diff_stab_ijk_f64(
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
     return SODHIP_OK;
+}
+
+extern "C" int sodhip_rk_stage_lps_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, const double *He, const double *gpvol, const double *dlxigp_ip,
+    const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+    const double *mu_e, const double *invMl, const double *tau_stab,
+    const sodhip_stage_params *prm, double c_lps, double dt, double lam1, double lams,
+    double gam, const double *rho1, const double *q1, const double *E1, double *rho, double *q,
+    double *E, double *u, double *e_int, double *pr, double *aux_h, double *Tem,
+    double *mu_fluid, double *ProjMass, double *ProjEner, double *ProjMX, double *ProjMY,
+    double *ProjMZ, double *Rmass, double *Rmom, double *Rener, double gam_next,
+    double *rho_next, double *q_next, double *E_next)
+{
+    SODHIP_REQUIRE(prm, "null argument");
+    MeshCache *m = nullptr;
+    int rc = lps_common(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                        gmshAtoI, gmshAtoJ, gmshAtoK, &m);
+    if (rc) return rc;
+    SODHIP_REQUIRE(lpoin_w && npoin_w > 0 && npoin_w <= npoin, "invalid working nodes");
+    SODHIP_REQUIRE(mu_e && invMl && tau_stab && rho1 && q1 && E1 && rho && q && E && u &&
+                       e_int && pr && aux_h && Tem && mu_fluid && ProjMass && ProjEner &&
+                       ProjMX && ProjMY && ProjMZ,
+                   "null array argument");
+    SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
+    SODHIP_REQUIRE(!!rho_next == !!q_next && !!q_next == !!E_next, "next stage: all or none");
+    SODHIP_REQUIRE(!rho_next || (rho_next != rho && q_next != q && E_next != E),
+                   "the next stage must not overwrite the new state");
+    if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
+    // updateF's EOS block and Sutherland's law (time_integ_ls.f90:572-599)
+    if ((rc = sodhip_eos_stage_f64(ctx, npoin, npoin_w, lpoin_w, prm->gamma_gas, prm->Rgas, rho, q,
+                                   E, u, e_int, pr, aux_h, Tem)))
+        return rc;
+    if ((rc = sodhip_sutherland_f64(ctx, npoin, Tem, nullptr, prm->mu_factor, mu_fluid))) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    if ((rc = ensure_buf(ctx->scratch, 17 * EN * sizeof(double)))) return rc;
+    double *rec = (double *)ctx->scratch.ptr, *prec = rec + 5 * EN;
+    // convection records (elem_convec.f90:227-470, E slot = enthalpy)
+    if ((rc = conv_elements<double>(ctx, porder, m, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI,
+                                    gmshAtoJ, gmshAtoK, u, q, rho, pr, aux_h, rec)))
+        return rc;
+    LpsArgs A{};
+    A.nelem = nelem;
+    A.npoin = npoin;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dlxigp_ip;
+    A.inv = invAtoIJK;
+    A.gI = gmshAtoI;
+    A.gJ = gmshAtoJ;
+    A.gK = gmshAtoK;
+    A.rho_n = rho;
+    A.rho = rho;
+    A.u = u;
+    A.Tem = Tem;
+    A.mu_fluid = mu_fluid;
+    A.mu_e = mu_e;
+    A.mu_sgs = nullptr;
+    A.pMass = ProjMass;
+    A.pEner = ProjEner;
+    A.pMX = ProjMX;
+    A.pMY = ProjMY;
+    A.pMZ = ProjMZ;
+    A.tau = tau_stab;
+    A.Cp = prm->Cp;
+    A.Pr = prm->Pr;
+    A.c_rho = prm->c_rho;
+    A.c_ener = prm->c_ener;
+    A.c_lps = c_lps;
+    // projections (elem_stab.f90:441-586)
+    A.out = prec;
+    if ((rc = proj_run(ctx, porder, m, A, invMl, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ)))
+        return rc;
+    // stabilised viscous operator added to the convection records: both carry
+    // the same fact (-1), applied once by the gather
+    A.out = rec;
+    const int grid = elem_grid(ctx, nelem, 16);
+    int tok = prof_begin(ctx, 0);
+    switch (porder) {
+    case 2: k_diff_stab<2, true><<<grid, EG<2>::NT, 0, ctx->stream>>>(A); break;
+    case 3: k_diff_stab<3, true><<<grid, EG<3>::NT, 0, ctx->stream>>>(A); break;
+    case 4: k_diff_stab<4, true><<<grid, EG<4>::NT, 0, ctx->stream>>>(A); break;
+    }
+    SODHIP_TRY_HIP(hipGetLastError());
+    prof_end(ctx, tok);
+    const NextStage nx{1.0, gam_next * dt, rho_next, q_next, E_next};
+    return gather_update<double>(ctx, m, npoin, npoin_w, lpoin_w, rec, EN, invMl, prm->fact, dt,
+                                 lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
+                                 rho_next ? &nx : nullptr);
 }

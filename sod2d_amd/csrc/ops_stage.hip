@@ -34,10 +34,6 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 // reference sequencing, yn = nl1 * y + 0 * 0 + ngdt * R (the stage update with
 // lam1 = 1, lams = 0 and the kept residual, time_integ_ls.f90:323-331), which
 // is stored to rhoN/qN/EN.
-struct NextStage {
-    double nl1, ngdt;
-    double *rho, *q, *E;
-};
 template <typename TS, bool RD1, bool RDY, bool NX>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
@@ -113,6 +109,47 @@ __global__ __launch_bounds__(256) void k_gather_update(
     }
 }
 
+// the gather/update of one stage from the element records scr (5 values per
+// element-node, type TS) over the working incidence of m
+template <typename TS>
+int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
+                  const int *lpoin_w, const TS *scr, size_t EN, const double *invMl, double fact,
+                  double dt, double lam1, double lams, double gam, const double *rho1,
+                  const double *q1, const double *E1, double *rho, double *q, double *E,
+                  double *Rmass, double *Rmom, double *Rener, const NextStage *next)
+{
+    int tok = prof_begin(ctx, 1);
+    const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
+    const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
+#define SODHIP_GU2(R1, RY, NX)                                                                 \
+    k_gather_update<TS, R1, RY, NX><<<grid, 256, 0, ctx->stream>>>(                            \
+        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, fact, lam1, lams, gam * dt,  \
+        rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
+#define SODHIP_GU(R1, RY)                                                                      \
+    if (next)                                                                                  \
+        SODHIP_GU2(R1, RY, true);                                                              \
+    else                                                                                       \
+        SODHIP_GU2(R1, RY, false)
+    if (lam1 != 0.0 && lams != 0.0)
+        SODHIP_GU(true, true);
+    else if (lam1 != 0.0)
+        SODHIP_GU(true, false);
+    else if (lams != 0.0)
+        SODHIP_GU(false, true);
+    else
+        SODHIP_GU(false, false);
+#undef SODHIP_GU
+#undef SODHIP_GU2
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+template int gather_update<double>(sodhip_ctx, const MeshCache *, int, int, const int *,
+                                   const double *, size_t, const double *, double, double, double,
+                                   double, double, const double *, const double *, const double *,
+                                   double *, double *, double *, double *, double *, double *,
+                                   const NextStage *);
+
 template <typename T>
 static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
                     const int *connec, const int *lpoin_w, const T *He, const T *gpvol,
@@ -134,31 +171,8 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
                                prm->c_rho, prm->c_ener, prm->p_ref, scr);
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
-    int tok = prof_begin(ctx, 1);
-    const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
-    const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
-#define SODHIP_GU2(R1, RY, NX)                                                                 \
-    k_gather_update<T, R1, RY, NX><<<grid, 256, 0, ctx->stream>>>(                             \
-        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, prm->fact, lam1, lams,      \
-        gam * dt, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
-#define SODHIP_GU(R1, RY)                                                                      \
-    if (next)                                                                                  \
-        SODHIP_GU2(R1, RY, true);                                                              \
-    else                                                                                       \
-        SODHIP_GU2(R1, RY, false)
-    if (lam1 != 0.0 && lams != 0.0)
-        SODHIP_GU(true, true);
-    else if (lam1 != 0.0)
-        SODHIP_GU(true, false);
-    else if (lams != 0.0)
-        SODHIP_GU(false, true);
-    else
-        SODHIP_GU(false, false);
-#undef SODHIP_GU
-#undef SODHIP_GU2
-    prof_end(ctx, tok);
-    SODHIP_TRY_HIP(hipGetLastError());
-    return SODHIP_OK;
+    return gather_update<T>(ctx, m, npoin, npoin_w, lpoin_w, scr, EN, invMl, prm->fact, dt, lam1,
+                            lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
 }
 
 // ---------------------------------------------------------------- CFL time step

@@ -33,9 +33,9 @@ set by the reference's compressible TGV solver).
 ``lps=True`` turns on the LPS stabilisation of the viscous operator
 (flag_lps_stab, the reference default): comp_tau at the start of every step
 and, in every updateF, full_proj_ijk and full_diff_stab_ijk in place of
-full_diffusion_ijk.  That path runs through the drop-in operators
-(sodhip_full_convec_ijk_f64, _full_proj_ijk_f64, _full_diff_stab_ijk_f64) and
-the stage update, not the fused stage kernel.
+full_diffusion_ijk.  A stage of that path is one C-ABI call,
+sodhip_rk_stage_lps_f64, built on the drop-in operators' kernels (convection
+and the stabilised viscous terms summed per element, one gather/update).
 """
 from __future__ import annotations
 
@@ -253,41 +253,25 @@ class TGVRun:
         if self.op_precision == 4:
             self.mu_e_op.copy_(self.mu_e)
 
-    def _update_f(self):
-        """updateF (time_integ_ls.f90:489-691) through the drop-in operators, with
-        LPS stabilisation: EOS, Sutherland, full_convec_ijk, full_proj_ijk and
-        full_diff_stab_ijk; R is left before the lumped solve."""
-        h, p, ne, n, nw, t = self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, self.t
-        tabs = [_ptr(x) for x in self.tabs]
-        geo = (_ptr(self.He), _ptr(self.gpvol), _ptr(self.dl))
-        R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener))
-        rho, q, u, Tem = self.rho[1], self.q[1], self.u[1], self.Tem[1]
-        check(lib().sodhip_eos_stage_f64(h, n, nw, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas,
-                                         _ptr(rho), _ptr(q), _ptr(self.E[1]), _ptr(u),
-                                         _ptr(self.e_int[1]), _ptr(self.pr[1]), _ptr(self.aux_h),
-                                         _ptr(Tem)), "eos_stage")
-        self._sutherland()
-        check(lib().sodhip_full_convec_ijk_f64(h, p, ne, n, _ptr(self.connec), *geo, *tabs,
-                                               _ptr(u), _ptr(q), _ptr(rho), _ptr(self.pr[1]),
-                                               _ptr(self.aux_h), *R, 1, -1.0), "full_convec_ijk")
-        check(lib().sodhip_full_proj_ijk_f64(h, p, ne, n, nw, _ptr(self.connec),
-                                             _ptr(self.lpoin_w), *geo, *tabs, _ptr(rho), _ptr(u),
-                                             _ptr(Tem), _ptr(self.invMl),
-                                             *[_ptr(x) for x in self.proj]), "full_proj_ijk")
-        check(lib().sodhip_full_diff_stab_ijk_f64(
-            h, p, ne, n, _ptr(self.connec), *geo, *tabs, t.Cp, t.Prt, _ptr(rho), _ptr(rho),
-            _ptr(u), _ptr(Tem), _ptr(self.mu_fluid), _ptr(self.mu_e), None,
-            *[_ptr(x) for x in self.proj], _ptr(self.tau_stab), 1.0, 1.0, C_LPS_COMP, *R, 0,
-            -1.0), "full_diff_stab_ijk")
-
     def _stage(self, lam1, lams, gam, store_R, next_gam=None):
         if self.lps:
-            self._update_f()
-            check(lib().sodhip_rk_stage_update_f64(
-                self.ctx.h, self.npoin, self.npoin_w, _ptr(self.lpoin_w), _ptr(self.invMl),
-                self.dt, lam1, lams, gam, _ptr(self.rho[0]), _ptr(self.q[0]), _ptr(self.E[0]),
-                _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.Rmass),
-                _ptr(self.Rmom), _ptr(self.Rener)), "rk_stage_update")
+            # updateF with LPS (EOS, Sutherland, full_convec_ijk, full_proj_ijk,
+            # full_diff_stab_ijk), lumped solve and stage combine in one call
+            R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener)) if store_R else (None,) * 3
+            nxt = ((_ptr(self.rho[0]), _ptr(self.q[0]), _ptr(self.E[0])) if next_gam is not None
+                   else (None,) * 3)
+            check(lib().sodhip_rk_stage_lps_f64(
+                self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
+                _ptr(self.lpoin_w), _ptr(self.He), _ptr(self.gpvol), _ptr(self.dl),
+                *[_ptr(x) for x in self.tabs], _ptr(self.mu_e), _ptr(self.invMl),
+                _ptr(self.tau_stab), C.byref(self.sp), C_LPS_COMP, self.dt, lam1, lams, gam,
+                _ptr(self.rho[0]), _ptr(self.q[0]), _ptr(self.E[0]), _ptr(self.rho[1]),
+                _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.u[1]), _ptr(self.e_int[1]),
+                _ptr(self.pr[1]), _ptr(self.aux_h), _ptr(self.Tem[1]), _ptr(self.mu_fluid),
+                *[_ptr(x) for x in self.proj], *R, 0.0 if next_gam is None else next_gam, *nxt),
+                "rk_stage_lps")
+            if next_gam is not None:
+                self._next_dt = self.dt
             return
         R = (_ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener)) if store_R else (None,) * 3
         args = (self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
@@ -356,8 +340,7 @@ class TGVRun:
             self._stage(0.0, 1.0, G[1, 0], False)
         self._next_dt = None
         # at fixed dt the last stage also stores the next step's stage 1
-        fuse_next = (self.fuse_next_stage and self.parity and self.dt_mode == "fixed"
-                     and not self.lps)
+        fuse_next = self.fuse_next_stage and self.parity and self.dt_mode == "fixed"
         for s in range(2, S + 1):
             self._stage(L[s, 0], L[s, s - 1], G[s, s - 1], store_R=(s == S and self.parity),
                         next_gam=G[1, 0] if (s == S and fuse_next) else None)
