@@ -273,8 +273,8 @@ def main(argv=None):
     k_ms, k_n = prof[0]
     g_ms, g_n = prof[1]
     if a.lps:
-        # two element kernels and two gathers per stage (convection, stabilised
-        # viscous operator); full_proj_ijk and the stage update count as other
+        # two element kernels (convection, stabilised viscous operator) and one
+        # gather/update per stage; full_proj_ijk counts as other
         t_elem = k_ms / (prof_steps * nrhs)
         t_gath = g_ms / (prof_steps * nrhs)
     else:
@@ -316,9 +316,9 @@ def main(argv=None):
             "setup_s": round(t_setup, 2),
         },
         "roofline": {
-            "kernel": ("LPS stage through the drop-in operators: element kernels (convection, "
-                       "stabilised viscous operator) + their gathers, per stage; full_proj_ijk "
-                       "and the stage update excluded" if a.lps else
+            "kernel": ("LPS stage (sodhip_rk_stage_lps_f64): element kernels (convection, "
+                       "stabilised viscous operator) + the gather/update, per stage; "
+                       "full_proj_ijk excluded" if a.lps else
                        "fused RK stage: k_rhs (element) + k_gather_update (node)"),
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
