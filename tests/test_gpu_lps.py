@@ -135,3 +135,30 @@ def test_time_loop_lps_high_mach(oracle):
     min rho, ce = ce_comp) at Ma = 1.25."""
     dev, orc = _loop(oracle, 3, 3, 6, M=1.25, high_mach=True)
     assert dev.vp.ce == orc.prm.ce == 1.0
+
+
+def test_time_loop_lps_fixed_dt_next_stage(oracle):
+    """At fixed dt the LPS stage also stores the next step's stage 1: same bits
+    as the separate stage-1 update, and the oracle's per-step bars."""
+    from sod2d_amd.solver import TGVParams, TGVRun
+    g = make_case(3, 4, periodic=True, lengths=(2 * np.pi,) * 3)
+    tp = TGVParams()
+    to = oracle.tgv_params()
+    ic = oracle.tgv_ic(g.mesh.coord, to)
+    kw = dict(geometry=g, params=tp, initial=ic, sensor_path="live", lps=True, dt_mode="fixed")
+    dev = TGVRun(3, **kw)
+    ref = TGVRun(3, **kw)
+    ref.fuse_next_stage = False
+    orc = oracle.TGVOracle(3, g, ic, to, cfl=(tp.cfl_conv, tp.cfl_diff), sensor_path="live",
+                           lps=True, dt_mode="fixed")
+    for step in range(5):
+        dev.step()
+        ref.step()
+        orc.step()
+        s, r = dev.state(), ref.state()
+        for k in ("rho", "q", "E", "mu_e"):
+            assert np.array_equal(s[k], r[k]), (step, k)
+        w = orc.lpoin_w - 1
+        assert relerr(s["rho"][w], orc.rho[1][w]) < 1e-12, step
+        assert relerr(s["q"][w], orc.q[1][w]) < 1e-12, step
+        assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
