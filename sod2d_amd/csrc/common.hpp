@@ -123,13 +123,14 @@ struct NextStage {
     double nl1, ngdt;
     double *rho, *q, *E;
 };
-// gather/update of one stage from 5-value element records (ops_stage.hip)
+// gather/update of one stage from element records of rw (5 or 6) values
+// (ops_stage.hip)
 template <typename TS>
 int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
-                  const int *lpoin_w, const TS *scr, size_t EN, const double *invMl, double fact,
-                  double dt, double lam1, double lams, double gam, const double *rho1,
-                  const double *q1, const double *E1, double *rho, double *q, double *E,
-                  double *Rmass, double *Rmom, double *Rener, const NextStage *next);
+                  const int *lpoin_w, const TS *scr, int rw, size_t EN, const double *invMl,
+                  double fact, double dt, double lam1, double lams, double gam,
+                  const double *rho1, const double *q1, const double *E1, double *rho, double *q,
+                  double *E, double *Rmass, double *Rmom, double *Rener, const NextStage *next);
 // full_convec_ijk element records (ops_element.hip)
 template <typename T>
 int conv_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He, const T *gpvol,

@@ -817,6 +817,13 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 //   B3  line derivatives of u and rho handed over: wave 0 forms them for the
 //       convection terms, and wave 1 takes them for grad u and grad rho
 //       instead of forming the same sums again
+// values per residual record written by the two-wave kernel.  With FP32
+// operators the five residuals are padded to six, so that a record starts on
+// an 8-byte boundary and the gather reads it with three 8-byte loads (the
+// gather then takes 0.25 instead of 0.43 ms at 64^3 elements); in FP64 the
+// 20 % larger records cost the element kernel more than the gather gains.
+template <typename T> constexpr int split_rw() { return sizeof(T) == 4 ? 6 : 5; }
+
 template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
     static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
@@ -956,15 +963,17 @@ __device__ __forceinline__ void split_read(const unsigned char *sl, int a, int p
         if (visc) x.mue = ((const T *)(sl + L::GEO + L::ST))[a];
 }
 
-// residual records of element e (5 values per node, lexicographic node
-// order) from their dense copy in LDS, 16 contiguous bytes per lane
+// residual records of element e (split_rw values per node: the five
+// residuals, and a zero when padded; lexicographic node order) from their
+// dense copy in LDS, 16 contiguous bytes per lane
 template <int P, typename T>
 __device__ __forceinline__ void split_store(const RhsArgs<T> &A, const void *dense, int e, int lane)
 {
-    constexpr int NN = (P + 1) * (P + 1) * (P + 1), CH = 5 * NN * (int)sizeof(T) / 16;
+    constexpr int RW = split_rw<T>();
+    constexpr int NN = (P + 1) * (P + 1) * (P + 1), CH = RW * NN * (int)sizeof(T) / 16;
     using U4 = unsigned __attribute__((ext_vector_type(4)));
     const U4 *src = (const U4 *)dense;
-    U4 *o = (U4 *)(A.out + (size_t)e * 5 * NN);
+    U4 *o = (U4 *)(A.out + (size_t)e * RW * NN);
 #pragma unroll
     for (int q = lane; q < CH; q += 64) o[q] = src[q];
 }
@@ -1012,9 +1021,10 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     constexpr int NN = L::NN, REC = sizeof(T) == 8 ? 14 : 12, GREC = 18, AH = L::AH;
     // s2 holds, in turn: the viscous flux records (GREC values per node) and
     // the handed-over viscous residuals (3 pairs per node), then the dense
-    // residual records (5 values per node) followed by the line derivatives
+    // residual records (RW values per node) followed by the line derivatives
     // of u and rho (6 pairs per node) from DREC on
-    constexpr int DREC = 5 * NN / 2;
+    constexpr int RW = split_rw<T>();
+    constexpr int DREC = RW * NN / 2;
     static_assert(DREC + 6 * NN <= 64 * GREC / 2, "derivative records fit in s2");
     static_assert(NN == 64, "one lane per node");
     __shared__ V2 s1[64 * REC / 2]; // records of REC values as pairs
@@ -1139,15 +1149,22 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
         T pend[5] = {0, 0, 0, 0, 0};
         // its records: pend plus the viscous residuals handed over in s2, as
         // dense records in lexicographic node order over the same LDS (all
-        // lanes read before any writes); the viscous wave stores them to HBM
+        // lanes read before any writes; a padded record is exactly the slot
+        // the lane read its handover from); the viscous wave stores them
         auto finish = [&]() {
             const V2 h01 = s2[3 * pa], h23 = s2[3 * pa + 1], h4 = s2[3 * pa + 2];
-            T *d = (T *)s2 + 5 * pa;
-            d[0] = pend[0] + h01.x;
-            d[1] = pend[1] + h01.y;
-            d[2] = pend[2] + h23.x;
-            d[3] = pend[3] + h23.y;
-            d[4] = pend[4] + h4.x;
+            if constexpr (RW == 6) {
+                s2[3 * pa] = V2{pend[0] + h01.x, pend[1] + h01.y};
+                s2[3 * pa + 1] = V2{pend[2] + h23.x, pend[3] + h23.y};
+                s2[3 * pa + 2] = V2{pend[4] + h4.x, T(0)};
+            } else {
+                T *d = (T *)s2 + 5 * pa;
+                d[0] = pend[0] + h01.x;
+                d[1] = pend[1] + h01.y;
+                d[2] = pend[2] + h23.x;
+                d[3] = pend[3] + h23.y;
+                d[4] = pend[4] + h4.x;
+            }
         };
         for (int t = 0; t < nit; ++t) {
             wait_vm<(AH - 1) * L::NOPS0>();
@@ -1643,6 +1660,17 @@ static int full_diffusion(sodhip_ctx ctx, int porder, int nelem, int npoin, cons
 
 // Fused RHS used by the stage driver (ops_stage.hip): convection + viscous
 // operator from the conserved FP64 state; writes element partials only.
+// values per record of rhs_fused_elements' output (operator precision 8 or 4)
+int fused_record_width(int porder, int op_bytes)
+{
+#if SODHIP_SPLIT_RHS
+    if (porder == 3) return op_bytes == 4 ? split_rw<float>() : split_rw<double>();
+#endif
+    (void)porder;
+    (void)op_bytes;
+    return 5;
+}
+
 template <typename T>
 int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He,
                        const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,

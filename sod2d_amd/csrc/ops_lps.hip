@@ -626,7 +626,7 @@ extern "C" int sodhip_rk_stage_lps_f64(
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
     const NextStage nx{1.0, gam_next * dt, rho_next, q_next, E_next};
-    return gather_update<double>(ctx, m, npoin, npoin_w, lpoin_w, rec, EN, invMl, prm->fact, dt,
+    return gather_update<double>(ctx, m, npoin, npoin_w, lpoin_w, rec, 5, EN, invMl, prm->fact, dt,
                                  lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
                                  rho_next ? &nx : nullptr);
 }

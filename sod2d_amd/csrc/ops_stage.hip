@@ -26,6 +26,7 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const int *gK, const double *rho, const double *q, const double *E,
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
                        double mu_factor, double c_rho, double c_ener, double p_ref, T *out);
+int fused_record_width(int porder, int op_bytes);
 
 // RD1 / RDY: the base state y1 / the current state y enters the combine (its
 // coefficient lam1 / lams is non-zero); otherwise the term is 0 * 0, the same
@@ -34,7 +35,34 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
 // reference sequencing, yn = nl1 * y + 0 * 0 + ngdt * R (the stage update with
 // lam1 = 1, lams = 0 and the kept residual, time_integ_ls.f90:323-331), which
 // is stored to rhoN/qN/EN.
-template <typename TS, bool RD1, bool RDY, bool NX>
+// RW: values per record, 5 or 6 (five residuals and a pad; the record then
+// starts on a 2*sizeof(TS) boundary and is read with three 2-wide loads).
+template <typename TS> struct Pair2;
+template <> struct Pair2<double> {
+    using V = double2;
+};
+template <> struct Pair2<float> {
+    using V = float2;
+};
+template <typename TS, int RW>
+__device__ __forceinline__ void load_record(const TS *__restrict__ scr, int sj, double v[5])
+{
+    if constexpr (RW == 6) {
+        using V = typename Pair2<TS>::V;
+        const V *r = (const V *)(scr + 6 * (size_t)sj);
+        const V a = r[0], b = r[1], c = r[2];
+        v[0] = (double)a.x;
+        v[1] = (double)a.y;
+        v[2] = (double)b.x;
+        v[3] = (double)b.y;
+        v[4] = (double)c.x;
+    } else {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = (double)scr[5 * (size_t)sj + f];
+    }
+}
+
+template <typename TS, bool RD1, bool RDY, bool NX, int RW>
 __global__ __launch_bounds__(256) void k_gather_update(
     int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
     const int *__restrict__ wsrc, const TS *__restrict__ scr, size_t EN,
@@ -70,9 +98,10 @@ __global__ __launch_bounds__(256) void k_gather_update(
         for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
         double v[4][5];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? (double)scr[5 * (size_t)sj[u] + f] : 0.0;
+        for (int u = 0; u < 4; ++u) {
+            // an absent slot reads record 0 (always present) and is not added
+            load_record<TS, RW>(scr, max(sj[u], 0), v[u]);
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (sj[u] >= 0) {
@@ -113,18 +142,24 @@ __global__ __launch_bounds__(256) void k_gather_update(
 // element-node, type TS) over the working incidence of m
 template <typename TS>
 int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
-                  const int *lpoin_w, const TS *scr, size_t EN, const double *invMl, double fact,
-                  double dt, double lam1, double lams, double gam, const double *rho1,
-                  const double *q1, const double *E1, double *rho, double *q, double *E,
-                  double *Rmass, double *Rmom, double *Rener, const NextStage *next)
+                  const int *lpoin_w, const TS *scr, int rw, size_t EN, const double *invMl,
+                  double fact, double dt, double lam1, double lams, double gam,
+                  const double *rho1, const double *q1, const double *E1, double *rho, double *q,
+                  double *E, double *Rmass, double *Rmom, double *Rener, const NextStage *next)
 {
+    if (rw != 5 && rw != 6) return set_error(SODHIP_ERR_ARG, "record width %d", rw);
     int tok = prof_begin(ctx, 1);
     const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
     const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
-#define SODHIP_GU2(R1, RY, NX)                                                                 \
-    k_gather_update<TS, R1, RY, NX><<<grid, 256, 0, ctx->stream>>>(                            \
+#define SODHIP_GU3(R1, RY, NX, RWV)                                                            \
+    k_gather_update<TS, R1, RY, NX, RWV><<<grid, 256, 0, ctx->stream>>>(                       \
         npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, fact, lam1, lams, gam * dt,  \
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
+#define SODHIP_GU2(R1, RY, NX)                                                                 \
+    if (rw == 6)                                                                               \
+        SODHIP_GU3(R1, RY, NX, 6);                                                             \
+    else                                                                                       \
+        SODHIP_GU3(R1, RY, NX, 5)
 #define SODHIP_GU(R1, RY)                                                                      \
     if (next)                                                                                  \
         SODHIP_GU2(R1, RY, true);                                                              \
@@ -140,15 +175,16 @@ int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
         SODHIP_GU(false, false);
 #undef SODHIP_GU
 #undef SODHIP_GU2
+#undef SODHIP_GU3
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
 }
 template int gather_update<double>(sodhip_ctx, const MeshCache *, int, int, const int *,
-                                   const double *, size_t, const double *, double, double, double,
-                                   double, double, const double *, const double *, const double *,
-                                   double *, double *, double *, double *, double *, double *,
-                                   const NextStage *);
+                                   const double *, int, size_t, const double *, double, double,
+                                   double, double, double, const double *, const double *,
+                                   const double *, double *, double *, double *, double *,
+                                   double *, double *, const NextStage *);
 
 template <typename T>
 static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
@@ -164,15 +200,16 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     if (rc) return rc;
     if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
     const size_t EN = (size_t)nelem * nnode_of(porder);
-    if ((rc = ensure_buf(ctx->scratch, 5 * EN * sizeof(T)))) return rc;
+    const int rw = fused_record_width(porder, (int)sizeof(T));
+    if ((rc = ensure_buf(ctx->scratch, rw * EN * sizeof(T)))) return rc;
     T *scr = (T *)ctx->scratch.ptr;
     rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
                                prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
                                prm->c_rho, prm->c_ener, prm->p_ref, scr);
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
-    return gather_update<T>(ctx, m, npoin, npoin_w, lpoin_w, scr, EN, invMl, prm->fact, dt, lam1,
-                            lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
+    return gather_update<T>(ctx, m, npoin, npoin_w, lpoin_w, scr, rw, EN, invMl, prm->fact, dt,
+                            lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
 }
 
 // ---------------------------------------------------------------- CFL time step
