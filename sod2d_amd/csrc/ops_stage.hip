@@ -36,7 +36,8 @@ int fused_record_width(int porder, int op_bytes);
 // lam1 = 1, lams = 0 and the kept residual, time_integ_ls.f90:323-331), which
 // is stored to rhoN/qN/EN.
 // RW: values per record, 5 or 6 (five residuals and a pad; the record then
-// starts on a 2*sizeof(TS) boundary and is read with three 2-wide loads).
+// starts on a 2*sizeof(TS) boundary and is read with three 2-wide loads; an
+// FP64 record of 5 is read with three aligned loads as well).
 template <typename TS> struct Pair2;
 template <> struct Pair2<double> {
     using V = double2;
@@ -56,6 +57,20 @@ __device__ __forceinline__ void load_record(const TS *__restrict__ scr, int sj, 
         v[2] = (double)b.x;
         v[3] = (double)b.y;
         v[4] = (double)c.x;
+    } else if constexpr (sizeof(TS) == 8) {
+        // a 40-byte record starts on a 16-byte boundary (even sj) or 8 bytes
+        // past one (odd sj): two aligned 16-byte loads and one 8-byte load
+        // cover it either way, at offsets chosen per lane
+        const double *r = (const double *)scr + 5 * (size_t)sj;
+        const int odd = sj & 1;
+        const double2 x = *(const double2 *)(r + odd);
+        const double2 y = *(const double2 *)(r + 2 + odd);
+        const double z = r[odd ? 0 : 4];
+        v[0] = odd ? z : x.x;
+        v[1] = odd ? x.x : x.y;
+        v[2] = odd ? x.y : y.x;
+        v[3] = odd ? y.x : y.y;
+        v[4] = odd ? y.y : z;
     } else {
 #pragma unroll
         for (int f = 0; f < 5; ++f) v[f] = (double)scr[5 * (size_t)sj + f];
