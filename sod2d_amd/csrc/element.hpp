@@ -63,4 +63,21 @@ __device__ __forceinline__ ElemRange elem_range(int nelem)
     return {e0 + r, e1, per};
 }
 
+// Loads the 5-value FP64 record sj (40 bytes at 40*sj): it starts on a 16-byte
+// boundary (even sj) or 8 bytes past one (odd sj), so two aligned 16-byte
+// loads and one 8-byte load at per-lane offsets cover it either way.
+__device__ __forceinline__ void load_rec5(const double *__restrict__ scr, int sj, double v[5])
+{
+    const double *r = scr + 5 * (size_t)sj;
+    const int odd = sj & 1;
+    const double2 x = *(const double2 *)(r + odd);
+    const double2 y = *(const double2 *)(r + 2 + odd);
+    const double z = r[odd ? 0 : 4];
+    v[0] = odd ? z : x.x;
+    v[1] = odd ? x.x : x.y;
+    v[2] = odd ? x.y : y.x;
+    v[3] = odd ? y.x : y.y;
+    v[4] = odd ? y.y : z;
+}
+
 } // namespace sodhip

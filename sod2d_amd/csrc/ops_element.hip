@@ -769,10 +769,19 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
         for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
         TR v[4][NF];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+            if constexpr (NF == 5 && sizeof(TS) == 8) {
+                // an absent slot reads record 0 (always present) and is not added
+                double w[5];
+                load_rec5((const double *)scr, max(sj[u], 0), w);
 #pragma unroll
-            for (int f = 0; f < NF; ++f)
-                v[u][f] = sj[u] >= 0 ? TR(scr[(size_t)sj[u] * NF + f]) : TR(0);
+                for (int f = 0; f < NF; ++f) v[u][f] = TR(w[f]);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    v[u][f] = sj[u] >= 0 ? TR(scr[(size_t)sj[u] * NF + f]) : TR(0);
+            }
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (sj[u] >= 0) {

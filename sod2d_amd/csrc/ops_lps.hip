@@ -293,9 +293,7 @@ __global__ __launch_bounds__(256) void k_gather5(int npoin, const int *__restric
         for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
         double v[4][5];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int f = 0; f < 5; ++f) v[u][f] = sj[u] >= 0 ? scr[5 * (size_t)sj[u] + f] : 0.0;
+        for (int u = 0; u < 4; ++u) load_rec5(scr, max(sj[u], 0), v[u]); // absent: not added
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (sj[u] >= 0) {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "element.hpp"
 
 namespace sodhip {
 
@@ -58,19 +59,7 @@ __device__ __forceinline__ void load_record(const TS *__restrict__ scr, int sj, 
         v[3] = (double)b.y;
         v[4] = (double)c.x;
     } else if constexpr (sizeof(TS) == 8) {
-        // a 40-byte record starts on a 16-byte boundary (even sj) or 8 bytes
-        // past one (odd sj): two aligned 16-byte loads and one 8-byte load
-        // cover it either way, at offsets chosen per lane
-        const double *r = (const double *)scr + 5 * (size_t)sj;
-        const int odd = sj & 1;
-        const double2 x = *(const double2 *)(r + odd);
-        const double2 y = *(const double2 *)(r + 2 + odd);
-        const double z = r[odd ? 0 : 4];
-        v[0] = odd ? z : x.x;
-        v[1] = odd ? x.x : x.y;
-        v[2] = odd ? x.y : y.x;
-        v[3] = odd ? y.x : y.y;
-        v[4] = odd ? y.y : z;
+        load_rec5((const double *)scr, sj, v);
     } else {
 #pragma unroll
         for (int f = 0; f < 5; ++f) v[f] = (double)scr[5 * (size_t)sj + f];
