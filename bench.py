@@ -205,6 +205,9 @@ def main(argv=None):
                          "stabilisation and the live sensor path (smart_visc_spectral_imex + "
                          "Ducros limiter), through the drop-in operators (FP64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the steps as replays of a HIP graph of two captured steps "
+                         "(--steps rounded up to even)")
     ap.add_argument("--workload", choices=("tgv", "operators"), default="tgv",
                     help="tgv: the TGV time step (default); operators: the isolated "
                          "full_convec_ijk / full_diffusion_ijk / smart_visc_spectral calls")
@@ -238,12 +241,21 @@ def main(argv=None):
                  dt_mode="fixed", lps=a.lps, sensor_path="live" if a.lps else "spectral")
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
-    for _ in range(a.warmup):
+    for _ in range(max(a.warmup, 1 if a.graph else 0)):
         run.step()
     torch.cuda.synchronize()
+    if a.graph:
+        graph = run.capture()
+        a.steps += a.steps % 2
+        graph.replay()  # the first replay uploads the graph
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        run.step()
+    if a.graph:
+        for _ in range(a.steps // 2):
+            graph.replay()
+    else:
+        for _ in range(a.steps):
+            run.step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     # per-kernel breakdown from further steps bracketed by HIP events on the
@@ -305,6 +317,7 @@ def main(argv=None):
         "dtype": "f64" if a.precision == 8 else "f32-operators/f64-state",
         "data": "synthetic",
         "config": {
+            "graph": bool(a.graph),
             "workload": f"TGV p={a.porder} {a.n}^3 elements, periodic, Re=1600 Ma=0.1, "
                         f"SSP(5,3)-2N rk_4_ls_main {'reference' if not a.clean else 'fresh'}"
                         f" sequencing, "

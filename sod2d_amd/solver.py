@@ -390,6 +390,31 @@ class TGVRun:
         return dict(time=self.time, EK=ek.value, eps_S=eS.value, eps_D=eD.value, eps_T=eT.value,
                     maxmach=mm.value)
 
+    def capture(self) -> "StepGraph":
+        """Captures two time steps into a HIP graph (torch.cuda.CUDAGraph on the
+        library's stream) and returns it; StepGraph.replay() then advances the run
+        by two steps with one graph launch.  Two, because the time levels are
+        exchanged every step: after two steps every array is back in its place.
+        Requires fixed dt (the adaptive time step reads its reduction back to the
+        host) and a first step already taken (it builds the library's mesh tables
+        and scratch, which allocate).  Nothing in the captured steps allocates,
+        synchronises or clears memory with a memset."""
+        if self.dt_mode != "fixed":
+            raise SodhipError("graph capture needs dt_mode='fixed'")
+        if self.first:
+            raise SodhipError("take one step before capturing (set-up allocates)")
+        saved = (self.time, self.istep)
+        self.ctx.prof_enable(False)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.ctx.sync_stream()  # launches go to the capturing stream
+            self.step()
+            self.step()
+        self.ctx.sync_stream()
+        self.time, self.istep = saved
+        return StepGraph(self, g)
+
     def state(self) -> dict:
         n = self.npoin
         return dict(rho=self.rho[1].cpu().numpy(),
@@ -401,6 +426,21 @@ class TGVRun:
     def rhs_per_step(self) -> int:
         """Fresh RHS evaluations per step (after the first step)."""
         return self.S - 1 if self.parity else self.S
+
+
+class StepGraph:
+    """Two captured time steps of a TGVRun (see TGVRun.capture)."""
+
+    def __init__(self, run: TGVRun, graph):
+        self.run = run
+        self.graph = graph
+
+    def replay(self):
+        """Advances the run by two steps."""
+        self.run.ctx.sync_stream()
+        self.graph.replay()
+        self.run.time += 2 * self.run.dt
+        self.run.istep += 2
 
 
 def rk_ls_coeffs(stages: int, variant: int = 1):
