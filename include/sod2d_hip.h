@@ -341,6 +341,19 @@ int sodhip_thermo_entropy_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int 
 int sodhip_entropy_residual_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
                                 double factor_comp, double dt, double *Reta, const double *eta,
                                 double *auxReta);
+/* The node updates at the end of rk_4_ls_main in one pass over the working
+ * nodes: sodhip_eos_step_f64 (time_integ_ls.f90:391-407), then
+ * sodhip_thermo_entropy_f64 with feta = NULL, then sutherland_viscosity
+ * (mod_fluid_viscosity.f90:21-41) with the uniform mu_factor.  Each output
+ * has the bits those three calls give on the working nodes; mu_fluid is not
+ * written elsewhere (the reference's loop covers all npoin, where nodes outside
+ * lpoin_w keep their temperature).  No counterpart in the reference. */
+int sodhip_eos_entropy_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                double gamma_gas, double Rgas, double p_inf, double rho_inf,
+                                double mu_factor, const double *rho, const double *q,
+                                const double *E, double *u, double *e_int, double *pr,
+                                double *csound, double *machno, double *Tem, double *eta,
+                                double *mu_fluid);
 
 /* ------------------------------------------------------------------------ */
 /* Fused Runge-Kutta stage                                                   */

@@ -193,6 +193,45 @@ __global__ __launch_bounds__(256) void k_eos_step(int npoin, int npoin_w,
     Tem[n] = p / (r * Rgas);
 }
 
+// end of a step on the working nodes: the EOS of k_eos_step, then
+// thermo_entropy (eta(:,1) <- eta(:,2), eta(:,2) from the new rho and p) and
+// sutherland_viscosity from the new T, each with the arithmetic of its own
+// kernel, so every output has the same bits as the three kernels give
+__global__ __launch_bounds__(256) void k_end_state(
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, double g, double Rgas, double p_inf,
+    double rho_inf, double mu_factor, const double *__restrict__ rho,
+    const double *__restrict__ q, const double *__restrict__ E, double *__restrict__ u,
+    double *__restrict__ e_int, double *__restrict__ pr, double *__restrict__ csound,
+    double *__restrict__ machno, double *__restrict__ Tem, double *__restrict__ eta,
+    double *__restrict__ mu_fluid)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const double r = rho[n];
+    const double e2 = eta[n + (size_t)npoin];
+    double umag = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double ud = q[n + (size_t)npoin * d] / r;
+        u[n + (size_t)npoin * d] = ud;
+        umag += ud * ud;
+    }
+    const double ei = E[n] / r - 0.5 * umag;
+    const double p = r * (g - 1.0) * ei;
+    e_int[n] = ei;
+    pr[n] = p;
+    const double c = sqrt(g * p / r);
+    csound[n] = c;
+    machno[n] = sqrt(umag) / c;
+    const double T = p / (r * Rgas);
+    Tem[n] = T;
+    eta[n] = e2;
+    eta[n + (size_t)npoin] =
+        (r / (g - 1.0)) * log(fmax(p / p_inf, 2.220446049250313e-16) / pow(r / rho_inf, g));
+    mu_fluid[n] = mu_factor * 0.000001458 * pow(T, 1.5) / (T + 110.4);
+}
+
 // sutherland_viscosity (mod_fluid_viscosity.f90:21-41), over all npoin
 __global__ __launch_bounds__(256) void k_sutherland(int npoin, const double *__restrict__ Tem,
                                                     const double *__restrict__ mu_factor,
@@ -282,6 +321,25 @@ extern "C" int sodhip_eos_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     SODHIP_NODE_LAUNCH(3, npoin_w, k_eos_step, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, rho, q, E,
                        u, e_int, pr, csound, machno, Tem);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_eos_entropy_step_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                           const int *lpoin_w, double gamma_gas, double Rgas,
+                                           double p_inf, double rho_inf, double mu_factor,
+                                           const double *rho, const double *q, const double *E,
+                                           double *u, double *e_int, double *pr, double *csound,
+                                           double *machno, double *Tem, double *eta,
+                                           double *mu_fluid)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && q && E && u && e_int && pr && csound && machno && Tem &&
+                       eta && mu_fluid,
+                   "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_end_state, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, p_inf,
+                       rho_inf, mu_factor, rho, q, E, u, e_int, pr, csound, machno, Tem, eta,
+                       mu_fluid);
     return SODHIP_OK;
 }
 

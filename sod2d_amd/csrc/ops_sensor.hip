@@ -56,8 +56,21 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int 
     __shared__ double sh[kRedThreads];
     const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
     const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    // four strided values are loaded ahead and added in the order of the
+    // plain strided loop, so the partial sums do not depend on the unrolling
+    const int bs = blockDim.x;
     double s = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += eta[lpoin_w[i] - 1];
+    int i = i0 + threadIdx.x;
+    for (; i + 3 * bs < i1; i += 4 * bs) {
+        const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
+                  n3 = lpoin_w[i + 3 * bs];
+        const double v0 = eta[n0 - 1], v1 = eta[n1 - 1], v2 = eta[n2 - 1], v3 = eta[n3 - 1];
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
+    }
+    for (; i < i1; i += bs) s += eta[lpoin_w[i] - 1];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -76,9 +89,17 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
     const double mean = block_sum(s, sh) / (double)npoin_w;
     const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
     const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    const int bs = blockDim.x;
     double m = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
-        m = fmax(m, fabs(eta[lpoin_w[i] - 1] - mean));
+    int i = i0 + threadIdx.x;
+    for (; i + 3 * bs < i1; i += 4 * bs) {
+        const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
+                  n3 = lpoin_w[i + 3 * bs];
+        const double v0 = eta[n0 - 1], v1 = eta[n1 - 1], v2 = eta[n2 - 1], v3 = eta[n3 - 1];
+        m = fmax(fmax(m, fabs(v0 - mean)), fabs(v1 - mean));
+        m = fmax(fmax(m, fabs(v2 - mean)), fabs(v3 - mean));
+    }
+    for (; i < i1; i += bs) m = fmax(m, fabs(eta[lpoin_w[i] - 1] - mean));
     m = block_max(m, sh);
     if (threadIdx.x == 0) pmax[blockIdx.x] = m;
 }

@@ -189,6 +189,8 @@ class TGVRun:
         self.first = True
         self._next_dt = None  # dt with which the next step's stage 1 was stored
         self.fuse_next_stage = True  # at fixed dt, see the module docstring
+        # EOS, entropy and Sutherland of the end of a step in one node pass
+        self.fuse_end_state = True
         self.time = 0.0
         self.istep = 0
         # FP32 operators: the uniform reference pressure is removed before the
@@ -354,13 +356,24 @@ class TGVRun:
         """EOS, entropy, entropy residual, sensor and Sutherland after the stages
         (time_integ_ls.f90:389-473)."""
         h, n, t = self.ctx.h, self.npoin, self.t
-        check(lib().sodhip_eos_step_f64(
-            h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas, _ptr(self.rho[1]),
-            _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.u[1]), _ptr(self.e_int[1]),
-            _ptr(self.pr[1]), _ptr(self.csound), _ptr(self.machno), _ptr(self.Tem[1])), "eos_step")
-        check(lib().sodhip_thermo_entropy_f64(
-            h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.po, t.rho0, _ptr(self.rho[1]),
-            _ptr(self.pr[1]), None, _ptr(self.eta), None), "thermo_entropy")
+        if self.fuse_end_state:
+            # Sutherland moves ahead of the sensor, which does not read mu_fluid
+            check(lib().sodhip_eos_entropy_step_f64(
+                h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas, t.po, t.rho0,
+                t.mu_factor, _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]),
+                _ptr(self.u[1]), _ptr(self.e_int[1]), _ptr(self.pr[1]), _ptr(self.csound),
+                _ptr(self.machno), _ptr(self.Tem[1]), _ptr(self.eta), _ptr(self.mu_fluid)),
+                "eos_entropy_step")
+        else:
+            check(lib().sodhip_eos_step_f64(
+                h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas, _ptr(self.rho[1]),
+                _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.u[1]), _ptr(self.e_int[1]),
+                _ptr(self.pr[1]), _ptr(self.csound), _ptr(self.machno), _ptr(self.Tem[1])),
+                "eos_step")
+            check(lib().sodhip_thermo_entropy_f64(
+                h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.po, t.rho0,
+                _ptr(self.rho[1]), _ptr(self.pr[1]), None, _ptr(self.eta), None),
+                "thermo_entropy")
         self._scalar_convec(self.Reta[n:])
         check(lib().sodhip_entropy_residual_f64(
             h, n, self.npoin_w, _ptr(self.lpoin_w), 1.0, self.dt, _ptr(self.Reta), _ptr(self.eta),
@@ -368,7 +381,8 @@ class TGVRun:
         if self.sensor:
             self._smart_visc(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[n:],
                              ducros=True)
-        self._sutherland()
+        if not self.fuse_end_state:
+            self._sutherland()
 
     # ------------------------------------------------------------ diagnostics
     def diagnostics(self) -> dict:

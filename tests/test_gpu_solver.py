@@ -103,6 +103,31 @@ def test_time_loop_fixed_dt_next_stage(oracle, op_precision):
             assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
 
 
+@pytest.mark.parametrize("sensor_path", ["spectral", "live"])
+def test_fused_end_of_step_same_bits(oracle, sensor_path):
+    """sodhip_eos_entropy_step_f64 (EOS, entropy and Sutherland in one node
+    pass) gives the bits of the separate sodhip_eos_step_f64,
+    sodhip_thermo_entropy_f64 and sodhip_sutherland_f64 calls: over five
+    adaptive-dt steps every array they write, the state, mu_e and dt agree
+    bitwise."""
+    dev, _ = _setup(oracle, 3, 4, sensor_path=sensor_path)
+    ref, _ = _setup(oracle, 3, 4, sensor_path=sensor_path)
+    ref.fuse_end_state = False
+    for step in range(5):
+        dev.step()
+        ref.step()
+        assert dev.dt == ref.dt, step
+        for k in ("u", "e_int", "pr", "Tem"):
+            assert np.array_equal(getattr(dev, k)[1].cpu().numpy(),
+                                  getattr(ref, k)[1].cpu().numpy()), (step, k)
+        for k in ("csound", "machno", "eta", "mu_fluid", "mu_e", "Reta"):
+            assert np.array_equal(getattr(dev, k).cpu().numpy(),
+                                  getattr(ref, k).cpu().numpy()), (step, k)
+        s, r = dev.state(), ref.state()
+        for k in ("rho", "q", "E"):
+            assert np.array_equal(s[k], r[k]), (step, k)
+
+
 @pytest.mark.parametrize("lps", [False, True])
 def test_graph_captured_steps(oracle, lps):
     """Two steps captured in a HIP graph and replayed give the same bits as the
