@@ -354,6 +354,22 @@ int sodhip_eos_entropy_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const in
                                 const double *E, double *u, double *e_int, double *pr,
                                 double *csound, double *machno, double *Tem, double *eta,
                                 double *mu_fluid);
+/* The entropy residual of rk_4_ls_main (time_integ_ls.f90:429-443) in one
+ * call: sodhip_generic_scalar_convec_ijk_f64 with q = NULL (f = u * eta(:,1)),
+ * initialize = 1 and fact = 1 into Reta(:,2), sodhip_lumped_solver_scal_f64 on
+ * it, then sodhip_entropy_residual_f64 (auxReta, Reta(:,1) <- Reta(:,2)), the
+ * last two on the gathered values of each working node.  Reta and eta are
+ * (npoin,2).  Each output has the bits of the three calls on the working
+ * nodes; Reta(:,2) is not written elsewhere (the separate gather writes 0 at
+ * nodes no element references).  No counterpart in the reference. */
+int sodhip_entropy_convec_residual_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                       int npoin_w, const int *connec, const int *lpoin_w,
+                                       const double *He, const double *gpvol,
+                                       const double *dlxigp_ip, const int *invAtoIJK,
+                                       const int *gmshAtoI, const int *gmshAtoJ,
+                                       const int *gmshAtoK, const double *u, const double *eta,
+                                       const double *invMl, double factor_comp, double dt,
+                                       double *Reta, double *auxReta);
 
 /* ------------------------------------------------------------------------ */
 /* Fused Runge-Kutta stage                                                   */

@@ -77,6 +77,7 @@ _SIGS = {
     "sodhip_sutherland_f64": (I, [P, I, P, P, D, P]),
     "sodhip_thermo_entropy_f64": (I, [P, I, I, P, D, D, D] + [P] * 5),
     "sodhip_eos_entropy_step_f64": (I, [P, I, I, P, D, D, D, D, D] + [P] * 11),
+    "sodhip_entropy_convec_residual_f64": (I, [P, I, I, I, I] + [P] * 12 + [D, D, P, P]),
     "sodhip_entropy_residual_f64": (I, [P, I, I, P, D, D, P, P, P]),
     "sodhip_rk_stage_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                             + [C.POINTER(StageParams), D, D, D, D] + [P] * 9),

@@ -793,6 +793,48 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
     for (int f = 0; f < NF; ++f) R[f][n] = acc[f];
 }
 
+// Tail of the entropy residual on the scalar gather (working nodes): the
+// slots of node n summed as k_gather sums them, times invMl (lumped_solver_scal),
+// then the residual of k_entropy_residual; Reta and eta are (npoin,2)
+struct EntropyTail {
+    int npoin_w;
+    const int *lpoin_w;
+    const double *invMl, *eta;
+    double factor_comp, dt;
+    double *Reta, *auxReta;
+};
+
+__global__ __launch_bounds__(256) void k_gather_entropy(int npoin, const int *__restrict__ wrow,
+                                                        const int *__restrict__ wsrc,
+                                                        const double *__restrict__ scr,
+                                                        EntropyTail t, double fact)
+{
+    const int per = gridDim.x >> 3; // grid is a multiple of 8
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = blk * blockDim.x + threadIdx.x;
+    if (i >= t.npoin_w) return;
+    const int n = t.lpoin_w[i] - 1;
+    const int j0 = wrow[i], j1 = wrow[i + 1];
+    double acc = 0.0;
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? wsrc[j + u] : -1;
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = sj[u] >= 0 ? scr[sj[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) acc += fact * v[u];
+    }
+    const double r2 = acc * t.invMl[n];
+    const double r1 = t.Reta[n];
+    t.Reta[n + (size_t)npoin] = r2;
+    t.auxReta[n] = (1.5 * r2 - 0.5 * r1) +
+                   t.factor_comp * (t.eta[n + (size_t)npoin] - t.eta[n]) / t.dt;
+    t.Reta[n] = r2;
+}
+
 // ---------------------------------------------------------------- launch helpers
 // ---------------------------------------------------------------------------
 // Fused-stage element kernel with the work of each element split over two
@@ -1756,11 +1798,11 @@ template <typename T>
 static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,
                          const T *He, const T *gpvol, const T *dl, const int *inv, const int *gI,
                          const int *gJ, const int *gK, const T *q, const T *eta, const T *u,
-                         T *Rconvec, int initialize, T fact)
+                         T *Rconvec, int initialize, T fact, const EntropyTail *tail = nullptr)
 {
     int rc = check_common<T>(ctx, porder, nelem, npoin, connec);
     if (rc) return rc;
-    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && eta && u && Rconvec,
+    SODHIP_REQUIRE(He && gpvol && dl && inv && gI && gJ && gK && eta && u && (Rconvec || tail),
                    "null array argument");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     MeshCache *m = nullptr;
@@ -1797,6 +1839,21 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
 #undef SODHIP_SC
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
+    if (tail) {
+        if constexpr (std::is_same<T, double>::value) {
+            if ((rc = get_working(ctx, m, tail->npoin_w, tail->lpoin_w))) return rc;
+            const int bs = 256;
+            const int grid = ((tail->npoin_w + bs - 1) / bs + 7) / 8 * 8;
+            tok = prof_begin(ctx, 3);
+            if (tail->npoin_w > 0)
+                k_gather_entropy<<<grid, bs, 0, ctx->stream>>>(npoin, m->wrow, m->wsrc, A.out,
+                                                               *tail, fact);
+            prof_end(ctx, tok);
+            SODHIP_TRY_HIP(hipGetLastError());
+            return SODHIP_OK;
+        }
+        return set_error(SODHIP_ERR_UNSUPPORTED, "entropy residual in FP64 only");
+    }
     return launch_gather<T, T>(ctx, m, A.out, EN, 1, Rconvec, nullptr, nullptr, nullptr, nullptr,
                                initialize, fact);
 }
@@ -1939,6 +1996,21 @@ extern "C" int sodhip_generic_scalar_convec_ijk_f64(
     return scalar_convec<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
                                  invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, q, eta, u, Rconvec,
                                  initialize, fact);
+}
+
+extern "C" int sodhip_entropy_convec_residual_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, const double *He, const double *gpvol, const double *dlxigp_ip,
+    const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+    const double *u, const double *eta, const double *invMl, double factor_comp, double dt,
+    double *Reta, double *auxReta)
+{
+    SODHIP_REQUIRE(lpoin_w && eta && invMl && Reta && auxReta, "null array argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    EntropyTail t{npoin_w, lpoin_w, invMl, eta, factor_comp, dt, Reta, auxReta};
+    return scalar_convec<double>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                 invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, nullptr, eta, u,
+                                 nullptr, 1, 1.0, &t);
 }
 
 extern "C" int sodhip_eval_divergence_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,

@@ -189,7 +189,8 @@ class TGVRun:
         self.first = True
         self._next_dt = None  # dt with which the next step's stage 1 was stored
         self.fuse_next_stage = True  # at fixed dt, see the module docstring
-        # EOS, entropy and Sutherland of the end of a step in one node pass
+        # EOS, entropy and Sutherland of the end of a step in one node pass,
+        # and the entropy residual's gather, lumped solve and update in another
         self.fuse_end_state = True
         self.time = 0.0
         self.istep = 0
@@ -374,10 +375,19 @@ class TGVRun:
                 h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.po, t.rho0,
                 _ptr(self.rho[1]), _ptr(self.pr[1]), None, _ptr(self.eta), None),
                 "thermo_entropy")
-        self._scalar_convec(self.Reta[n:])
-        check(lib().sodhip_entropy_residual_f64(
-            h, n, self.npoin_w, _ptr(self.lpoin_w), 1.0, self.dt, _ptr(self.Reta), _ptr(self.eta),
-            _ptr(self.auxReta)), "entropy_residual")
+        if self.fuse_end_state:
+            # Reta(:,2) of f_eta = u(:,:,1)*eta(:,1), its lumped solve and the
+            # residual in one element pass and one node pass
+            check(lib().sodhip_entropy_convec_residual_f64(
+                h, self.p, self.nelem, n, self.npoin_w, _ptr(self.connec), _ptr(self.lpoin_w),
+                _ptr(self.He), _ptr(self.gpvol), _ptr(self.dl), *[_ptr(x) for x in self.tabs],
+                _ptr(self.u[0]), _ptr(self.eta), _ptr(self.invMl), 1.0, self.dt, _ptr(self.Reta),
+                _ptr(self.auxReta)), "entropy_convec_residual")
+        else:
+            self._scalar_convec(self.Reta[n:])
+            check(lib().sodhip_entropy_residual_f64(
+                h, n, self.npoin_w, _ptr(self.lpoin_w), 1.0, self.dt, _ptr(self.Reta),
+                _ptr(self.eta), _ptr(self.auxReta)), "entropy_residual")
         if self.sensor:
             self._smart_visc(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[n:],
                              ducros=True)

@@ -107,9 +107,10 @@ def test_time_loop_fixed_dt_next_stage(oracle, op_precision):
 def test_fused_end_of_step_same_bits(oracle, sensor_path):
     """sodhip_eos_entropy_step_f64 (EOS, entropy and Sutherland in one node
     pass) gives the bits of the separate sodhip_eos_step_f64,
-    sodhip_thermo_entropy_f64 and sodhip_sutherland_f64 calls: over five
-    adaptive-dt steps every array they write, the state, mu_e and dt agree
-    bitwise."""
+    sodhip_thermo_entropy_f64 and sodhip_sutherland_f64 calls, and
+    sodhip_entropy_convec_residual_f64 those of the scalar convection, lumped
+    solve and entropy residual calls: over five adaptive-dt steps every array
+    they write, the state, mu_e and dt agree bitwise."""
     dev, _ = _setup(oracle, 3, 4, sensor_path=sensor_path)
     ref, _ = _setup(oracle, 3, 4, sensor_path=sensor_path)
     ref.fuse_end_state = False
@@ -120,7 +121,7 @@ def test_fused_end_of_step_same_bits(oracle, sensor_path):
         for k in ("u", "e_int", "pr", "Tem"):
             assert np.array_equal(getattr(dev, k)[1].cpu().numpy(),
                                   getattr(ref, k)[1].cpu().numpy()), (step, k)
-        for k in ("csound", "machno", "eta", "mu_fluid", "mu_e", "Reta"):
+        for k in ("csound", "machno", "eta", "mu_fluid", "mu_e", "Reta", "auxReta"):
             assert np.array_equal(getattr(dev, k).cpu().numpy(),
                                   getattr(ref, k).cpu().numpy()), (step, k)
         s, r = dev.state(), ref.state()
