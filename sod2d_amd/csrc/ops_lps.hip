@@ -64,6 +64,34 @@ __device__ __forceinline__ void lps_gradients(const LineTab<P, double> &lt, cons
             gU[i][j] = hp[j] * g[2 + i][0] + hp[j + 3] * g[2 + i][1] + hp[j + 6] * g[2 + i][2];
 }
 
+// Lane L of an element kernel evaluates the node at lexicographic position L
+// (I fastest) and the LDS fields are kept in that order, so the 16 lanes of
+// an LDS access phase read 16 distinct positions along every line.  Returns
+// the gmsh index of the lane's node and turns the line table's neighbours
+// into lexicographic positions; tmp holds NN ints (LDS).
+template <int P>
+__device__ __forceinline__ int lex_lane(const int *__restrict__ perm, int lane, bool act,
+                                        int *tmp)
+{
+    constexpr int NN = EG<P>::NN;
+    if (act) tmp[min(max(perm[lane], 0), NN - 1)] = lane;
+    lds_barrier();
+    const int a = act ? min(max(tmp[lane], 0), NN - 1) : 0;
+    lds_barrier();
+    return a;
+}
+
+template <int P>
+__device__ __forceinline__ void lex_neighbours(const int *__restrict__ perm,
+                                               LineTab<P, double> &lt)
+{
+    constexpr int NN = EG<P>::NN;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 0; o < P; ++o) lt.off[k][o] = min(max(perm[lt.off[k][o]], 0), NN - 1);
+}
+
 // full_proj_ijk's element part (elem_stab.f90:489-563): records of 12 values
 // per element-node, gpvol * (the upper triangle t11 t12 t13 t22 t23 t33 of the
 // symmetric tensor 2 S - 2/3 div u I, grad rho, grad T).  The tensor rows the
@@ -76,19 +104,20 @@ __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
     __shared__ double s[5 * NT];
     const int lane = threadIdx.x;
     const bool act = lane < NN;
-    const int a = act ? lane : 0;
+    const int a = lex_lane<P>(A.perm, lane, act, (int *)s);
+    const int pa = act ? lane : 0;
     LineTab<P, double> lt;
     build_linetab<P, double>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
-    const int pa = min(max(A.perm[a], 0), NN - 1);
+    lex_neighbours<P>(A.perm, lt);
     const ElemRange er = elem_range(A.nelem);
     for (int e = er.begin; e < er.end; e += er.stride) {
         const size_t ea = (size_t)e * NN + a;
         const int n = act ? A.connecT[ea] : 0;
         if (act) {
-            s[a] = A.rho[n];
-            s[NT + a] = A.Tem[n];
+            s[pa] = A.rho[n];
+            s[NT + pa] = A.Tem[n];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + a] = A.u[n + (size_t)A.npoin * d];
+            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + pa] = A.u[n + (size_t)A.npoin * d];
         }
         double hp[9];
 #pragma unroll
@@ -96,7 +125,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
         const double w = A.gpvol[ea];
         lds_barrier();
         double gR[3], gT[3], gU[3][3];
-        lps_gradients<P>(lt, s, a, hp, gR, gT, gU);
+        lps_gradients<P>(lt, s, pa, hp, gR, gT, gU);
         const double divU = gU[0][0] + gU[1][1] + gU[2][2];
         if (act) {
             double o[12];
@@ -137,10 +166,11 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
     __shared__ double G[15 * NT]; // flux f (mass, mom x, y, z, energy), direction k
     const int lane = threadIdx.x;
     const bool act = lane < NN;
-    const int a = act ? lane : 0;
+    const int a = lex_lane<P>(A.perm, lane, act, (int *)s);
+    const int pa = act ? lane : 0;
     LineTab<P, double> lt;
     build_linetab<P, double>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
-    const int pa = min(max(A.perm[a], 0), NN - 1);
+    lex_neighbours<P>(A.perm, lt);
     const double twoThirds = 2.0 / 3.0;
     const ElemRange er = elem_range(A.nelem);
     for (int e = er.begin; e < er.end; e += er.stride) {
@@ -151,10 +181,10 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
 #pragma unroll
         for (int d = 0; d < 3; ++d) ul[d] = A.u[n + np * d];
         if (act) {
-            s[a] = A.rho[n];
-            s[NT + a] = A.Tem[n];
+            s[pa] = A.rho[n];
+            s[NT + pa] = A.Tem[n];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + a] = ul[d];
+            for (int d = 0; d < 3; ++d) s[(2 + d) * NT + pa] = ul[d];
         }
         double hp[9];
 #pragma unroll
@@ -167,7 +197,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
         const double taus = A.tau[e];
         lds_barrier();
         double gR[3], gT[3], gU[3][3];
-        lps_gradients<P>(lt, s, a, hp, gR, gT, gU);
+        lps_gradients<P>(lt, s, pa, hp, gR, gT, gU);
         const double divU = gU[0][0] + gU[1][1] + gU[2][2];
         const double nu_e = A.c_rho * mue / rhon;
         const double mu_fgp = mufl + rhon * musgs;
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
             for (int f = 0; f < 5; ++f)
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    G[(3 * f + k) * NT + a] =
+                    G[(3 * f + k) * NT + pa] =
                         w * (hp[3 * k] * F[f][0] + hp[1 + 3 * k] * F[f][1] +
                              hp[2 + 3 * k] * F[f][2]);
         }
@@ -223,7 +253,7 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const double *gk = G + (3 * f + k) * NT;
-                acc += lt.ds[k] * gk[a];
+                acc += lt.ds[k] * gk[pa];
 #pragma unroll
                 for (int o = 0; o < P; ++o) acc += lt.t[k][o] * gk[lt.off[k][o]];
             }
