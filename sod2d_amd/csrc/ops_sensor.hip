@@ -195,19 +195,55 @@ __global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int npoin,
                                                 const double *__restrict__ helem, double dmin,
                                                 double *__restrict__ mu_e)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)nelem * nnode) return;
-    const int n = connec[t] - 1;
+    // Each thread takes KA consecutive element nodes of one element and issues
+    // all their loads before the arithmetic, so that a wave has KA gathers in
+    // flight per lane.  Threads are in tiles of 64 consecutive elements x all
+    // node groups, element fastest: a wave reads 64 consecutive entries of the
+    // element-fastest arrays, and the waves of one tile gather the nodes those
+    // elements share.
+    constexpr int KA = 4;
+    const int nag = (nnode + KA - 1) / KA;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tile = i / (64 * (size_t)nag);
+    const int r = (int)(i - tile * 64 * (size_t)nag);
+    const size_t e = tile * 64 + (r & 63);
+    if (e >= (size_t)nelem) return;
+    const int a0 = (r >> 6) * KA;
     const size_t P1 = (size_t)npoin, P2 = 2 * (size_t)npoin;
-    const double dv = divU[n] / Ml[n];
-    const double w0 = vorti[n], w1 = vorti[n + P1], w2 = vorti[n + P2];
-    const double v0 = u[n], v1 = u[n + P1], v2 = u[n + P2];
-    const double umag = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    const double d2 = dv * dv;
-    const double phi = fmin((4.0 / 3.0) * d2 / (d2 + (w0 * w0 + w1 * w1 + w2 * w2) + DBL_EPSILON), 1.0);
-    const double r = 1.0 * umag / helem[t];
-    const double phi2 = d2 / (d2 + r * r + DBL_EPSILON);
-    mu_e[t] = mu_e[t] * (fmax(phi - dmin, 0.0) * phi2 + dmin);
+    size_t t[KA];
+    int n[KA];
+#pragma unroll
+    for (int j = 0; j < KA; ++j) {
+        const int a = min(a0 + j, nnode - 1); // a group past nnode repeats the last node
+        t[j] = (size_t)a * nelem + e;
+        n[j] = connec[t[j]] - 1;
+    }
+    double dv[KA], w0[KA], w1[KA], w2[KA], v0[KA], v1[KA], v2[KA], he[KA], mu[KA];
+#pragma unroll
+    for (int j = 0; j < KA; ++j) {
+        dv[j] = divU[n[j]] / Ml[n[j]];
+        w0[j] = vorti[n[j]];
+        w1[j] = vorti[n[j] + P1];
+        w2[j] = vorti[n[j] + P2];
+        v0[j] = u[n[j]];
+        v1[j] = u[n[j] + P1];
+        v2[j] = u[n[j] + P2];
+        he[j] = helem[t[j]];
+        mu[j] = mu_e[t[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < KA; ++j) {
+        if (a0 + j >= nnode) break;
+        const double umag = sqrt(v0[j] * v0[j] + v1[j] * v1[j] + v2[j] * v2[j]);
+        const double d2 = dv[j] * dv[j];
+        const double phi = fmin((4.0 / 3.0) * d2 /
+                                    (d2 + (w0[j] * w0[j] + w1[j] * w1[j] + w2[j] * w2[j]) +
+                                     DBL_EPSILON),
+                                1.0);
+        const double rr = 1.0 * umag / he[j];
+        const double phi2 = d2 / (d2 + rr * rr + DBL_EPSILON);
+        mu_e[t[j]] = mu[j] * (fmax(phi - dmin, 0.0) * phi2 + dmin);
+    }
 }
 
 template <int P> struct SG {
@@ -503,7 +539,8 @@ extern "C" int sodhip_ducros_visc_limit_f64(sodhip_ctx ctx, int porder, int nele
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     const size_t EN = (size_t)nelem * (porder + 1) * (porder + 1) * (porder + 1);
     int tok = prof_begin(ctx, 2);
-    k_ducros<<<(unsigned)((EN + 255) / 256), 256, 0, ctx->stream>>>(
+    const size_t nthr = (((size_t)nelem + 63) / 64) * 64 * ((EN / nelem + 3) / 4);
+    k_ducros<<<(unsigned)((nthr + 255) / 256), 256, 0, ctx->stream>>>(
         nelem, (porder + 1) * (porder + 1) * (porder + 1), npoin, connec, u, divU, vorti, Ml, helem,
         ducros_min_val, mu_e);
     prof_end(ctx, tok);
