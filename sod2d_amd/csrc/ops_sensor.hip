@@ -165,22 +165,60 @@ __global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
     const size_t NE = (size_t)A.nelem;
     const double *u1 = A.u + A.npoin, *u2 = A.u + 2 * (size_t)A.npoin;
     const double hk = A.helem_k[e], cp = A.cmax / (double)A.porder;
+    // the nodes are taken KA at a time with all their loads issued before the
+    // stores and the arithmetic (in node order, as the plain loop), so that a
+    // lane has KA gathers in flight; a group past nnode repeats the last node
+    constexpr int KA = 4;
     double betae = 0.0;
-    for (int a = 0; a < A.nnode; ++a) {
-        const size_t ea = e + NE * a;
-        A.mue_l[ea] = A.mu_e[ea];
-        const int n = A.connec[ea] - 1;
-        const double v0 = A.u[n], v1 = u1[n], v2 = u2[n];
-        const double speed = sqrt(v0 * v0 + v1 * v1 + v2 * v2) + A.fact_low_mach * A.csound[n];
-        betae = fmax(betae, (A.rho[n] * hk) * cp * speed);
+    for (int a0 = 0; a0 < A.nnode; a0 += KA) {
+        size_t ea[KA];
+        int n[KA];
+        double mu[KA], v0[KA], v1[KA], v2[KA], cs[KA], rh[KA];
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            ea[j] = e + NE * min(a0 + j, A.nnode - 1);
+            n[j] = A.connec[ea[j]] - 1;
+            mu[j] = A.mu_e[ea[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            v0[j] = A.u[n[j]];
+            v1[j] = u1[n[j]];
+            v2[j] = u2[n[j]];
+            cs[j] = A.csound[n[j]];
+            rh[j] = A.rho[n[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            if (a0 + j >= A.nnode) break;
+            A.mue_l[ea[j]] = mu[j];
+            const double speed =
+                sqrt(v0[j] * v0[j] + v1[j] * v1[j] + v2[j] * v2[j]) + A.fact_low_mach * cs[j];
+            betae = fmax(betae, (rh[j] * hk) * cp * speed);
+        }
     }
     const double hp = hk / (double)A.porder;
-    for (int a = 0; a < A.nnode; ++a) {
-        const size_t ea = e + NE * a;
-        const int n = A.connec[ea] - 1;
-        const double R1 = A.rho[n] * fabs(A.Reta[n]) / norm;
-        const double Ve = A.ce * R1 * (hp * hp);
-        A.mu_e[ea] = A.cglob * fmin(Ve, betae);
+    for (int a0 = 0; a0 < A.nnode; a0 += KA) {
+        size_t ea[KA];
+        int n[KA];
+        double rh[KA], re[KA];
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            ea[j] = e + NE * min(a0 + j, A.nnode - 1);
+            n[j] = A.connec[ea[j]] - 1;
+        }
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            rh[j] = A.rho[n[j]];
+            re[j] = A.Reta[n[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            if (a0 + j >= A.nnode) break;
+            const double R1 = rh[j] * fabs(re[j]) / norm;
+            const double Ve = A.ce * R1 * (hp * hp);
+            A.mu_e[ea[j]] = A.cglob * fmin(Ve, betae);
+        }
     }
 }
 
