@@ -234,16 +234,41 @@ __global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     double dtl = 1.0e14;
     if (e < nelem) {
+        // nodes KA at a time, all loads of a group issued before its
+        // arithmetic (fmax is exact, so the grouping does not change the
+        // result); a group past nnode repeats the last node
+        constexpr int KA = 4;
         double L3 = 0.0, maxmu = 0.0;
-        for (int a = 0; a < nnode; ++a) {
-            const int n = connecT[(size_t)e * nnode + a];
-            double um = fabs(u[n]);
-            um = fmax(um, fabs(u[n + npoin]));
-            um = fmax(um, fabs(u[n + 2 * (size_t)npoin]));
-            L3 = fmax(L3, um + csound[n]);
-            if (mu_fluid) {
-                const double sg = mu_sgs ? mu_sgs[e + (size_t)nelem * a] : 0.0;
-                maxmu = fmax(maxmu, (sg + mu_fluid[n] + mu_e[e + (size_t)nelem * a]) / rho[n]);
+        for (int a0 = 0; a0 < nnode; a0 += KA) {
+            int n[KA];
+            size_t ea[KA];
+#pragma unroll
+            for (int j = 0; j < KA; ++j) {
+                const int a = min(a0 + j, nnode - 1);
+                n[j] = connecT[(size_t)e * nnode + a];
+                ea[j] = e + (size_t)nelem * a;
+            }
+            double v0[KA], v1[KA], v2[KA], cs[KA], mf[KA], me[KA], sg[KA], rh[KA];
+#pragma unroll
+            for (int j = 0; j < KA; ++j) {
+                v0[j] = u[n[j]];
+                v1[j] = u[n[j] + npoin];
+                v2[j] = u[n[j] + 2 * (size_t)npoin];
+                cs[j] = csound[n[j]];
+                if (mu_fluid) {
+                    mf[j] = mu_fluid[n[j]];
+                    me[j] = mu_e[ea[j]];
+                    sg[j] = mu_sgs ? mu_sgs[ea[j]] : 0.0;
+                    rh[j] = rho[n[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < KA; ++j) {
+                double um = fabs(v0[j]);
+                um = fmax(um, fabs(v1[j]));
+                um = fmax(um, fabs(v2[j]));
+                L3 = fmax(L3, um + cs[j]);
+                if (mu_fluid) maxmu = fmax(maxmu, (sg[j] + mf[j] + me[j]) / rh[j]);
             }
         }
         const double hh = helem[e] / (double)(2 * porder + 1);
