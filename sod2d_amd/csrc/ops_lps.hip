@@ -281,12 +281,27 @@ __global__ __launch_bounds__(256) void k_comp_tau(int nelem, int nnode, int pord
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nelem) return;
     const double h = helem_k[e] * (c_species_stab / (double)porder);
+    // nodes KA at a time with all loads of a group issued first (fmax is
+    // exact); a group past nnode repeats the last node
+    constexpr int KA = 4;
     double taul = 0.0;
-    for (int a = 0; a < nnode; ++a) {
-        const int n = connec[e + (size_t)nelem * a] - 1;
-        const double u0 = u[n], u1 = u[n + (size_t)npoin], u2 = u[n + 2 * (size_t)npoin];
-        const double aux1 = fact_c * csound[n] + sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-        taul = fmax(taul, h * aux1);
+    for (int a0 = 0; a0 < nnode; a0 += KA) {
+        int n[KA];
+#pragma unroll
+        for (int j = 0; j < KA; ++j) n[j] = connec[e + (size_t)nelem * min(a0 + j, nnode - 1)] - 1;
+        double u0[KA], u1[KA], u2[KA], cs[KA];
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            u0[j] = u[n[j]];
+            u1[j] = u[n[j] + (size_t)npoin];
+            u2[j] = u[n[j] + 2 * (size_t)npoin];
+            cs[j] = csound[n[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < KA; ++j) {
+            const double aux1 = fact_c * cs[j] + sqrt(u0[j] * u0[j] + u1[j] * u1[j] + u2[j] * u2[j]);
+            taul = fmax(taul, h * aux1);
+        }
     }
     tau[e] = taul;
 }
