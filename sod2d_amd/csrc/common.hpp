@@ -108,6 +108,12 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
              MeshCache **out);
 // working-node incidence of mesh m for the list lpoin_w (1-based, npoin_w entries)
 int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w);
+// updateF's EOS on the working nodes of m (after get_working) and Sutherland's
+// law with a uniform factor on all its nodes, in one pass (ops_node.hip)
+int eos_stage_sutherland(sodhip_ctx ctx, const MeshCache *m, double gamma_gas, double Rgas,
+                         double mu_factor, const double *rho, const double *q, const double *E,
+                         double *u, double *e_int, double *pr, double *aux_h, double *Tem,
+                         double *mu_fluid);
 
 // launch timing helpers: prof_begin returns a token (or -1); prof_end closes it
 int prof_begin(sodhip_ctx ctx, int kind);

@@ -611,10 +611,9 @@ extern "C" int sodhip_rk_stage_lps_f64(
                    "the next stage must not overwrite the new state");
     if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
     // updateF's EOS block and Sutherland's law (time_integ_ls.f90:572-599)
-    if ((rc = sodhip_eos_stage_f64(ctx, npoin, npoin_w, lpoin_w, prm->gamma_gas, prm->Rgas, rho, q,
-                                   E, u, e_int, pr, aux_h, Tem)))
+    if ((rc = eos_stage_sutherland(ctx, m, prm->gamma_gas, prm->Rgas, prm->mu_factor, rho, q, E, u,
+                                   e_int, pr, aux_h, Tem, mu_fluid)))
         return rc;
-    if ((rc = sodhip_sutherland_f64(ctx, npoin, Tem, nullptr, prm->mu_factor, mu_fluid))) return rc;
     const size_t EN = (size_t)nelem * nnode_of(porder);
     if ((rc = ensure_buf(ctx->scratch, 17 * EN * sizeof(double)))) return rc;
     double *rec = (double *)ctx->scratch.ptr, *prec = rec + 5 * EN;

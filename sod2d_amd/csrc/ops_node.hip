@@ -160,6 +160,58 @@ __global__ __launch_bounds__(256) void k_eos_stage(int npoin, int npoin_w,
     Tem[n] = p / (r * Rgas);
 }
 
+// updateF's EOS block on the working nodes (wflag) followed by
+// sutherland_viscosity on every node, in one pass over all npoin: each output
+// has the bits of k_eos_stage then k_sutherland (nodes outside the working
+// list keep their temperature)
+__global__ __launch_bounds__(256) void k_eos_stage_suth(
+    int npoin, const unsigned char *__restrict__ wflag, double g, double Rgas,
+    double mu_factor, const double *__restrict__ rho, const double *__restrict__ q,
+    const double *__restrict__ E, double *__restrict__ u, double *__restrict__ e_int,
+    double *__restrict__ pr, double *__restrict__ aux_h, double *__restrict__ Tem,
+    double *__restrict__ mu_fluid)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npoin) return;
+    double T;
+    if (wflag[n]) {
+        const double r = rho[n];
+        double umag = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double ud = q[n + (size_t)npoin * d] / r;
+            u[n + (size_t)npoin * d] = ud;
+            umag += ud * ud;
+        }
+        const double ei = E[n] / r - 0.5 * umag;
+        const double p = r * (g - 1.0) * ei;
+        e_int[n] = ei;
+        pr[n] = p;
+        if (aux_h) aux_h[n] = (g / (g - 1.0)) * p / r;
+        T = p / (r * Rgas);
+        Tem[n] = T;
+    } else {
+        T = Tem[n];
+    }
+    mu_fluid[n] = mu_factor * 0.000001458 * pow(T, 1.5) / (T + 110.4);
+}
+
+int eos_stage_sutherland(sodhip_ctx ctx, const MeshCache *m, double gamma_gas, double Rgas,
+                         double mu_factor, const double *rho, const double *q, const double *E,
+                         double *u, double *e_int, double *pr, double *aux_h, double *Tem,
+                         double *mu_fluid)
+{
+    SODHIP_REQUIRE(m && m->wflag, "working-node flags not built");
+    const int npoin = m->npoin;
+    int tok = prof_begin(ctx, 3);
+    k_eos_stage_suth<<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(
+        npoin, m->wflag, gamma_gas, Rgas, mu_factor, rho, q, E, u, e_int, pr, aux_h, Tem,
+        mu_fluid);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
 // rk_4_ls_main end-of-step EOS (time_integ_ls.f90:391-407)
 __global__ __launch_bounds__(256) void k_eos_step(int npoin, int npoin_w,
                                                   const int *__restrict__ lpoin_w, double g,
