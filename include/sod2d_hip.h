@@ -97,6 +97,26 @@ int sodhip_destroy(sodhip_ctx ctx);
 int sodhip_set_stream(sodhip_ctx ctx, void *stream);
 int sodhip_synchronize(sodhip_ctx ctx);
 
+/* Lifetime of the cached connectivity tables.  The first call with a given
+ * connec (device pointer, porder, nelem, npoin) builds element-major and
+ * node-ordered copies of it, and the first call with a given lpoin_w pointer
+ * builds its working-node incidence; later calls with the same pointers reuse
+ * them without reading the arrays again.  A caller that frees such an array, or
+ * rewrites it with another mesh, calls sodhip_forget_mesh with its device
+ * address first (sodhip_exit_data does this for the range it releases).  No
+ * counterpart in the reference: OpenACC's present table has no derived data. */
+int sodhip_forget_mesh(sodhip_ctx ctx, const void *connec_or_lpoin_w);
+
+/* Scratch buffers of ctx that a captured HIP graph records by address: the
+ * per element-node residual partials and the step's reduction partials.
+ * sodhip_reserve_scratch grows them to at least the given sizes; while pinned
+ * (sodhip_pin_scratch(ctx, 1)) they are never reallocated, and a call needing
+ * more fails with SODHIP_ERR_ARG instead of moving them under the graph.  Host-
+ * read diagnostics (dt, EK, dissipation, max Mach) use a buffer of their own.
+ * No counterpart in the reference. */
+int sodhip_reserve_scratch(sodhip_ctx ctx, size_t scratch_bytes, size_t scratch2_bytes);
+int sodhip_pin_scratch(sodhip_ctx ctx, int pin);
+
 /* OpenACC-style present table for callers that hold host arrays (the Fortran
  * shim): enter data create/copyin, update device/host, exit data, and the
  * host -> device address lookup (NULL when not present). */

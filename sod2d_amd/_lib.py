@@ -48,6 +48,9 @@ _SIGS = {
     "sodhip_destroy": (I, [P]),
     "sodhip_set_stream": (I, [P, P]),
     "sodhip_synchronize": (I, [P]),
+    "sodhip_forget_mesh": (I, [P, P]),
+    "sodhip_reserve_scratch": (I, [P, SZ, SZ]),
+    "sodhip_pin_scratch": (I, [P, I]),
     "sodhip_enter_data": (I, [P, P, SZ, I]),
     "sodhip_update_device": (I, [P, P, SZ]),
     "sodhip_update_host": (I, [P, P, SZ]),

@@ -39,6 +39,9 @@ inline int nnode_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    // set while a captured graph holds the address: the buffer may then not be
+    // reallocated, and a request beyond its size is an error
+    bool pinned = false;
 };
 
 // Node-ordered incidence of a connectivity: node n (0-based) owns the slots
@@ -86,7 +89,8 @@ struct sodhip_ctx_s {
     std::map<uintptr_t, sodhip::PresentEntry> present; // host start address -> mirror
     std::vector<std::unique_ptr<sodhip::MeshCache>> meshes;
     sodhip::DevBuf scratch;  // per element-node residual partials
-    sodhip::DevBuf scratch2; // reduction partials
+    sodhip::DevBuf scratch2; // reduction partials of the step (the sensor's normalisation)
+    sodhip::DevBuf diag;     // reduction partials read back by the host (dt, diagnostics)
     sodhip::DevBuf hostbuf;  // pinned staging (small reductions)
     bool prof = false;
     struct Pending {

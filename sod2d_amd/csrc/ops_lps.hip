@@ -615,8 +615,11 @@ extern "C" int sodhip_rk_stage_lps_f64(
                                    e_int, pr, aux_h, Tem, mu_fluid)))
         return rc;
     const size_t EN = (size_t)nelem * nnode_of(porder);
-    if ((rc = ensure_buf(ctx->scratch, 17 * EN * sizeof(double)))) return rc;
-    double *rec = (double *)ctx->scratch.ptr, *prec = rec + 5 * EN;
+    // the projection records start on a 256-byte boundary (their gathers read
+    // them with 16-byte loads, and 5*EN doubles is odd when EN is)
+    const size_t prec_at = (5 * EN + 31) & ~(size_t)31;
+    if ((rc = ensure_buf(ctx->scratch, (prec_at + 12 * EN) * sizeof(double)))) return rc;
+    double *rec = (double *)ctx->scratch.ptr, *prec = rec + prec_at;
     // convection records (elem_convec.f90:227-470, E slot = enthalpy)
     if ((rc = conv_elements<double>(ctx, porder, m, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI,
                                     gmshAtoJ, gmshAtoK, u, q, rho, pr, aux_h, rec)))

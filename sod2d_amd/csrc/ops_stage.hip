@@ -524,8 +524,8 @@ extern "C" int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, in
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
     const int nb = (nelem + 255) / 256;
-    if ((rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double)))) return rc;
-    double *part = (double *)ctx->scratch2.ptr;
+    if ((rc = ensure_buf(ctx->diag, (size_t)nb * sizeof(double)))) return rc;
+    double *part = (double *)ctx->diag.ptr;
     int tok = prof_begin(ctx, 3);
     k_dt_cfl<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), porder, m->connecT, helem, u,
                                           csound, npoin, cfl_conv, cfl_diff, mu_fluid, mu_sgs,
@@ -550,8 +550,8 @@ extern "C" int sodhip_volavg_ek_f64(sodhip_ctx ctx, int porder, int nelem, int n
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
     const int nb = (nelem + 255) / 256;
-    if ((rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double)))) return rc;
-    double *part = (double *)ctx->scratch2.ptr;
+    if ((rc = ensure_buf(ctx->diag, (size_t)nb * sizeof(double)))) return rc;
+    double *part = (double *)ctx->diag.ptr;
     k_ek<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), m->connecT, gpvol, rho, u, npoin,
                                       part);
     SODHIP_TRY_HIP(hipGetLastError());
@@ -582,8 +582,8 @@ extern "C" int sodhip_visc_dissipation_f64(sodhip_ctx ctx, int porder, int nelem
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
     const int nb = (nelem + 255) / 256;
-    if ((rc = ensure_buf(ctx->scratch2, (2 * (size_t)nelem + 2 * nb) * sizeof(double)))) return rc;
-    double *elS = (double *)ctx->scratch2.ptr, *elD = elS + nelem, *pS = elD + nelem, *pD = pS + nb;
+    if ((rc = ensure_buf(ctx->diag, (2 * (size_t)nelem + 2 * nb) * sizeof(double)))) return rc;
+    double *elS = (double *)ctx->diag.ptr, *elD = elS + nelem, *pS = elD + nelem, *pD = pS + nb;
     const int grid = std::min(nelem, ctx->num_cu * 16);
     switch (porder) {
     case 2:
@@ -626,9 +626,9 @@ extern "C" int sodhip_max_mach_f64(sodhip_ctx ctx, int npoin, int npoin_w, const
     SODHIP_REQUIRE(npoin_w > 0 && npoin_w <= npoin, "npoin_w outside 1..npoin");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     const int nb = (npoin_w + 255) / 256;
-    int rc = ensure_buf(ctx->scratch2, (size_t)nb * sizeof(double));
+    int rc = ensure_buf(ctx->diag, (size_t)nb * sizeof(double));
     if (rc) return rc;
-    double *part = (double *)ctx->scratch2.ptr;
+    double *part = (double *)ctx->diag.ptr;
     k_max_mach<<<nb, 256, 0, ctx->stream>>>(npoin_w, lpoin_w, machno, part);
     SODHIP_TRY_HIP(hipGetLastError());
     std::vector<double> h;

@@ -25,6 +25,11 @@ int set_error(int code, const char *fmt, ...)
 int ensure_buf(DevBuf &b, size_t bytes)
 {
     if (b.bytes >= bytes && b.ptr) return SODHIP_OK;
+    if (b.pinned)
+        return set_error(SODHIP_ERR_ARG,
+                         "a captured graph holds this scratch buffer (%zu bytes); %zu bytes "
+                         "requested: release the graph or reserve more before capturing",
+                         b.bytes, bytes);
     if (b.ptr) {
         SODHIP_TRY_HIP(hipFree(b.ptr));
         b.ptr = nullptr;
@@ -261,6 +266,7 @@ extern "C" int sodhip_destroy(sodhip_ctx ctx)
     for (auto &kv : ctx->present) (void)hipFree(kv.second.dev);
     (void)hipFree(ctx->scratch.ptr);
     (void)hipFree(ctx->scratch2.ptr);
+    (void)hipFree(ctx->diag.ptr);
     if (ctx->hostbuf.ptr) (void)hipHostFree(ctx->hostbuf.ptr);
     for (auto &p : ctx->pending) {
         (void)hipEventDestroy(p.a);
@@ -268,6 +274,67 @@ extern "C" int sodhip_destroy(sodhip_ctx ctx)
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     delete ctx;
+    return SODHIP_OK;
+}
+
+// Drops every cached table derived from device memory in [lo, hi): meshes keyed
+// on a connectivity there, and working-node incidences keyed on a list there.
+static int forget_range(sodhip_ctx ctx, uintptr_t lo, uintptr_t hi)
+{
+    bool synced = false;
+    auto sync = [&]() -> int {
+        if (!synced) SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+        synced = true;
+        return SODHIP_OK;
+    };
+    int rc;
+    for (auto it = ctx->meshes.begin(); it != ctx->meshes.end();) {
+        MeshCache *m = it->get();
+        const uintptr_t k = (uintptr_t)m->connec_key;
+        if (k >= lo && k < hi) {
+            if ((rc = sync())) return rc;
+            free_mesh(m);
+            it = ctx->meshes.erase(it);
+            continue;
+        }
+        const uintptr_t w = (uintptr_t)m->lpw_key;
+        if (m->wrow && w >= lo && w < hi) {
+            if ((rc = sync())) return rc;
+            (void)hipFree(m->wrow);
+            (void)hipFree(m->wsrc);
+            (void)hipFree(m->wflag);
+            m->wrow = m->wsrc = nullptr;
+            m->wflag = nullptr;
+            m->lpw_key = nullptr;
+            m->npoin_w = 0;
+        }
+        ++it;
+    }
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_forget_mesh(sodhip_ctx ctx, const void *dev)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    if (!dev) return SODHIP_OK;
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    return forget_range(ctx, (uintptr_t)dev, (uintptr_t)dev + 1);
+}
+
+extern "C" int sodhip_reserve_scratch(sodhip_ctx ctx, size_t scratch_bytes, size_t scratch2_bytes)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    int rc = scratch_bytes ? ensure_buf(ctx->scratch, scratch_bytes) : SODHIP_OK;
+    if (!rc && scratch2_bytes) rc = ensure_buf(ctx->scratch2, scratch2_bytes);
+    return rc;
+}
+
+extern "C" int sodhip_pin_scratch(sodhip_ctx ctx, int pin)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    ctx->scratch.pinned = pin != 0;
+    ctx->scratch2.pinned = pin != 0;
     return SODHIP_OK;
 }
 
@@ -352,6 +419,11 @@ extern "C" int sodhip_exit_data(sodhip_ctx ctx, const void *host, int copyout)
                                       hipMemcpyDeviceToHost, ctx->stream));
     }
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    {
+        const uintptr_t lo = (uintptr_t)it->second.dev;
+        int rc = forget_range(ctx, lo, lo + it->second.bytes);
+        if (rc) return rc;
+    }
     SODHIP_TRY_HIP(hipFree(it->second.dev));
     ctx->present.erase(it);
     return SODHIP_OK;

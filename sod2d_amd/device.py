@@ -16,6 +16,7 @@ starting at 1.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -91,6 +92,33 @@ def default_context() -> Context:
     return _default
 
 
+_tracked: dict = {}
+
+
+def _bind(ctx, *mesh_arrays):
+    """The context to use (ctx or the default one).  The library caches tables
+    derived from connectivity and working-node arrays by device address; each
+    such tensor gets a finaliser that drops those tables when it is freed, so a
+    later array at the same address is never matched to them."""
+    ctx = ctx or default_context()
+    for t in mesh_arrays:
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+            continue
+        key = (id(ctx), id(t))
+        if key in _tracked:
+            continue
+        ptr = t.data_ptr()
+        _tracked[key] = weakref.finalize(t, _forget, weakref.ref(ctx), ptr, key)
+    return ctx
+
+
+def _forget(ctx_ref, ptr, key):
+    _tracked.pop(key, None)
+    ctx = ctx_ref()
+    if ctx is not None and getattr(ctx, "h", None):
+        lib().sodhip_forget_mesh(ctx.h, C.c_void_p(ptr))
+
+
 def _porder(nelem, connec):
     nn = connec.numel() // nelem
     p = round(nn ** (1.0 / 3.0)) - 1
@@ -113,7 +141,7 @@ def full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, 
                     gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener,
                     initialze=None, fact=None, ctx: Context | None = None):
     """full_convec_ijk (elem_convec.f90:227-470) on the device."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     init, f = _resolve("convec", initialze, fact)
@@ -141,7 +169,7 @@ def full_diffusion_ijk(nelem, npoin, connec, Ngp, He, gpvol, dlxigp_ip, invAtoIJ
                        Rmass, Rmom, Rener, initialze=None, fact=None, c_rho=1.0, c_ener=1.0,
                        ctx: Context | None = None):
     """full_diffusion_ijk (elem_diffu.f90:12-223) on the device."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     init, f = _resolve("diffu", initialze, fact)
@@ -174,7 +202,7 @@ C_SPECIES_STAB = 0.5
 def comp_tau(nelem, npoin, connec, csound, u, helem_k, dt, tau, c_species_stab=C_SPECIES_STAB,
              flag_drop_c_in_envit=0, ctx: Context | None = None):
     """comp_tau (elem_stab.f90:589-616) on the device (dt is unused there too)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -190,7 +218,7 @@ def full_proj_ijk(nelem, npoin, npoin_w, connec, lpoin_w, Ngp, He, gpvol, dlxigp
                   gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, Ml, invMl, ProjMass, ProjEner,
                   ProjMX, ProjMY, ProjMZ, ctx: Context | None = None):
     """full_proj_ijk (elem_stab.f90:441-586) on the device."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec, lpoin_w)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -213,7 +241,7 @@ def full_diff_stab_ijk(nelem, npoin, connec, Ngp, He, gpvol, dlxigp_ip, invAtoIJ
                        initialze=None, fact=None, c_rho=1.0, c_ener=1.0, c_lps=C_LPS_COMP,
                        ctx: Context | None = None):
     """full_diff_stab_ijk (elem_stab.f90:207-438) on the device."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     init, f = _resolve("diffstab", initialze, fact)
@@ -241,7 +269,7 @@ def generic_scalar_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp
                               initialze=None, fact=None, ctx: Context | None = None):
     """generic_scalar_convec_ijk (elem_convec.f90:472-594) on the device (FP64); q may
     be None: the flux is then u * eta, formed in the kernel."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     init, f = _resolve("scalar", initialze, fact)
@@ -268,7 +296,7 @@ def smart_visc_spectral(nelem, npoin, npoin_w, connec, lpoin_w, Reta, Ngp, coord
                         invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mue_l, params: ViscParams,
                         ctx: Context | None = None):
     """smart_visc_spectral (mod_entropy_viscosity.f90:14-111) on the device (FP64)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec, lpoin_w)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -291,7 +319,7 @@ def smart_visc_spectral_imex(nelem, npoin, npoin_w, connec, lpoin_w, Reta, Ngp, 
     """smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227) on the device
     (FP64), the reference's live-path sensor.  With the normalisation and
     flag_high_mach, params.ce becomes ce_comp (the reference's module global)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec, lpoin_w)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -310,7 +338,7 @@ def smart_visc_spectral_imex(nelem, npoin, npoin_w, connec, lpoin_w, Reta, Ngp, 
 def eval_divergence(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
                     gmshAtoK, u, divU, ctx: Context | None = None):
     """eval_divergence (mod_operators.f90:485-570) on the device (FP64)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -326,7 +354,7 @@ def eval_divergence(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, gmshA
 def compute_vorticity(nelem, npoin, npoin_w, lpoin_w, connec, lelpn, He, dlxigp_ip, invAtoIJK,
                       gmshAtoI, gmshAtoJ, gmshAtoK, u, curlU, ctx: Context | None = None):
     """compute_vorticity with makeConti (mod_operators.f90:684-775) on the device."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, lpoin_w, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -343,7 +371,7 @@ def compute_vorticity(nelem, npoin, npoin_w, lpoin_w, connec, lelpn, He, dlxigp_
 def div_curl(nelem, npoin, npoin_w, lpoin_w, connec, lelpn, He, gpvol, dlxigp_ip, invAtoIJK,
              gmshAtoI, gmshAtoJ, gmshAtoK, u, divU, curlU, ctx: Context | None = None):
     """eval_divergence and compute_vorticity of one u in a single element pass."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, lpoin_w, connec)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -365,7 +393,7 @@ def ducros_visc_limit(nelem, npoin, npoin_w, connec, lpoin_w, Ngp, coord, dNgp, 
     """ducros_visc_limit (mod_entropy_viscosity.f90:277-314) on the device: mu_e
     scaled in place (rho, csound, helem_k and the other reference arguments do
     not enter the result)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, connec, lpoin_w)
     ctx.sync_stream()
     p = _porder(nelem, connec)
     nn = (p + 1) ** 3
@@ -379,7 +407,7 @@ def ducros_visc_limit(nelem, npoin, npoin_w, connec, lpoin_w, Ngp, coord, dNgp, 
 
 
 def lumped_solver_scal(npoin, npoin_w, lpoin_w, invMl, R, ctx: Context | None = None):
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, lpoin_w)
     ctx.sync_stream()
     check(lib().sodhip_lumped_solver_scal_f64(ctx.h, npoin, npoin_w,
                                               _ptr(lpoin_w, torch.int32, npoin_w),
@@ -388,7 +416,7 @@ def lumped_solver_scal(npoin, npoin_w, lpoin_w, invMl, R, ctx: Context | None = 
 
 
 def lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, R, ctx: Context | None = None):
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, lpoin_w)
     ctx.sync_stream()
     check(lib().sodhip_lumped_solver_vect_f64(ctx.h, npoin, npoin_w,
                                               _ptr(lpoin_w, torch.int32, npoin_w),
@@ -400,7 +428,7 @@ def lumped_solver_vect(npoin, npoin_w, lpoin_w, invMl, R, ctx: Context | None = 
 def rk_stage_update(npoin, npoin_w, lpoin_w, invMl, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E,
                     Rmass, Rmom, Rener, ctx: Context | None = None):
     """Lumped inverse fused with the Shu-Osher stage combine (time_integ_ls.f90:323-358)."""
-    ctx = ctx or default_context()
+    ctx = _bind(ctx, lpoin_w)
     ctx.sync_stream()
     d, I32 = torch.float64, torch.int32
     check(lib().sodhip_rk_stage_update_f64(

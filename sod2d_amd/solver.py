@@ -48,7 +48,7 @@ import torch
 
 from . import case as case_mod
 from ._lib import SodhipError, StageParams, ViscParams, check, lib
-from .device import C_LPS_COMP, C_SPECIES_STAB, Context, _ptr
+from .device import C_LPS_COMP, C_SPECIES_STAB, Context, _bind, _ptr
 
 F64, F32, I32 = torch.float64, torch.float32, torch.int32
 
@@ -144,6 +144,7 @@ class TGVRun:
         # static device data
         self.connec = _dev(m.connec)
         self.lpoin_w = _dev(m.lpoin_w)
+        _bind(self.ctx, self.connec, self.lpoin_w)
         self.tabs = [_dev(x) for x in (T.invAtoIJK, T.gmshAtoI, T.gmshAtoJ, T.gmshAtoK)]
         self.He = _dev(g.He)
         self.gpvol = _dev(g.gpvol)
@@ -430,11 +431,19 @@ class TGVRun:
         saved = (self.time, self.istep)
         self.ctx.prof_enable(False)
         torch.cuda.synchronize()
+        # the graph records the library's scratch buffers by address: from here
+        # until the graph is released they may not move (a call that would grow
+        # them fails instead)
+        check(lib().sodhip_pin_scratch(self.ctx.h, 1), "pin_scratch")
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.ctx.sync_stream()  # launches go to the capturing stream
-            self.step()
-            self.step()
+        try:
+            with torch.cuda.graph(g):
+                self.ctx.sync_stream()  # launches go to the capturing stream
+                self.step()
+                self.step()
+        except BaseException:
+            lib().sodhip_pin_scratch(self.ctx.h, 0)
+            raise
         self.ctx.sync_stream()
         self.time, self.istep = saved
         return StepGraph(self, g)
@@ -461,10 +470,26 @@ class StepGraph:
 
     def replay(self):
         """Advances the run by two steps."""
+        if self.graph is None:
+            raise SodhipError("the graph was released")
         self.run.ctx.sync_stream()
         self.graph.replay()
         self.run.time += 2 * self.run.dt
         self.run.istep += 2
+
+    def release(self):
+        """Frees the graph and lets the library's scratch buffers move again."""
+        if self.graph is not None:
+            torch.cuda.synchronize()
+            self.graph = None
+            if getattr(self.run.ctx, "h", None):
+                check(lib().sodhip_pin_scratch(self.run.ctx.h, 0), "pin_scratch")
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 def rk_ls_coeffs(stages: int, variant: int = 1):

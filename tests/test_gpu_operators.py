@@ -229,3 +229,43 @@ def test_smart_visc_spectral_f64(oracle, p, normalise):
     # both limiter branches are exercised
     lim = mue_l.ravel(order="F")
     assert 0 < np.mean(lim == lim.max()) < 1
+
+
+def test_mesh_cache_follows_freed_connectivity(oracle):
+    """A connectivity freed and replaced by a different mesh of the same size
+    (here the same elements in reversed order, which torch's caching allocator
+    puts at the same address) is not matched to the first mesh's cached
+    incidence tables."""
+    import gc
+
+    from sod2d_amd import device as D
+    p = 3
+    g = make_case(p, 3, periodic=True)
+    m, T = g.mesh, g.tables
+    s = random_state(m.npoin, m.nelem, T.nnode, seed=5)
+    OT = oracle.Tables(p)
+    tabs = [_dev(x) for x in (T.invAtoIJK, T.gmshAtoI, T.gmshAtoJ, T.gmshAtoK)]
+    flds = [_dev(s[k]) for k in ("u", "q", "rho", "pr", "h")]
+    dl = _dev(T.dlxigp_ip)
+
+    def run(connec, He, gpvol):
+        c = _dev(connec)
+        out = [torch.zeros(k, dtype=torch.float64, device="cuda")
+               for k in (m.npoin, 3 * m.npoin, m.npoin)]
+        D.full_convec_ijk(m.nelem, m.npoin, c, None, None, _dev(He), _dev(gpvol), dl, None,
+                          *tabs, *flds, *out, True, -1.0)
+        torch.cuda.synchronize()
+        return c.data_ptr(), [x.cpu().numpy() for x in out]
+
+    ptr_a, _ = run(m.connec, g.He, g.gpvol)
+    gc.collect()
+    rev = np.asfortranarray(m.connec[::-1, :])
+    He_r = np.asfortranarray(g.He[..., ::-1])
+    gv_r = np.asfortranarray(g.gpvol[..., ::-1])
+    ptr_b, (dRm, dRq, dRe) = run(rev, He_r, gv_r)
+    Rm, Rq, Re = oracle.full_convec_ijk(OT, rev, He_r, gv_r, s["u"], s["q"], s["rho"], s["pr"],
+                                        s["h"], fact=-1.0)
+    print(f"connec addresses: first {ptr_a:#x}, second {ptr_b:#x}")
+    assert relerr(dRm, Rm) < 1e-13
+    assert relerr(dRq, Rq.ravel(order="F")) < 1e-13
+    assert relerr(dRe, Re) < 1e-13

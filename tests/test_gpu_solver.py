@@ -129,15 +129,17 @@ def test_fused_end_of_step_same_bits(oracle, sensor_path):
             assert np.array_equal(s[k], r[k]), (step, k)
 
 
-@pytest.mark.parametrize("lps", [False, True])
-def test_graph_captured_steps(oracle, lps):
+@pytest.mark.parametrize("lps,n", [(False, 4), (True, 4), (False, 8)])
+def test_graph_captured_steps(oracle, lps, n):
     """Two steps captured in a HIP graph and replayed give the same bits as the
     same steps launched eagerly, on every replay (with eager work and
-    synchronisations between the replays)."""
+    synchronisations between the replays).  The diagnostics called between the
+    replays need more reduction space than the captured sensor at 8^3 elements
+    (2*nelem + 2*blocks doubles): they must not move a buffer the graph holds."""
     import torch
 
     from sod2d_amd.solver import TGVParams, TGVRun
-    g = make_case(3, 4, periodic=True, lengths=(2 * np.pi,) * 3)
+    g = make_case(3, n, periodic=True, lengths=(2 * np.pi,) * 3)
     to = oracle.tgv_params()
     ic = oracle.tgv_ic(g.mesh.coord, to)
     kw = dict(geometry=g, params=TGVParams(), initial=ic, dt_mode="fixed", lps=lps,
@@ -157,4 +159,4 @@ def test_graph_captured_steps(oracle, lps):
             assert np.array_equal(a[k], b[k]), (rep, k)
         assert cap.istep == eager.istep and cap.time == eager.time
         da, db = eager.diagnostics(), cap.diagnostics()
-        assert da["EK"] == db["EK"], rep
+        assert da == db, rep
