@@ -406,6 +406,53 @@ def adapt_dt_cfl(p, connec, helem, u, csound, cfl_conv, cfl_diff, mu_fluid, mu_s
              _d(rho))
 
 
+# ---------------------------------------------------------------- LS-RK sequencing
+def ls_rk_step(update_f, combine, L, G, stages, dt, first, parity, on_first=None):
+    """The stage sequence of one rk_4_ls_main step (time_integ_ls.f90:270-361),
+    on any state: ``update_f()`` stores a fresh residual R of the working level
+    (updateF, :489-691) and ``combine(l1, ls, gdt)`` sets, on the working
+    level, y = l1*y1 + ls*y + gdt*R (the accumulation loops, :323-331 and
+    :350-358); y1 is the level copied from the working level before the step
+    (CFDSolverBase.f90:1952-1959), a copy the caller makes.
+
+    The first step (``first``) runs ``on_first()`` (the entropy residual and
+    sensor of the first-step block, :270-309) and a fresh updateF (:310-316).
+    After it, stage 1 reuses the residual left by the previous step's last
+    updateF (reference sequencing, ``parity``); ``parity=False`` evaluates a
+    fresh one, which turns the step into the plain SSP scheme of the table.
+    Returns the number of updateF calls made."""
+    n = 0
+    if first:
+        if on_first is not None:
+            on_first()
+        update_f()
+        n += 1
+    if not parity:
+        update_f()
+        n += 1
+    combine(0.0, 1.0, G[1, 0] * dt)
+    for s in range(2, stages + 1):
+        update_f()
+        n += 1
+        combine(L[s, 0], L[s, s - 1], G[s, s - 1] * dt)
+    return n
+
+
+def entropy_residual(lpoin_w, Reta, eta, dt, factor_comp=1.0):
+    """The end-of-step entropy residual (time_integ_ls.f90:436-444):
+    auxReta = 1.5 Reta(:,2) - 0.5 Reta(:,1) + factor_comp (eta(:,2) - eta(:,1))/dt
+    on the working nodes, then Reta(:,1) = Reta(:,2).  Reta(:,2) holds the
+    convective residual of the level-1 entropy (this step's start), Reta(:,1)
+    the previous step's; eta(:,1), eta(:,2) the entropy before and after the
+    step.  Returns auxReta (zero off the working nodes)."""
+    lw = lpoin_w - 1
+    aux = np.zeros(Reta.shape[0])
+    aux[lw] = (1.5 * Reta[lw, 1] - 0.5 * Reta[lw, 0]) + \
+        factor_comp * (eta[lw, 1] - eta[lw, 0]) / dt
+    Reta[lw, 0] = Reta[lw, 1]
+    return aux
+
+
 # ---------------------------------------------------------------- TGV
 def tgv_params(Cp=1004.0, Prt=0.71, M=0.1, Re=1600.0, rho0=1.0, gamma_gas=1.4):
     t = TGVParams(Cp, Prt, M, Re, rho0, gamma_gas, 0, 0, 0, 0)
@@ -559,35 +606,29 @@ class TGVOracle:
             lv[0][...] = lv[1]
         self.eta[:, 0] = self.eta[:, 1]
         lw = self.lpoin_w - 1
-        L, G = self.L, self.G
         if self.lps:
             self.tau_stab = comp_tau(self.p, self.connec, self.csound, self.u[1], self.geo.helem)
-        if self.first:
-            self.first = False
+
+        def on_first():
             self.f_eta[lw, :] = self.u[0][lw, :] * self.eta[lw, 0][:, None]
             self._scalar_convec(0)
             if self.sensor:
                 self._sensor(self.Reta[:, 0], self.rho[0], self.u[0], self.Tem[0], self.eta[:, 0])
-            self.update_f()
-        if not self.parity:
-            self.update_f()
-        rk_ls_update(self.lpoin_w, 0.0, 1.0, G[1, 0] * self.dt, self.rho[0], self.q[0],
-                     self.E[0], self.rho[1], self.q[1], self.E[1], self.Rmass, self.Rmom,
-                     self.Rener)
-        for s in range(2, self.S + 1):
-            self.update_f()
-            rk_ls_update(self.lpoin_w, L[s, 0], L[s, s - 1], G[s, s - 1] * self.dt, self.rho[0],
-                         self.q[0], self.E[0], self.rho[1], self.q[1], self.E[1], self.Rmass,
-                         self.Rmom, self.Rener)
+
+        def combine(l1, ls, gdt):
+            rk_ls_update(self.lpoin_w, l1, ls, gdt, self.rho[0], self.q[0], self.E[0],
+                         self.rho[1], self.q[1], self.E[1], self.Rmass, self.Rmom, self.Rener)
+
+        ls_rk_step(self.update_f, combine, self.L, self.G, self.S, self.dt, self.first,
+                   self.parity, on_first)
+        self.first = False
         t = self.t
         eos_step(self.lpoin_w, t.gamma_gas, t.Rgas, self.rho[1], self.q[1], self.E[1], self.u[1],
                  self.e_int[1], self.pr[1], self.csound, self.machno, self.Tem[1])
         thermo_entropy(self.lpoin_w, t.gamma_gas, t.po, t.rho0, self.rho[1], self.pr[1],
                        self.u[0], self.eta, self.f_eta)
         self._scalar_convec(1)
-        self.auxReta[lw] = (1.5 * self.Reta[lw, 1] - 0.5 * self.Reta[lw, 0]) + \
-            1.0 * (self.eta[lw, 1] - self.eta[lw, 0]) / self.dt
-        self.Reta[lw, 0] = self.Reta[lw, 1]
+        self.auxReta[...] = entropy_residual(self.lpoin_w, self.Reta, self.eta, self.dt)
         if self.sensor:
             self._sensor(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[:, 1],
                          ducros=True)

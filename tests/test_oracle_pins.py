@@ -615,60 +615,3 @@ def test_rk4_classical(oracle):
             s += b[i] * z * yi
         assert abs(1.0 + s - sum(z ** j / math.factorial(j) for j in range(5))) < 1e-15
     assert np.allclose(c, a)
-
-
-def _lagged_step_matrix(oracle, L, G, S, z):
-    """One step of rk_4_ls_main's sequencing for y' = (z/dt) y, dt = 1: stage 1
-    advances with the residual kept from the previous step's last updateF
-    (time_integ_ls.f90:323-331; no updateF follows the stage loop, :339-487),
-    stages 2..S evaluate it fresh.  Returns the map (y, r) -> (y', r') with r
-    the kept residual."""
-    cols = []
-    for y0v, rv in ((1.0, 0.0), (0.0, 1.0)):
-        lw = np.array([1], np.int32)
-        y0 = [np.full(1, y0v, complex), np.full((1, 3), y0v, complex, order="F"),
-              np.full(1, y0v, complex)]
-        y = [a.copy(order="F") for a in y0]
-        r = [np.full(1, rv, complex), np.full((1, 3), rv, complex, order="F"),
-             np.full(1, rv, complex)]
-        for s in range(1, S + 1):
-            if s > 1:
-                r = [z * y[0], np.asfortranarray(z * y[1]), z * y[2]]
-            lam1, lams = (0.0, 1.0) if s == 1 else (L[s, 0], L[s, s - 1])
-            for a, a0, ra in zip(y, y0, r):  # the oracle's update, complex-valued
-                a[...] = lam1 * a0 + lams * a + G[s, s - 1] * ra
-        cols.append([y[0][0], r[0][0]])
-    return np.array(cols).T
-
-
-def test_reference_sequencing_has_no_imaginary_axis_stability(oracle):
-    """The kept stage-1 residual turns SSP(5,3) into a two-level scheme whose
-    spectral radius exceeds 1 for every purely imaginary z: undamped
-    oscillatory modes grow, and only the viscous, entropy-viscosity (or LPS)
-    damping of the discrete operator bounds them.  With fresh residuals the
-    scheme is the stability polynomial above, stable on the imaginary axis up
-    to |z| = 2.398.  Fixed dt at CFL 0.5 with the plain method (no LPS) is
-    therefore stable with fresh residuals and grows an acoustic mode with the
-    reference's sequencing."""
-    L, G = oracle.rk_ls_coeffs(5, 1)
-    # the update written above is the oracle's rk_ls_update (checked on reals)
-    lw = np.array([1], np.int32)
-    y0 = [np.full(1, 0.7), np.full((1, 3), 0.7, order="F"), np.full(1, 0.7)]
-    y = [np.full(1, 1.3), np.full((1, 3), 1.3, order="F"), np.full(1, 1.3)]
-    r = [np.full(1, -0.4), np.full((1, 3), -0.4, order="F"), np.full(1, -0.4)]
-    oracle.rk_ls_update(lw, L[4, 0], L[4, 3], G[4, 3], *y0, *y, *r)
-    assert abs(y[0][0] - (L[4, 0] * 0.7 + L[4, 3] * 1.3 - G[4, 3] * 0.4)) < 1e-15
-    rho = lambda z: max(abs(np.linalg.eigvals(_lagged_step_matrix(oracle, L, G, 5, z))))  # noqa
-    for w in (0.05, 0.5, 1.0, 2.0):
-        assert rho(1j * w) > 1.0 + 1e-5 * w, w
-    assert rho(-0.5 + 0j) < 1.0 and rho(-0.3 + 0.2j) < 1.0
-    def fresh(z):
-        y0 = y = 1.0 + 0j
-        for s in range(1, 6):
-            lam1, lams = (0.0, 1.0) if s == 1 else (L[s, 0], L[s, s - 1])
-            y = lam1 * y0 + lams * y + G[s, s - 1] * z * y
-        return y
-    zs = np.linspace(-1.0, 1.0, 6)
-    assert np.allclose([fresh(z) for z in zs], _amplification(oracle, L, G, 5, zs), rtol=1e-15)
-    assert all(abs(fresh(1j * w)) <= 1.0 for w in np.linspace(0.01, 2.39, 60))
-    assert abs(fresh(2.41j)) > 1.0
