@@ -441,6 +441,28 @@ int sodhip_rk_stage_next_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, i
                              double *Rener, double gam_next, double *rho_next, double *q_next,
                              double *E_next);
 
+/* sodhip_rk_stage_f64 without any device allocation (same arguments, same
+ * kernels, same results).  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * The residual records live in a library-owned device array of 64 MiB per GPU,
+ * mapped by sodhip_create and held until the process ends, instead of the
+ * context's scratch; a mesh whose records exceed it (nelem*nnode*40 bytes with
+ * FP64 operators, *24 with FP32 operators) is rejected with SODHIP_ERR_ARG.
+ * The mesh and working-node tables are looked up, never built: an allocating
+ * call on the same connec and lpoin_w (sodhip_rk_stage_f64) must come first,
+ * else SODHIP_ERR_ARG.  Launch timing is off during the call.  Every call on
+ * one GPU shares the array, which keeps the last call's records: calls that
+ * overlap on one GPU (several contexts or streams of it) race. */
+int sodhip_rk_stage_noalloc_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                const int *connec, const int *lpoin_w, int op_precision,
+                                const void *He, const void *gpvol, const void *dlxigp_ip,
+                                const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+                                const int *gmshAtoK, const void *mu_e, const double *invMl,
+                                const sodhip_stage_params *prm, double dt, double lam1,
+                                double lams, double gam, const double *rho1, const double *q1,
+                                const double *E1, double *rho, double *q, double *E,
+                                double *Rmass, double *Rmom, double *Rener);
+
 /* One LS-RK stage of the reference's default viscous path (flag_lps_stab):
  * updateF (time_integ_ls.f90:489-691) with full_proj_ijk and
  * full_diff_stab_ijk in place of full_diffusion_ijk, the lumped solve and the

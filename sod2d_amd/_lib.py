@@ -87,6 +87,8 @@ _SIGS = {
     "sodhip_rk_stage_next_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                                  + [C.POINTER(StageParams), D, D, D, D] + [P] * 9
                                  + [D, P, P, P]),
+    "sodhip_rk_stage_noalloc_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
+                                    + [C.POINTER(StageParams), D, D, D, D] + [P] * 9),
     "sodhip_rk_stage_lps_f64": (I, [P, I, I, I, I, P, P] + [P] * 10 + [C.POINTER(StageParams)]
                                 + [D] * 5 + [P] * 20 + [D, P, P, P]),
     "sodhip_adapt_dt_cfl_f64": (I, [P, I, I, I, P, P, P, P, D, D, P, P, P, P, C.POINTER(D)]),

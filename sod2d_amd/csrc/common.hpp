@@ -33,6 +33,11 @@ int set_error(int code, const char *fmt, ...);
 // ---------------------------------------------------------------- element sizes
 constexpr int kMaxP = 4; // device kernels are instantiated for porder 2..4
 
+// size of the allocation-free stage's record scratch, per GPU
+#ifndef SODHIP_NOALLOC_SCRATCH_MIB
+#define SODHIP_NOALLOC_SCRATCH_MIB 64
+#endif
+
 inline int nnode_of(int p) { return (p + 1) * (p + 1) * (p + 1); }
 
 // ---------------------------------------------------------------- device buffers
@@ -112,6 +117,14 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
              MeshCache **out);
 // working-node incidence of mesh m for the list lpoin_w (1-based, npoin_w entries)
 int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w);
+// lookup only (allocates nothing): the cached tables of (connec, porder, nelem,
+// npoin) whose working incidence was built for lpoin_w; an error if a call that
+// may allocate has not built them yet
+int find_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, int npoin_w,
+              const int *lpoin_w, const MeshCache **out);
+// maps the allocation-free stage's scratch array on `device` (the current
+// device), once per device (ops_stage.hip)
+int noalloc_init(int device);
 // updateF's EOS on the working nodes of m (after get_working) and Sutherland's
 // law with a uniform factor on all its nodes, in one pass (ops_node.hip)
 int eos_stage_sutherland(sodhip_ctx ctx, const MeshCache *m, double gamma_gas, double Rgas,

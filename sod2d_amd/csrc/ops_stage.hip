@@ -14,6 +14,8 @@
 //      unless the caller asks for it (the reference keeps it for the next step's
 //      first stage).
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -190,6 +192,33 @@ template int gather_update<double>(sodhip_ctx, const MeshCache *, int, int, cons
                                    const double *, double *, double *, double *, double *,
                                    double *, double *, const NextStage *);
 
+// One fused stage on the mesh tables m (with its working incidence built) and
+// the record scratch scr of scr_bytes
+template <typename T>
+static int rk_stage_on(sodhip_ctx ctx, int porder, const MeshCache *m, int npoin, int npoin_w,
+                       const int *lpoin_w, const T *He, const T *gpvol, const T *dl,
+                       const int *inv, const int *gI, const int *gJ, const int *gK, const T *mu_e,
+                       const double *invMl, const sodhip_stage_params *prm, double dt,
+                       double lam1, double lams, double gam, const double *rho1, const double *q1,
+                       const double *E1, double *rho, double *q, double *E, double *Rmass,
+                       double *Rmom, double *Rener, const NextStage *next, void *scr_mem,
+                       size_t scr_bytes)
+{
+    const size_t EN = (size_t)m->nelem * nnode_of(porder);
+    const int rw = fused_record_width(porder, (int)sizeof(T));
+    if (rw * EN * sizeof(T) > scr_bytes)
+        return set_error(SODHIP_ERR_ARG, "the stage needs %zu bytes of record scratch, %zu available",
+                         rw * EN * sizeof(T), scr_bytes);
+    T *scr = (T *)scr_mem;
+    int rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
+                                   prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
+                                   prm->c_rho, prm->c_ener, prm->p_ref, scr);
+    if (rc) return rc;
+    SODHIP_TRY_HIP(hipGetLastError());
+    return gather_update<T>(ctx, m, npoin, npoin_w, lpoin_w, scr, rw, EN, invMl, prm->fact, dt,
+                            lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
+}
+
 template <typename T>
 static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
                     const int *connec, const int *lpoin_w, const T *He, const T *gpvol,
@@ -206,14 +235,42 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     const size_t EN = (size_t)nelem * nnode_of(porder);
     const int rw = fused_record_width(porder, (int)sizeof(T));
     if ((rc = ensure_buf(ctx->scratch, rw * EN * sizeof(T)))) return rc;
-    T *scr = (T *)ctx->scratch.ptr;
-    rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
-                               prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
-                               prm->c_rho, prm->c_ener, prm->p_ref, scr);
-    if (rc) return rc;
-    SODHIP_TRY_HIP(hipGetLastError());
-    return gather_update<T>(ctx, m, npoin, npoin_w, lpoin_w, scr, rw, EN, invMl, prm->fact, dt,
-                            lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
+    return rk_stage_on<T>(ctx, porder, m, npoin, npoin_w, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK,
+                          mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass,
+                          Rmom, Rener, next, ctx->scratch.ptr, ctx->scratch.bytes);
+}
+
+// ---------------------------------------------------------------- allocation-free stage
+// The allocation-free stage keeps its residual records in this file-scope
+// device array: one copy per GPU, created when the library's code object is
+// loaded on that GPU and held until the process ends.  In host code the name
+// `scratch` is a host-side stand-in, not a device address; noalloc_init, run
+// by sodhip_create with the context's device current, obtains each GPU's
+// address of its copy once, and only that address is ever passed to kernels.
+// Every call on one GPU shares the array and it keeps the last call's records
+// (nothing in it has to start at zero: every record is written before it is
+// read); calls that overlap on one GPU race on it.
+constexpr size_t kNoAllocScratchBytes = (size_t)SODHIP_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_noalloc_mu;
+static std::map<int, void *> g_noalloc_ptr; // device -> address of its copy of scratch
+
+int noalloc_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr.count(device)) return SODHIP_OK;
+    void *ptr = nullptr;
+    SODHIP_TRY_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SODHIP_REQUIRE(ptr, "no device address for the allocation-free scratch on device %d", device);
+    g_noalloc_ptr[device] = ptr;
+    return SODHIP_OK;
+}
+
+static void *noalloc_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    return it == g_noalloc_ptr.end() ? nullptr : it->second;
 }
 
 // ---------------------------------------------------------------- CFL time step
@@ -509,6 +566,56 @@ extern "C" int sodhip_rk_stage_next_f64(
                                prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom,
                                Rener, &nx);
     return set_error(SODHIP_ERR_ARG, "op_precision must be 8 or 4 (got %d)", op_precision);
+}
+
+extern "C" int sodhip_rk_stage_noalloc_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, int op_precision, const void *He, const void *gpvol,
+    const void *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+    const int *gmshAtoK, const void *mu_e, const double *invMl, const sodhip_stage_params *prm,
+    double dt, double lam1, double lams, double gam, const double *rho1, const double *q1,
+    const double *E1, double *rho, double *q, double *E, double *Rmass, double *Rmom,
+    double *Rener)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && He && gpvol && dlxigp_ip && invAtoIJK && gmshAtoI &&
+                       gmshAtoJ && gmshAtoK && mu_e && invMl && rho1 && q1 && E1 && rho && q && E,
+                   "null array argument");
+    SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
+    SODHIP_REQUIRE(op_precision == 8 || op_precision == 4,
+                   "op_precision must be 8 or 4 (got %d)", op_precision);
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    void *scr = noalloc_ptr(ctx->device);
+    SODHIP_REQUIRE(scr, "no allocation-free scratch on device %d", ctx->device);
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    const size_t need = fused_record_width(porder, op_precision) * EN * (size_t)op_precision;
+    SODHIP_REQUIRE(need <= kNoAllocScratchBytes,
+                   "the allocation-free stage holds at most %zu bytes of records (%d MiB); this "
+                   "mesh needs %zu (nelem = %d at porder %d)",
+                   kNoAllocScratchBytes, SODHIP_NOALLOC_SCRATCH_MIB, need, nelem, porder);
+    const MeshCache *m = nullptr;
+    int rc = find_mesh(ctx, porder, nelem, npoin, connec, npoin_w, lpoin_w, &m);
+    if (rc) return rc;
+    // launch timing may create events: it is off for this call
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    if (op_precision == 8)
+        rc = rk_stage_on<double>(ctx, porder, m, npoin, npoin_w, lpoin_w, (const double *)He,
+                                 (const double *)gpvol, (const double *)dlxigp_ip, invAtoIJK,
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, (const double *)mu_e, invMl, prm,
+                                 dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
+                                 nullptr, scr, kNoAllocScratchBytes);
+    else
+        rc = rk_stage_on<float>(ctx, porder, m, npoin, npoin_w, lpoin_w, (const float *)He,
+                                (const float *)gpvol, (const float *)dlxigp_ip, invAtoIJK,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e, invMl, prm, dt,
+                                lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
+                                nullptr, scr, kNoAllocScratchBytes);
+    ctx->prof = prof;
+    return rc;
 }
 
 extern "C" int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,

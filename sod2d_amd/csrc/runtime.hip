@@ -141,6 +141,23 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     return SODHIP_OK;
 }
 
+int find_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, int npoin_w,
+              const int *lpoin_w, const MeshCache **out)
+{
+    for (auto &m : ctx->meshes)
+        if (m->connec_key == connec && m->porder == porder && m->nelem == nelem &&
+            m->npoin == npoin) {
+            if (m->lpw_key != lpoin_w || m->npoin_w != npoin_w || !m->wrow)
+                return set_error(SODHIP_ERR_ARG,
+                                 "no working-node tables for this lpoin_w: an allocating call "
+                                 "(sodhip_rk_stage_f64) must build them first");
+            *out = m.get();
+            return SODHIP_OK;
+        }
+    return set_error(SODHIP_ERR_ARG, "no mesh tables for this connec: an allocating call "
+                                     "(sodhip_rk_stage_f64) must build them first");
+}
+
 int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
 {
     if (m->lpw_key == lpoin_w && m->npoin_w == npoin_w && m->wrow) return SODHIP_OK;
@@ -249,6 +266,8 @@ extern "C" int sodhip_create(int device, sodhip_ctx *out)
     SODHIP_TRY_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     SODHIP_TRY_HIP(hipGetDeviceProperties(&prop, device));
+    int rc = noalloc_init(device);
+    if (rc) return rc;
     auto *c = new sodhip_ctx_s();
     c->device = device;
     c->stream = nullptr;
