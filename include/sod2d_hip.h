@@ -330,6 +330,23 @@ int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int
                                const double *E1, double *rho, double *q, double *E,
                                double *Rmass, double *Rmom, double *Rener);
 
+/* The node update of one stage of the classical RK4 integrator rk_4_main
+ * (time_integ.f90; compiled out in the reference by #if 0, restated from its
+ * loops), with the lumped-mass inverse fused in.  Over working nodes:
+ * R <- R*invMl, written back (lumped_solver_*, time_integ.f90:431-435; skipped
+ * when invMl is NULL); Rsum <- Rsum + b*R (:442-451; first != 0: Rsum starts
+ * from zero, as the step's initialisation :234-240 leaves it); then, unless
+ * last, the next stage's state aux = y - dt*a_next*R (:266-275), or, at the
+ * last stage, y <- y - dt*Rsum (:460-468).  R has the sign of the operators
+ * called without fact (aux_fact = 1): y advances by -dt*R.  aux must not alias
+ * y; aux may be NULL at the last stage. */
+int sodhip_rk4_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                const double *invMl, double dt, double a_next, double b,
+                                int first, int last, double *rho, double *q, double *E,
+                                double *aux_rho, double *aux_q, double *aux_E, double *Rmass,
+                                double *Rmom, double *Rener, double *Rmass_sum,
+                                double *Rmom_sum, double *Rener_sum);
+
 /* ------------------------------------------------------------------------ */
 /* Nodal pieces of the time step (FP64, device pointers)                    */
 /* ------------------------------------------------------------------------ */
@@ -440,6 +457,28 @@ int sodhip_rk_stage_next_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, i
                              double *rho, double *q, double *E, double *Rmass, double *Rmom,
                              double *Rener, double gam_next, double *rho_next, double *q_next,
                              double *E_next);
+
+/* One stage of the classical RK4 integrator rk_4_main (time_integ.f90:250-470,
+ * compiled out in the reference by #if 0) with the RHS fused in: at the stage
+ * state (srho, sq, sE) -- y itself at the first stage, where a_1 = 0 -- it forms
+ * the primitives and Sutherland's viscosity, evaluates R = fact*(full_convec_ijk
+ * with the enthalpy slot + full_diffusion_ijk, mu_sgs = 0) (the reference's
+ * rk_4_main calls them without fact: prm->fact = 1), applies the lumped
+ * inverse, and performs the node update of sodhip_rk4_stage_update_f64 with
+ * the stage's a_next (a_{i+1}) and b (b_i).  The stage state may be the aux
+ * arrays themselves (the update reads every node's state in an earlier
+ * launch).  op_precision as sodhip_rk_stage_f64.  Rmass/Rmom/Rener (all or
+ * none) receive R*invMl. */
+int sodhip_rk4_stage_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                         const int *connec, const int *lpoin_w, int op_precision, const void *He,
+                         const void *gpvol, const void *dlxigp_ip, const int *invAtoIJK,
+                         const int *gmshAtoI, const int *gmshAtoJ, const int *gmshAtoK,
+                         const void *mu_e, const double *invMl, const sodhip_stage_params *prm,
+                         double dt, double a_next, double b, int first, int last, double *rho,
+                         double *q, double *E, const double *srho, const double *sq,
+                         const double *sE, double *aux_rho, double *aux_q, double *aux_E,
+                         double *Rmass_sum, double *Rmom_sum, double *Rener_sum, double *Rmass,
+                         double *Rmom, double *Rener);
 
 /* sodhip_rk_stage_f64 without any device allocation (same arguments, same
  * kernels, same results).  No counterpart in the reference: it is for callers

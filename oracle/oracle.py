@@ -438,6 +438,39 @@ def ls_rk_step(update_f, combine, L, G, stages, dt, first, parity, on_first=None
     return n
 
 
+def rk4_step(update_f, node_update, a, b, order=4):
+    """The stage sequence of one step of the classical RK4 integrator rk_4_main
+    (time_integ.f90:250-470; the reference compiles it out with ``#if 0`` at
+    :197 and :561, so it is restated from its loops).  ``update_f(i)`` stores
+    the residual R of stage i at that stage's state -- y itself at i = 0, where
+    a_1 = 0 and R was zeroed (:234-240, :266-275) -- with the mass inverse
+    applied (:372-435); ``node_update(a_next, b_i, first, last)`` is
+    `rk4_update`'s accumulation and update.  R has the sign of the operators
+    called without ``fact``: y advances by -dt * sum b_i R."""
+    for i in range(order):
+        update_f(i)
+        last = i == order - 1
+        node_update(0.0 if last else a[i + 1], b[i], i == 0, last)
+
+
+def rk4_update(lpoin_w, dt, a_next, b, first, last, y, aux, R, Rsum):
+    """The node loops of an rk_4_main stage over the working nodes (the
+    reference loops over all npoin; with the working connectivity R is zero
+    elsewhere): Rsum = Rsum + b R (time_integ.f90:442-451, zero at the first
+    stage, :234-240), then aux = y - dt a_next R (:266-275) or, at the last
+    stage, y = y - dt Rsum (:460-468).  y, aux, R, Rsum are tuples of
+    (rho, q, E) arrays; q is (npoin, 3)."""
+    lw = lpoin_w - 1
+    for f in range(3):
+        r = R[f][lw]
+        s = (0.0 if first else Rsum[f][lw]) + b * r
+        Rsum[f][lw] = s
+        if last:
+            y[f][lw] = y[f][lw] - dt * s
+        else:
+            aux[f][lw] = y[f][lw] - (dt * a_next) * r
+
+
 def entropy_residual(lpoin_w, Reta, eta, dt, factor_comp=1.0):
     """The end-of-step entropy residual (time_integ_ls.f90:436-444):
     auxReta = 1.5 Reta(:,2) - 0.5 Reta(:,1) + factor_comp (eta(:,2) - eta(:,1))/dt
@@ -495,8 +528,9 @@ class TGVOracle:
 
     def __init__(self, p, geo, ic, t, stages=5, variant=1, parity=True, dt_mode="adaptive",
                  cfl=(0.5, 0.5), sensor=True, sensor_path="spectral", nthreads=1, lps=False,
-                 high_mach=False):
+                 high_mach=False, integrator="ls"):
         self.p = p
+        self.integrator = integrator
         self.nthreads = nthreads
         self.lps = lps
         self.high_mach = high_mach
@@ -601,7 +635,59 @@ class TGVOracle:
             ducros_visc_limit(self.T, self.connec, u, divU, vorti, g.Ml, g.helem_l, self.mu_e,
                               0.1)
 
+    def update_f_rk4(self, rho, q, E):
+        """The RHS of an rk_4_main stage (time_integ.f90:276-435) at the state
+        (rho, q, E): EOS on the working nodes, Sutherland, full_diffusion_ijk
+        and full_convec_ijk without ``fact`` (aux_fact = 1), their sum and the
+        lumped solve, into Rmass/Rmom/Rener."""
+        lw, t, g = self.lpoin_w, self.t, self.geo
+        u, e_int, pr, Tem = fzeros((self.npoin, 3)), np.zeros(self.npoin), np.zeros(self.npoin), \
+            np.zeros(self.npoin)
+        eos_stage(lw, t.gamma_gas, t.Rgas, rho, q, E, u, e_int, pr, self.aux_h, Tem)
+        sutherland(Tem, self.mu_factor, self.mu_fluid)
+        Rd = [np.zeros(self.npoin), fzeros((self.npoin, 3)), np.zeros(self.npoin)]
+        full_diffusion_ijk(self.T, self.connec, g.He, g.gpvol, t.Cp, t.Prt, rho, rho, u, Tem,
+                           self.mu_fluid, self.mu_e, self.mu_sgs, *Rd, True, 1.0,
+                           nthreads=self.nthreads)
+        full_convec_ijk(self.T, self.connec, g.He, g.gpvol, u, q, rho, pr, self.aux_h,
+                        self.Rmass, self.Rmom, self.Rener, True, 1.0, nthreads=self.nthreads)
+        self.Rmass += Rd[0]
+        self.Rener += Rd[2]
+        self.Rmom += Rd[1]
+        lumped_solver_scal(lw, g.invMl, self.Rmass)
+        lumped_solver_scal(lw, g.invMl, self.Rener)
+        lumped_solver_vect(lw, g.invMl, self.Rmom)
+
+    def step_rk4(self):
+        """One step of rk_4_main (time_integ.f90:250-470) for the TGV: the four
+        stages with the entropy viscosity held (the step's per-stage entropy
+        residual and end-of-step sensor are not restated), then the EOS and
+        Sutherland of the new state (:489-533)."""
+        a, b, _ = rk4_coeffs()
+        y = (self.rho[1], self.q[1], self.E[1])
+        aux = tuple(np.array(v, order="F", copy=True) for v in y)
+        Rsum = (np.zeros(self.npoin), fzeros((self.npoin, 3)), np.zeros(self.npoin))
+        R = (self.Rmass, self.Rmom, self.Rener)
+
+        def update_f(i):
+            st = y if i == 0 else aux
+            self.update_f_rk4(*st)
+
+        def node_update(a_next, bi, first, last):
+            rk4_update(self.lpoin_w, self.dt, a_next, bi, first, last, y, aux, R, Rsum)
+
+        rk4_step(update_f, node_update, a, b)
+        t = self.t
+        eos_step(self.lpoin_w, t.gamma_gas, t.Rgas, self.rho[1], self.q[1], self.E[1], self.u[1],
+                 self.e_int[1], self.pr[1], self.csound, self.machno, self.Tem[1])
+        sutherland(self.Tem[1], self.mu_factor, self.mu_fluid)
+        self.time += self.dt
+        if self.dt_mode == "adaptive":
+            self.dt = self.adapt_dt()
+
     def step(self):
+        if self.integrator == "rk4":
+            return self.step_rk4()
         for lv in (self.rho, self.q, self.E, self.u, self.pr, self.Tem, self.e_int):
             lv[0][...] = lv[1]
         self.eta[:, 0] = self.eta[:, 1]

@@ -87,6 +87,9 @@ _SIGS = {
     "sodhip_rk_stage_next_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                                  + [C.POINTER(StageParams), D, D, D, D] + [P] * 9
                                  + [D, P, P, P]),
+    "sodhip_rk4_stage_update_f64": (I, [P, I, I, P, P, D, D, D, I, I] + [P] * 12),
+    "sodhip_rk4_stage_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
+                             + [C.POINTER(StageParams), D, D, D, I, I] + [P] * 15),
     "sodhip_rk_stage_noalloc_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                                     + [C.POINTER(StageParams), D, D, D, D] + [P] * 9),
     "sodhip_rk_stage_lps_f64": (I, [P, I, I, I, I, P, P] + [P] * 10 + [C.POINTER(StageParams)]

@@ -127,6 +127,84 @@ extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w
     return SODHIP_OK;
 }
 
+namespace sodhip {
+// The node update of a classical RK4 stage of rk_4_main (time_integ.f90,
+// compiled out in the reference by #if 0) on assembled residuals: R <- R*invMl
+// (lumped_solver_*, :431-435; skipped when invMl is NULL), Rsum <- Rsum + b R
+// (:442-451, FIRST: Rsum starts from zero, :234-240), then the next stage's
+// state aux = y - dt a_next R (:266-275) or, at the LAST stage,
+// y <- y - dt Rsum (:460-468), over the working nodes.
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_rk4_update(
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, const double *__restrict__ invMl,
+    double dta, double b, double dt, double *__restrict__ rho, double *__restrict__ q,
+    double *__restrict__ E, double *__restrict__ arho, double *__restrict__ aq,
+    double *__restrict__ aE, double *__restrict__ Rmass, double *__restrict__ Rmom,
+    double *__restrict__ Rener, double *__restrict__ Smass, double *__restrict__ Smom,
+    double *__restrict__ Sener)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const size_t k[5] = {(size_t)n, (size_t)n, n + (size_t)npoin, n + 2 * (size_t)npoin,
+                         (size_t)n};
+    double *yv[5] = {rho, q, q, q, E};
+    double *av[5] = {arho, aq, aq, aq, aE};
+    double *rv[5] = {Rmass, Rmom, Rmom, Rmom, Rener};
+    double *sv[5] = {Smass, Smom, Smom, Smom, Sener};
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        double r = rv[f][k[f]];
+        if (invMl) {
+            r = r * invMl[n];
+            rv[f][k[f]] = r;
+        }
+        const double sn = (FIRST ? 0.0 : sv[f][k[f]]) + b * r;
+        sv[f][k[f]] = sn;
+        if constexpr (LAST)
+            yv[f][k[f]] = yv[f][k[f]] - dt * sn;
+        else
+            av[f][k[f]] = yv[f][k[f]] - dta * r;
+    }
+}
+} // namespace sodhip
+
+extern "C" int sodhip_rk4_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                           const int *lpoin_w, const double *invMl, double dt,
+                                           double a_next, double b, int first, int last,
+                                           double *rho, double *q, double *E, double *aux_rho,
+                                           double *aux_q, double *aux_E, double *Rmass,
+                                           double *Rmom, double *Rener, double *Rmass_sum,
+                                           double *Rmom_sum, double *Rener_sum)
+{
+    SODHIP_REQUIRE(ctx && lpoin_w && rho && q && E && Rmass && Rmom && Rener && Rmass_sum &&
+                       Rmom_sum && Rener_sum,
+                   "null argument");
+    SODHIP_REQUIRE(last || (aux_rho && aux_q && aux_E), "the next stage's state is needed");
+    SODHIP_REQUIRE(last || (aux_rho != rho && aux_q != q && aux_E != E),
+                   "the next stage's state must not overwrite y");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    if (npoin_w == 0) return SODHIP_OK;
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    int tok = prof_begin(ctx, 3);
+#define SODHIP_RK4U(FI, LA)                                                                    \
+    k_rk4_update<FI, LA><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(                      \
+        npoin, npoin_w, lpoin_w, invMl, dt * a_next, b, dt, rho, q, E, aux_rho, aux_q, aux_E,   \
+        Rmass, Rmom, Rener, Rmass_sum, Rmom_sum, Rener_sum)
+    if (first && last)
+        SODHIP_RK4U(true, true);
+    else if (first)
+        SODHIP_RK4U(true, false);
+    else if (last)
+        SODHIP_RK4U(false, true);
+    else
+        SODHIP_RK4U(false, false);
+#undef SODHIP_RK4U
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
 // ---------------------------------------------------------------- EOS and friends
 namespace sodhip {
 

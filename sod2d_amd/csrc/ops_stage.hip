@@ -144,6 +144,70 @@ __global__ __launch_bounds__(256) void k_gather_update(
     }
 }
 
+// Classical RK4 stage of rk_4_main (time_integ.f90, compiled out in the
+// reference by #if 0): per working node the partials are gathered as in
+// k_gather_update, scaled by fact and the lumped inverse (R, lumped_solver_*,
+// :431-435), accumulated Rsum <- Rsum + b_i R (:442-451; FIRST: Rsum was
+// zeroed at the start of the step, :234-240), and then either the next
+// stage's state aux = y - dt a_{i+1} R (:266-275) or, at the LAST stage, the
+// step update y <- y - dt Rsum (:460-468).
+template <typename TS, int RW, bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_gather_rk4(
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, const int *__restrict__ wrow,
+    const int *__restrict__ wsrc, const TS *__restrict__ scr, const double *__restrict__ invMl,
+    double fact, double dta, double bi, double dt, double *__restrict__ rho,
+    double *__restrict__ q, double *__restrict__ E, double *__restrict__ arho,
+    double *__restrict__ aq, double *__restrict__ aE, double *__restrict__ Smass,
+    double *__restrict__ Smom, double *__restrict__ Sener, double *__restrict__ Rmass,
+    double *__restrict__ Rmom, double *__restrict__ Rener)
+{
+    const int per = gridDim.x >> 3; // grid is a multiple of 8
+    const int blk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int i = blk * blockDim.x + threadIdx.x;
+    if (i >= npoin_w) return;
+    const int n = lpoin_w[i] - 1;
+    const int j0 = wrow[i], j1 = wrow[i + 1];
+    const size_t k[5] = {(size_t)n, (size_t)n, n + (size_t)npoin, n + 2 * (size_t)npoin,
+                         (size_t)n};
+    double *yv[5] = {rho, q, q, q, E};
+    double *av[5] = {arho, aq, aq, aq, aE};
+    double *sv[5] = {Smass, Smom, Smom, Smom, Sener};
+    double *rv[5] = {Rmass, Rmom, Rmom, Rmom, Rener};
+    const double m = invMl[n];
+    double y[5], sold[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        y[f] = yv[f][k[f]];
+        sold[f] = FIRST ? 0.0 : sv[f][k[f]];
+    }
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = j0; j < j1; j += 4) {
+        int sj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? wsrc[j + u] : -1;
+        double v[4][5];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) load_record<TS, RW>(scr, max(sj[u], 0), v[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (sj[u] >= 0) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) acc[f] += fact * v[u][f];
+            }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        const double r = acc[f] * m;
+        const double sn = sold[f] + bi * r;
+        sv[f][k[f]] = sn;
+        if (Rmass) rv[f][k[f]] = r;
+        if constexpr (LAST)
+            yv[f][k[f]] = y[f] - dt * sn;
+        else
+            av[f][k[f]] = y[f] - dta * r;
+    }
+}
+
 // the gather/update of one stage from the element records scr (5 values per
 // element-node, type TS) over the working incidence of m
 template <typename TS>
@@ -238,6 +302,56 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     return rk_stage_on<T>(ctx, porder, m, npoin, npoin_w, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK,
                           mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass,
                           Rmom, Rener, next, ctx->scratch.ptr, ctx->scratch.bytes);
+}
+
+// One classical RK4 stage (see k_gather_rk4): the fused RHS at the stage state
+// (srho, sq, sE), then the gather with the RK4 node update
+template <typename T>
+static int rk4_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                     const int *connec, const int *lpoin_w, const T *He, const T *gpvol,
+                     const T *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                     const T *mu_e, const double *invMl, const sodhip_stage_params *prm, double dt,
+                     double a_next, double b, int first, int last, double *rho, double *q,
+                     double *E, const double *srho, const double *sq, const double *sE,
+                     double *arho, double *aq, double *aE, double *Smass, double *Smom,
+                     double *Sener, double *Rmass, double *Rmom, double *Rener)
+{
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
+    const size_t EN = (size_t)nelem * nnode_of(porder);
+    const int rw = fused_record_width(porder, (int)sizeof(T));
+    if ((rc = ensure_buf(ctx->scratch, rw * EN * sizeof(T)))) return rc;
+    T *scr = (T *)ctx->scratch.ptr;
+    rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, srho, sq, sE, mu_e,
+                               prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
+                               prm->c_rho, prm->c_ener, prm->p_ref, scr);
+    if (rc) return rc;
+    int tok = prof_begin(ctx, 1);
+    const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
+#define SODHIP_RK4G(RWV, FI, LA)                                                               \
+    k_gather_rk4<T, RWV, FI, LA><<<grid, 256, 0, ctx->stream>>>(                               \
+        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, invMl, prm->fact, dt * a_next, b, dt,   \
+        rho, q, E, arho, aq, aE, Smass, Smom, Sener, Rmass, Rmom, Rener)
+#define SODHIP_RK4F(RWV)                                                                       \
+    if (first && last)                                                                         \
+        SODHIP_RK4G(RWV, true, true);                                                          \
+    else if (first)                                                                            \
+        SODHIP_RK4G(RWV, true, false);                                                         \
+    else if (last)                                                                             \
+        SODHIP_RK4G(RWV, false, true);                                                         \
+    else                                                                                       \
+        SODHIP_RK4G(RWV, false, false)
+    if (rw == 6)
+        SODHIP_RK4F(6);
+    else
+        SODHIP_RK4F(5);
+#undef SODHIP_RK4F
+#undef SODHIP_RK4G
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
 }
 
 // ---------------------------------------------------------------- allocation-free stage
@@ -616,6 +730,46 @@ extern "C" int sodhip_rk_stage_noalloc_f64(
                                 nullptr, scr, kNoAllocScratchBytes);
     ctx->prof = prof;
     return rc;
+}
+
+extern "C" int sodhip_rk4_stage_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, int op_precision, const void *He, const void *gpvol,
+    const void *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI, const int *gmshAtoJ,
+    const int *gmshAtoK, const void *mu_e, const double *invMl, const sodhip_stage_params *prm,
+    double dt, double a_next, double b, int first, int last, double *rho, double *q, double *E,
+    const double *srho, const double *sq, const double *sE, double *aux_rho, double *aux_q,
+    double *aux_E, double *Rmass_sum, double *Rmom_sum, double *Rener_sum, double *Rmass,
+    double *Rmom, double *Rener)
+{
+    SODHIP_REQUIRE(ctx && prm, "null argument");
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    SODHIP_REQUIRE(nelem > 0 && npoin > 0 && npoin_w > 0 && npoin_w <= npoin,
+                   "invalid mesh sizes");
+    SODHIP_REQUIRE(connec && lpoin_w && He && gpvol && dlxigp_ip && invAtoIJK && gmshAtoI &&
+                       gmshAtoJ && gmshAtoK && mu_e && invMl && rho && q && E && srho && sq &&
+                       sE && Rmass_sum && Rmom_sum && Rener_sum,
+                   "null array argument");
+    SODHIP_REQUIRE(last || (aux_rho && aux_q && aux_E), "the next stage's state is needed");
+    SODHIP_REQUIRE(last || (aux_rho != rho && aux_q != q && aux_E != E),
+                   "the next stage's state must not overwrite y");
+    SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    if (op_precision == 8)
+        return rk4_stage<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                                 (const double *)He, (const double *)gpvol,
+                                 (const double *)dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                                 gmshAtoK, (const double *)mu_e, invMl, prm, dt, a_next, b, first,
+                                 last, rho, q, E, srho, sq, sE, aux_rho, aux_q, aux_E, Rmass_sum,
+                                 Rmom_sum, Rener_sum, Rmass, Rmom, Rener);
+    if (op_precision == 4)
+        return rk4_stage<float>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w,
+                                (const float *)He, (const float *)gpvol, (const float *)dlxigp_ip,
+                                invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e,
+                                invMl, prm, dt, a_next, b, first, last, rho, q, E, srho, sq, sE,
+                                aux_rho, aux_q, aux_E, Rmass_sum, Rmom_sum, Rener_sum, Rmass,
+                                Rmom, Rener);
+    return set_error(SODHIP_ERR_ARG, "op_precision must be 8 or 4 (got %d)", op_precision);
 }
 
 extern "C" int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,

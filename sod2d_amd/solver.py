@@ -52,6 +52,11 @@ from .device import C_LPS_COMP, C_SPECIES_STAB, Context, _bind, _ptr
 
 F64, F32, I32 = torch.float64, torch.float32, torch.int32
 
+# classical RK4 tableau of rk_4_main (time_integ.f90:84-87): stage abscissae
+# a_i and weights b_i
+RK4_A = (0.0, 0.5, 0.5, 1.0)
+RK4_B = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)
+
 
 @dataclass
 class TGVParams:
@@ -117,8 +122,13 @@ class TGVRun:
                  op_precision: int = 8, parity: bool = True, dt_mode: str = "adaptive",
                  uniform_rho: bool = False, sensor: bool = True, ctx: Context | None = None,
                  geometry=None, initial=None, sensor_path: str = "spectral", lps: bool = False,
-                 high_mach: bool = False):
+                 high_mach: bool = False, integrator: str = "ls"):
         self.t = params or TGVParams()
+        if integrator not in ("ls", "rk4"):
+            raise SodhipError(f"unknown integrator {integrator!r}")
+        if integrator == "rk4" and lps:
+            raise SodhipError("the classical RK4 integrator runs the plain viscous path")
+        self.integrator = integrator
         self.ctx = ctx or Context(torch.cuda.current_device())
         self.p = porder
         self.parity = parity
@@ -187,6 +197,13 @@ class TGVRun:
             self.proj = [z(3 * n) for _ in range(5)]  # ProjMass, ProjEner, ProjMX, MY, MZ
             self.tau_stab = z(ne)
             self.aux_h = z(n)
+        if integrator == "rk4":
+            # the stage state (aux_*) and the residual sums of rk_4_main
+            self.aux = [z(n), z(3 * n), z(n)]
+            self.Rsum = [z(n), z(3 * n), z(n)]
+            self.sp4 = StageParams(self.t.gamma_gas, self.t.Rgas, self.t.Cp, self.t.Prt,
+                                   self.t.mu_factor, 1.0, 1.0, 1.0,
+                                   self.t.po if op_precision == 4 else 0.0)
         self.first = True
         self._next_dt = None  # dt with which the next step's stage 1 was stored
         self.fuse_next_stage = True  # at fixed dt, see the module docstring
@@ -295,9 +312,44 @@ class TGVRun:
             self._next_dt = self.dt
 
     # ------------------------------------------------------------ time step
+    def step_rk4(self):
+        """One step of the classical RK4 integrator rk_4_main (time_integ.f90:
+        250-470; compiled out in the reference by #if 0): four fused stages
+        (sodhip_rk4_stage_f64; operators without fact, so y advances by
+        -dt * sum b_i R), then the EOS of the new state and Sutherland
+        (:489-533).  The entropy viscosity mu_e is held: rk_4_main's per-stage
+        entropy residual and end-of-step sensor are not part of this path."""
+        a, b = RK4_A, RK4_B
+        y = (self.rho[1], self.q[1], self.E[1])
+        h = self.ctx.h
+        for i in range(4):
+            last = i == 3
+            st = y if i == 0 else self.aux
+            check(lib().sodhip_rk4_stage_f64(
+                h, self.p, self.nelem, self.npoin, self.npoin_w, _ptr(self.connec),
+                _ptr(self.lpoin_w), self.op_precision, _ptr(self.He_op), _ptr(self.gpvol_op),
+                _ptr(self.dl_op), *[_ptr(x) for x in self.tabs], _ptr(self.mu_e_op),
+                _ptr(self.invMl), C.byref(self.sp4), self.dt, 0.0 if last else a[i + 1], b[i],
+                int(i == 0), int(last), *[_ptr(x) for x in y], *[_ptr(x) for x in st],
+                *[_ptr(x) for x in self.aux], *[_ptr(x) for x in self.Rsum],
+                _ptr(self.Rmass), _ptr(self.Rmom), _ptr(self.Rener)), "rk4_stage")
+        t = self.t
+        check(lib().sodhip_eos_step_f64(
+            h, self.npoin, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas,
+            _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]), _ptr(self.u[1]),
+            _ptr(self.e_int[1]), _ptr(self.pr[1]), _ptr(self.csound), _ptr(self.machno),
+            _ptr(self.Tem[1])), "eos_step")
+        self._sutherland()
+        self.time += self.dt
+        self.istep += 1
+        if self.dt_mode == "adaptive":
+            self.dt = self.adapt_dt()
+
     def step(self):
         """One time step: evalTimeIteration's level copy, rk_4_ls_main, time += dt,
         then evalDt in adaptive mode."""
+        if self.integrator == "rk4":
+            return self.step_rk4()
         L, G, S = self.L, self.G, self.S
         # level 2 -> level 1.  The primitive levels are exchanged rather than
         # copied: level 2 of u, pr, Tem and e_int is rewritten by the EOS

@@ -346,3 +346,97 @@ def test_convection_is_kinetic_energy_preserving(oracle, p, n):
         assert np.abs(z0[0]).max() < 1e-12 and np.abs(z0[1]).max() < 1e-12
         Dm, Dq = _divergence_form(T, m.connec, g.He, g.gpvol, s["u"], s["q"], pr)
         assert abs(_ke_balance(m.lpoin_w, s["u"], Dm, Dq)) > 1e-3
+
+
+# ---------------------------------------------------------------- classical RK4
+def _rk4_amplification(oracle, A, dt, a=None, b=None, seq=None):
+    """The matrix M with y_{n+1} = M y_n of one oracle RK4 step on y' = A y,
+    column by column (R = -A y: rk_4_main's y advances by -dt sum b_i R)."""
+    a0, b0, _ = oracle.rk4_coeffs()
+    a = a0 if a is None else a
+    b = b0 if b is None else b
+    n = A.shape[0]
+    M = np.zeros((n, n))
+    lw = np.arange(1, n + 1, dtype=np.int32)
+    for j in range(n):
+        y = (np.eye(n)[:, j].copy(), np.zeros((n, 3), order="F"), np.zeros(n))
+        aux = tuple(np.zeros_like(v) for v in y)
+        R = tuple(np.zeros_like(v) for v in y)
+        Rs = tuple(np.zeros_like(v) for v in y)
+
+        def update_f(i):
+            st = y if i == 0 else aux
+            R[0][:] = -(A @ st[0])
+
+        def node_update(an, bi, first, last):
+            oracle.rk4_update(lw, dt, an, bi, first, last, y, aux, R, Rs)
+
+        (seq or oracle.rk4_step)(update_f, node_update, a, b)
+        M[:, j] = y[0]
+    return M
+
+
+def _rk4_poly(A, dt):
+    Z = A * dt
+    out, term = np.eye(A.shape[0]), np.eye(A.shape[0])
+    for k in range(1, 5):
+        term = term @ Z / k
+        out = out + term
+    return out
+
+
+@pytest.mark.parametrize("A", [np.array([[-1.3]]), np.array([[0.7]]),
+                               np.array([[0.0, -2.0], [2.0, 0.0]]),
+                               np.array([[-0.5, -1.5], [1.5, -0.5]])])
+def test_rk4_step_is_the_classical_fourth_order_polynomial(oracle, A):
+    """One step on y' = A y multiplies y by 1 + Z + Z^2/2 + Z^3/6 + Z^4/24
+    (Z = A dt), for real, imaginary and complex spectra, exactly to rounding;
+    the tableau is rk_4_main's (time_integ.f90:84-87)."""
+    for dt in (0.1, 0.37):
+        M = _rk4_amplification(oracle, A, dt)
+        assert np.abs(M - _rk4_poly(A, dt)).max() < 4e-16 * max(1.0, np.abs(M).max())
+
+
+def test_rk4_planted_mistakes_fail(oracle):
+    """A wrong weight, a wrong abscissa, or a stage built from the previous
+    stage state instead of y each move the amplification off the RK4
+    polynomial by far more than rounding."""
+    A = np.array([[-0.5, -1.5], [1.5, -0.5]])
+    dt = 0.3
+    ref = _rk4_poly(A, dt)
+    a, b, _ = oracle.rk4_coeffs()
+    bad_b = np.array([0.25, 0.25, 0.25, 0.25])
+    bad_a = np.array([0.0, 0.5, 1.0, 1.0])
+    assert np.abs(_rk4_amplification(oracle, A, dt, b=bad_b) - ref).max() > 1e-3
+    assert np.abs(_rk4_amplification(oracle, A, dt, a=bad_a) - ref).max() > 1e-3
+
+    def chained(update_f, node_update, a_, b_, order=4):
+        # stage state from the previous stage state: aux = aux - dt a R
+        for i in range(order):
+            update_f(i)
+            last = i == order - 1
+            node_update(0.0 if last else a_[i + 1] - (a_[i] if i else 0.0), b_[i], i == 0, last)
+
+    # a chain that reaches the same abscissae through increments is still not RK4
+    assert np.abs(_rk4_amplification(oracle, A, dt, seq=chained) - ref).max() > 1e-4
+
+
+def test_rk4_tgv_step_goes_through_the_sequencer(oracle, monkeypatch):
+    """TGVOracle with the classical integrator runs its stages through
+    oracle.rk4_step, once per step, with rk_4_main's tableau."""
+    g = make_case(3, 2, periodic=True, lengths=(2 * np.pi,) * 3)
+    t = oracle.tgv_params()
+    ic = oracle.tgv_ic(g.mesh.coord, t)
+    orc = oracle.TGVOracle(3, g, ic, t, integrator="rk4", dt_mode="fixed")
+    calls = []
+    real = oracle.rk4_step
+
+    def spy(update_f, node_update, a, b, order=4):
+        calls.append((tuple(a), tuple(b)))
+        return real(update_f, node_update, a, b, order)
+
+    monkeypatch.setattr(oracle, "rk4_step", spy)
+    orc.step()
+    orc.step()
+    assert calls == [((0.0, 0.5, 0.5, 1.0), (1 / 6, 1 / 3, 1 / 3, 1 / 6))] * 2
+    assert np.isfinite(orc.rho[1]).all()
