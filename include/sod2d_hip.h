@@ -88,6 +88,26 @@ int sodhip_lumped_mass(int porder, int nelem, int npoin, const int *connec_work,
 int sodhip_char_length(int porder, int nelem, int npoin, const int *connec_orig,
                        const double *coord, double *helem, double *helem_l);
 
+/* Element clusters of a p=3 mesh (host arrays; set-up, no device needed), the
+ * tables of the brick-local assembly the fused FP64 stage uses: elements are
+ * grouped greedily into clusters of up to 8 (2x2x2 bricks on a box mesh
+ * numbered along i, j, k), each with at most 344 distinct nodes.  No
+ * counterpart in the reference (its assembly is an atomic scatter,
+ * elem_convec.f90:451-465).  connec(nelem,64) 1-based column-major; lpoin_w
+ * 1-based.  sizes[0..3] receive ncl, nloc (local nodes over all clusters),
+ * ns (working nodes finished by the surface gather) and the number of their
+ * records; each non-NULL array receives its table: cel(ncl*8) element ids
+ * (0-based, -1 = empty slot, ascending within a cluster), cloc(ncl*8*64)
+ * the local index of the node at each lexicographic position of each slot's
+ * element, cnb(ncl+1) local-node offsets, cnode(nloc) 0-based node ids
+ * (ascending within a cluster), ckind(nloc) 0 interior working node / 1
+ * surface working node / 2 not working, snode(ns) 1-based nodes in working-list
+ * order, srow(ns+1), ssrc (records cnb[c]+l, ascending cluster per node).
+ * SODHIP_ERR_UNSUPPORTED when an element lists a node twice. */
+int sodhip_element_clusters(int porder, int nelem, int npoin, const int *connec, int npoin_w,
+                            const int *lpoin_w, int *sizes, int *cel, int *cloc, int *cnb,
+                            int *cnode, int *ckind, int *snode, int *srow, int *ssrc);
+
 /* ------------------------------------------------------------------------ */
 /* Device context                                                            */
 /* ------------------------------------------------------------------------ */

@@ -44,6 +44,7 @@ _SIGS = {
     "sodhip_elem_jacobian": (I, [I, I, I, P, P, P, P]),
     "sodhip_lumped_mass": (I, [I, I, I, P, P, I, P, P, P]),
     "sodhip_char_length": (I, [I, I, I, P, P, P, P]),
+    "sodhip_element_clusters": (I, [I, I, I, P, I, P] + [P] * 9),
     "sodhip_create": (I, [I, P]),
     "sodhip_destroy": (I, [P]),
     "sodhip_set_stream": (I, [P, P]),

@@ -57,6 +57,31 @@ struct DevBuf {
 // perm groups an element's nodes by their (k, j, i) line indices so that the
 // four cache lines of an element hold one k-plane each); src[slot] is that
 // position, read by the per-node gathers in slot order.
+// Element clusters of a p=3 mesh for the brick-local assembly of the fused
+// stage: a block evaluates the (up to) kClusterG elements of a cluster in a
+// row and sums their residuals in LDS per local node.  Nodes all of whose
+// elements lie in the cluster ("interior") are finished by the element kernel
+// itself (mass inverse and stage combine); the others ("surface") get one
+// partial record per cluster that contains them, which the surface gather
+// sums in cluster order.
+constexpr int kClusterG = 8;      // elements per cluster (a 2x2x2 brick on a box mesh)
+constexpr int kClusterLMax = 344; // local nodes per cluster (343 for a 2x2x2 brick at p=3)
+constexpr int kClusterBlockMax = 16; // clusters one block of the element kernel walks
+struct ClusterTables {
+    int ncl = 0;          // clusters
+    int nloc = 0;         // local nodes over all clusters (= partial records)
+    int *cel = nullptr;   // ncl*kClusterG element ids in evaluation order, -1 = none
+    int *cloc = nullptr;  // (ncl*kClusterG, 64): local index of the node at each
+                          // lexicographic position of the element
+    int *cnb = nullptr;   // ncl+1: first local node of each cluster
+    int *cnode = nullptr; // nloc: global node (0-based), ascending within a cluster
+    int *ckind = nullptr; // nloc: 0 interior working node, 1 surface working node, 2 other
+    int ns = 0;           // working nodes finished by the surface gather
+    int *snode = nullptr; // ns: node ids, 1-based (working-list order)
+    int *srow = nullptr;  // ns+1
+    int *ssrc = nullptr;  // record indices (cnb[c] + l), ascending cluster per node
+};
+
 struct MeshCache {
     const int *connec_key = nullptr; // device pointer of the caller's connec
     int porder = 0, nelem = 0, npoin = 0;
@@ -65,7 +90,10 @@ struct MeshCache {
     int *src = nullptr;     // nelem*nnode, scratch position of each slot
     int *perm = nullptr;    // nnode, element-local scratch position of node a
     // host copies (for derived tables)
-    std::vector<int> h_row, h_src;
+    std::vector<int> h_row, h_src, h_cT;
+    std::vector<unsigned char> h_wflag; // working flags of the current list
+    // clusters of the current working list (p=3), built on first use
+    std::unique_ptr<ClusterTables> clusters;
     // the incidence restricted to a working-node list, in list order:
     // working node i owns wsrc[wrow[i]..wrow[i+1]-1]
     const int *lpw_key = nullptr; // device pointer of the caller's lpoin_w
@@ -106,6 +134,12 @@ struct sodhip_ctx_s {
     std::vector<hipEvent_t> event_pool;
     sodhip::ProfSlot slots[sodhip::kProfKinds];
     int num_cu = 256;
+    // brick-local assembly for the fused p=3 FP64 stage, opt-in
+    // (SODHIP_STAGE_CLUSTERS=1 in the environment at sodhip_create): its LDS
+    // accumulator costs the element kernel a quarter of its resident blocks,
+    // more than the smaller gather saves (measured 0.79 -> 1.50 ms element,
+    // 0.37 -> 0.29 ms gather per stage at 64^3)
+    bool cluster_stage = false;
 };
 
 namespace sodhip {
@@ -122,6 +156,10 @@ int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w);
 // may allocate has not built them yet
 int find_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, int npoin_w,
               const int *lpoin_w, const MeshCache **out);
+// the element clusters of mesh m for its current working list (p=3), built
+// on first use (allocates); find_clusters only looks them up
+int get_clusters(sodhip_ctx ctx, MeshCache *m, const ClusterTables **out);
+void free_clusters(MeshCache *m);
 // maps the allocation-free stage's scratch array on `device` (the current
 // device), once per device (ops_stage.hip)
 int noalloc_init(int device);
@@ -146,6 +184,25 @@ struct NextStage {
     double nl1, ngdt;
     double *rho, *q, *E;
 };
+// gather/update of one stage over the working nodes list[0..nw) (1-based)
+// with their records src[row[i]..row[i+1]) (ops_stage.hip)
+template <typename TS>
+int gather_update_on(sodhip_ctx ctx, int npoin, int nw, const int *list, const int *row,
+                     const int *src, const TS *scr, int rw, const double *invMl, double fact,
+                     double dt, double lam1, double lams, double gam, const double *rho1,
+                     const double *q1, const double *E1, double *rho, double *q, double *E,
+                     double *Rmass, double *Rmom, double *Rener, const NextStage *next);
+// the fused p=3 FP64 stage's element kernel in cluster mode: interior nodes
+// updated in place, surface partial records into rec (ops_element.hip)
+int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *cl,
+                      const double *He, const double *gpvol, const double *dl, const int *inv,
+                      const int *gI, const int *gJ, const int *gK, const double *mu_e,
+                      double gamma, double Rgas, double Cp, double Pr, double mu_factor,
+                      double c_rho, double c_ener, double p_ref, const double *invMl, double fact,
+                      double dt, double lam1, double lams, double gam, const double *rho1,
+                      const double *q1, const double *E1, double *rho, double *q, double *E,
+                      double *Rmass, double *Rmom, double *Rener, const NextStage *next,
+                      double *rec);
 // gather/update of one stage from element records of rw (5 or 6) values
 // (ops_stage.hip)
 template <typename TS>

@@ -102,6 +102,21 @@ template <typename T> struct RhsArgs {
     T *out;          // records of 5 (mass, mom1, mom2, mom3, energy) per element-node;
                      // (e,a) at record e*NN + perm[a]
     int geo16;       // He and gpvol blocks of every element are 16-byte aligned
+    // brick-local assembly of the fused p=3 stage (see ClusterTables): the
+    // element order, local-node tables, the partial records of surface nodes,
+    // and the stage update the element kernel applies to interior nodes
+    // (the arithmetic of k_gather_update)
+    const int *cel, *cloc, *cnb, *cnode, *ckind;
+    int ncl;
+    double *rec;
+    const double *invMl;
+    double fact, lam1, lams, gdt;
+    int rd1, rdy;
+    const double *y1r, *y1q, *y1E;
+    double *yr, *yq, *yE;
+    double *Rm, *Rq, *Re;
+    double nx_l1, nx_gdt;
+    double *nxr, *nxq, *nxE;
 };
 
 // Sutherland's law (mod_fluid_viscosity.f90:21-41)
@@ -875,7 +890,7 @@ __global__ __launch_bounds__(256) void k_gather_entropy(int npoin, const int *__
 // 20 % larger records cost the element kernel more than the gather gains.
 template <typename T> constexpr int split_rw() { return sizeof(T) == 4 ? 6 : 5; }
 
-template <int P, typename T, bool G16> struct SplitLDS {
+template <int P, typename T, bool G16, int RUNV = SODHIP_SPLIT_RUN> struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
     static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
     // geometry, He then gpvol, copied in 64-lane rows of GW bytes per lane
@@ -888,7 +903,7 @@ template <int P, typename T, bool G16> struct SplitLDS {
     // RUN values per node, into one of three blocks, two runs ahead (each
     // 64-lane row then reads 8 runs of 32 contiguous bytes instead of 32
     // scattered doubles); with FP32 operators per element into its slot.
-    static constexpr int RUN = SODHIP_SPLIT_RUN;
+    static constexpr int RUN = RUNV;
     static constexpr bool MUBLK = sizeof(T) == 8;
     static constexpr int ST = 5 * NN * 8, MU = MUBLK ? 0 : NN * (int)sizeof(T);
     static constexpr int SLOT = GEO + ST + MU;
@@ -898,7 +913,7 @@ template <int P, typename T, bool G16> struct SplitLDS {
     static constexpr int MUB_AT = AH * SLOT + CROWS * CROW;
     static constexpr int BYTES = MUB_AT + MUBS * MUB;
     static constexpr int MU_OPS = (int)sizeof(T) / 4;
-    static_assert(!MUBLK || (RUN == 4 && MUB == 256 * MU_OPS * RUN),
+    static_assert(!MUBLK || ((RUN == 4 || RUN == 2) && MUB == 256 * MU_OPS * RUN),
                   "one run's mu_e block in MU_OPS rows per element");
     // copies per element: wave 0 takes the connectivity row, the even
     // geometry rows and rho, q1, q2; wave 1 the odd rows, q3, E and mu_e
@@ -990,6 +1005,24 @@ __device__ __forceinline__ void mu_issue(const RhsArgs<T> &A, unsigned blk, int 
     }
 }
 
+// FP64 mu_e copies of one iteration for runs of two elements (the cluster
+// order, where elements pair up as e, e+1 on a box mesh): rows 2j and 2j+1 of
+// the block of the run of elements e0, e1 (j = iteration within its run).
+// The block is node-major and element-fastest: lane L moves half L&1 of
+// mu_e(e_{(L/2)&1}, 16i + L/4) of row i.
+template <typename T>
+__device__ __forceinline__ void mu_issue2(const RhsArgs<T> &A, unsigned blk, int e0, int e1, int j,
+                                          int lane)
+{
+    const int hw = 4 * (lane & 1), nd16 = lane >> 2;
+    const int e = ((lane >> 1) & 1) ? e1 : e0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = 2 * j + h;
+        dma_lds<4>((const char *)(A.mu_e + e + (size_t)A.nelem * (16 * i + nd16)) + hw, blk + i * 256);
+    }
+}
+
 // reads ring slot sl into registers: He, gpvol, the conserved fields and mu_e
 // of node a (lexicographic position pa)
 template <int P, typename T, bool G16>
@@ -1061,10 +1094,17 @@ __device__ __forceinline__ void split_store(const RhsArgs<T> &A, const void *den
             DIV1 += w1;                                                                    \
         }                                                                                  \
     } while (0)
-template <int P, typename T, bool G16>
+// CL: brick-local assembly (FP64).  The block walks element clusters, one
+// after the other; the viscous wave adds each element's residual records into
+// an LDS accumulator per cluster-local node instead of storing them, and after
+// a cluster's last element finishes its interior nodes (mass inverse and
+// stage combine, as k_gather_update) and stores one partial record per
+// surface node.  mu_e runs are then pairs of elements.
+template <int P, typename T, bool G16, bool CL = false>
 __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_SPLIT_WAVES_F32) void k_rhs_split(RhsArgs<T> A)
 {
-    using L = SplitLDS<P, T, G16>;
+    using L = SplitLDS<P, T, G16, CL ? 2 : SODHIP_SPLIT_RUN>;
+    static_assert(!CL || (sizeof(T) == 8 && P == 3), "cluster assembly: FP64, p=3");
     using V2 = T __attribute__((ext_vector_type(2)));
     // node records of REC values: 112 bytes in FP64, an odd multiple of 16,
     // so that 16 lanes reading 16 consecutive records with 16-byte reads hit
@@ -1082,23 +1122,56 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     __shared__ V2 s2[64 * GREC / 2];
     __shared__ __attribute__((aligned(16))) unsigned char ring[L::BYTES];
     const unsigned ring_lds = (unsigned)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)ring;
+    // cluster mode: the block's element order and local-node ranges, the
+    // local-index rows of the last AH + 1 elements, the accumulator
+    constexpr int G = kClusterG, CMAX = kClusterBlockMax, CR = AH + 1;
+    __shared__ int cel_s[CL ? CMAX * G : 1];
+    __shared__ int cb_s[CL ? 2 * CMAX : 1];
+    __shared__ __attribute__((aligned(16))) int cloc_s[CL ? CR * 64 : 1];
+    __shared__ double acc_s[CL ? 5 * kClusterLMax : 1];
+    const unsigned cloc_lds =
+        (unsigned)(uintptr_t)(__attribute__((address_space(3))) int *)cloc_s;
 
     const int role = threadIdx.x >> 6; // 0: convection, 1: viscous (wave-uniform)
     const int lane = threadIdx.x & 63;
     const bool act = true;
     const bool visc = role == 1;
 
-    constexpr int RUN = SODHIP_SPLIT_RUN;
-    const ElemRange er = elem_range((A.nelem + RUN - 1) / RUN); // over runs
+    constexpr int RUN = L::RUN, RSH = RUN == 4 ? 2 : 1;
+    // over runs of RUN elements, or over clusters
+    const ElemRange er = CL ? elem_range(A.ncl) : elem_range((A.nelem + RUN - 1) / RUN);
     if (er.begin >= er.end) return;
     const int rlast = er.begin + (er.end - 1 - er.begin) / er.stride * er.stride;
-    const int nit = RUN * ((er.end - er.begin + er.stride - 1) / er.stride);
+    const int nunits = (er.end - er.begin + er.stride - 1) / er.stride;
+    const int nit = (CL ? G : RUN) * nunits;
+    if constexpr (CL) {
+        // a padding slot (-1) of a cluster evaluates its first element again
+        // and adds nothing
+        for (int i = threadIdx.x; i < nunits * G; i += 128) {
+            const long c = er.begin + (long)(i / G) * er.stride;
+            cel_s[i] = A.cel[c * G + i % G];
+        }
+        for (int i = threadIdx.x; i < nunits; i += 128) {
+            const long c = er.begin + (long)i * er.stride;
+            cb_s[2 * i] = A.cnb[c];
+            cb_s[2 * i + 1] = A.cnb[c + 1];
+        }
+        for (int i = threadIdx.x; i < 5 * kClusterLMax; i += 128) acc_s[i] = 0.0;
+        __syncthreads();
+    }
     // element of iteration t.  Past the block's last run, and past nelem in
     // the last run, elements repeat: the copies stay unconditional and a
-    // repeated element stores the same records again.
+    // repeated element stores the same records again.  In cluster mode past
+    // the block's last element the copies repeat that element.
     auto elem_of = [&](int t) {
-        const long r = min((long)er.begin + (long)(t / RUN) * er.stride, (long)rlast);
-        return (int)min(r * RUN + t % RUN, (long)A.nelem - 1);
+        if constexpr (CL) {
+            const int i = min(t, nit - 1);
+            const int e = cel_s[i];
+            return e >= 0 ? e : cel_s[i - i % G];
+        } else {
+            const long r = min((long)er.begin + (long)(t / RUN) * er.stride, (long)rlast);
+            return (int)min(r * RUN + t % RUN, (long)A.nelem - 1);
+        }
     };
     const int elast = elem_of(nit - 1);
     // FP64 mu_e blocks: iteration t (t >= -2 RUN) copies its share of the
@@ -1106,13 +1179,27 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     // iteration RUN r on
     auto mu_at = [&](int t) {
         if constexpr (L::MUBLK) {
-            const int r = (t >> 2) + 2;
-            const long g = min((long)er.begin + (long)r * er.stride, (long)rlast);
-            mu_issue<T>(A, ring_lds + L::MUB_AT + (r % 3) * L::MUB, (int)(g * RUN), t & 3, lane);
+            const int r = (t >> RSH) + 2;
+            const unsigned blk = ring_lds + L::MUB_AT + (r % 3) * L::MUB;
+            if constexpr (CL) {
+                mu_issue2<T>(A, blk, elem_of(r * RUN), elem_of(r * RUN + 1), t & (RUN - 1), lane);
+            } else {
+                const long g = min((long)er.begin + (long)r * er.stride, (long)rlast);
+                mu_issue<T>(A, blk, (int)(g * RUN), t & 3, lane);
+            }
         }
     };
     auto mu_read = [&](int t, int node) {
-        return ((const T *)(ring + L::MUB_AT + ((t >> 2) % 3) * L::MUB))[node * RUN + (t & 3)];
+        return ((const T *)(ring + L::MUB_AT + ((t >> RSH) % 3) * L::MUB))[node * RUN + (t & (RUN - 1))];
+    };
+    // cluster mode: the local-index row of iteration t's element, copied by
+    // the viscous wave with its share of that element's inputs
+    auto cloc_issue = [&](int t) {
+        if constexpr (CL) {
+            const int i = min(t, nit - 1);
+            const long c = er.begin + (long)(i / G) * er.stride;
+            dma_lds<4>(A.cloc + ((size_t)c * G + i % G) * NN + lane, cloc_lds + (t % CR) * 256);
+        }
     };
     auto slot_of = [&](int t) { return (t % AH) * L::SLOT; };
     auto crow_of = [&](int t) { return AH * L::SLOT + (t % L::CROWS) * L::CROW; };
@@ -1161,6 +1248,7 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
                                       elem_of(t + AH), n0, n1, lane);
             mu_at(t - AH);
+            cloc_issue(t);
         } else
             split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
                                       elem_of(t + AH), n0, n1, lane);
@@ -1367,8 +1455,74 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
         for (int k = 0; k < 3; ++k)
 #pragma unroll
             for (int o = 0; o < P; ++o) gnb[k][o] = lex(lt.off[k][o]) * GREC + 6 * k;
+        // cluster mode: adds iteration tt's dense records (s2, lexicographic
+        // node order) into the accumulator at the element's local nodes (64
+        // distinct local nodes per element: no two lanes collide)
+        auto cl_accumulate = [&](int tt) {
+            if constexpr (CL) {
+                const int i = min(tt, nit - 1);
+                if (cel_s[i] < 0) return; // padding slot
+                const int sl = min(max(cloc_s[(tt % CR) * 64 + lane], 0), kClusterLMax - 1);
+                const T *d = (const T *)s2 + RW * lane;
+                double *ac = acc_s + 5 * sl;
+#pragma unroll
+                for (int f = 0; f < 5; ++f) ac[f] += (double)d[f];
+            }
+        };
+        // cluster mode, after the last element of the cluster of iteration tt:
+        // interior working nodes get R = fact * sum * invMl and the stage
+        // combine (k_gather_update's arithmetic; the sum runs over the same
+        // elements in the same ascending order), surface working nodes one
+        // partial record; the accumulator is cleared for the next cluster
+        auto cl_flush = [&](int tt) {
+            if constexpr (CL) {
+                const int k = min(tt, nit - 1) / G;
+                const int base = cb_s[2 * k], lim = cb_s[2 * k + 1];
+                const size_t np = (size_t)A.npoin;
+                for (int l = base + lane; l < lim; l += 64) {
+                    double v[5];
+                    double *ac = acc_s + 5 * (l - base);
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) {
+                        v[f] = ac[f];
+                        ac[f] = 0.0;
+                    }
+                    const int kind = A.ckind[l];
+                    if (kind == 1) {
+                        double *r = A.rec + 5 * (size_t)l;
+#pragma unroll
+                        for (int f = 0; f < 5; ++f) r[f] = v[f];
+                    } else if (kind == 0) {
+                        const size_t n = (size_t)A.cnode[l];
+                        const size_t k3[5] = {n, n, n + np, n + 2 * np, n};
+                        const double *y1v[5] = {A.y1r, A.y1q, A.y1q, A.y1q, A.y1E};
+                        double *yv[5] = {A.yr, A.yq, A.yq, A.yq, A.yE};
+                        double *rv[5] = {A.Rm, A.Rq, A.Rq, A.Rq, A.Re};
+                        double *xv[5] = {A.nxr, A.nxq, A.nxq, A.nxq, A.nxE};
+                        const double m = A.invMl[n];
+                        double b[5], y[5];
+#pragma unroll
+                        for (int f = 0; f < 5; ++f) {
+                            b[f] = A.rd1 ? y1v[f][k3[f]] : 0.0;
+                            y[f] = A.rdy ? yv[f][k3[f]] : 0.0;
+                        }
+#pragma unroll
+                        for (int f = 0; f < 5; ++f) {
+                            const double r = (A.fact * v[f]) * m;
+                            const double z = A.lam1 * b[f] + A.lams * y[f] + A.gdt * r;
+                            yv[f][k3[f]] = z;
+                            if (A.nxr) {
+                                const double zero = 0.0, nls = 0.0;
+                                xv[f][k3[f]] = A.nx_l1 * z + nls * zero + A.nx_gdt * r;
+                            }
+                            if (A.Rm) rv[f][k3[f]] = r;
+                        }
+                    }
+                }
+            }
+        };
         for (int t = 0; t < nit; ++t) {
-            wait_vm<(AH - 1) * L::NOPS1>();
+            wait_vm<(AH - 1) * (L::NOPS1 + (CL ? 1 : 0))>();
             lds_barrier(); // B1
             T He[3][3], vol;
             split_read<P, T, G16>(ring + slot_of(t), a, pa, true, He, vol, cur);
@@ -1386,16 +1540,25 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                 if (act) s1[(own + 10) >> 1] = V2{Tk, T(0)};
             }
             lds_barrier(); // B2
+            // cluster mode: the previous element's records go into the
+            // accumulator before its local-index row is overwritten below
+            if constexpr (CL)
+                if (t != 0) cl_accumulate(t - 1);
             {
                 const int *c = (const int *)(ring + crow_of(t + AH));
                 split_issue<P, T, G16, 1>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + 2 * AH),
                                           elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
                 mu_at(t);
+                cloc_issue(t + AH);
             }
             // residual records of the previous element, completed in s2 by the
             // convection wave: their stores complete while this element runs
             // (read here, before this wave's flux records overwrite s2)
-            if (t != 0) split_store<P, T>(A, s2, elem_of(t - 1), lane);
+            if constexpr (CL) {
+                if (t != 0 && t % G == 0) cl_flush(t - 1);
+            } else if (t != 0) {
+                split_store<P, T>(A, s2, elem_of(t - 1), lane);
+            }
             // grad T from the T lines; grad u and grad rho from the line
             // derivatives handed over by the convection wave
             T dT[3];
@@ -1497,7 +1660,12 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
         // the last element's records, completed by the convection wave
         lds_barrier();
         lds_barrier();
-        split_store<P, T>(A, s2, elast, lane);
+        if constexpr (CL) {
+            cl_accumulate(nit - 1);
+            cl_flush(nit - 1);
+        } else {
+            split_store<P, T>(A, s2, elast, lane);
+        }
     }
     // no copy may still be writing this block's LDS when it retires
     wait_vm<0>();
@@ -1783,6 +1951,89 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     prof_end(ctx, tok);
     return rc;
 }
+int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *cl,
+                      const double *He, const double *gpvol, const double *dl, const int *inv,
+                      const int *gI, const int *gJ, const int *gK, const double *mu_e,
+                      double gamma, double Rgas, double Cp, double Pr, double mu_factor,
+                      double c_rho, double c_ener, double p_ref, const double *invMl, double fact,
+                      double dt, double lam1, double lams, double gam, const double *rho1,
+                      const double *q1, const double *E1, double *rho, double *q, double *E,
+                      double *Rmass, double *Rmom, double *Rener, const NextStage *next,
+                      double *rec)
+{
+    using T = double;
+    RhsArgs<T> A{};
+    A.nelem = m->nelem;
+    A.npoin = m->npoin;
+    A.EN = (size_t)m->nelem * 64;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.cr = rho;
+    A.cq = q;
+    A.cE = E;
+    A.mu_e = mu_e;
+    A.gamma = gamma;
+    A.Rgas = Rgas;
+    A.mu_factor = mu_factor;
+    A.p_ref = p_ref;
+    A.Cp = Cp;
+    A.Pr = Pr;
+    A.c_rho = c_rho;
+    A.c_ener = c_ener;
+    A.out = nullptr;
+    A.cel = cl->cel;
+    A.cloc = cl->cloc;
+    A.cnb = cl->cnb;
+    A.cnode = cl->cnode;
+    A.ckind = cl->ckind;
+    A.ncl = cl->ncl;
+    A.rec = rec;
+    A.invMl = invMl;
+    A.fact = fact;
+    A.lam1 = lam1;
+    A.lams = lams;
+    A.gdt = gam * dt;
+    A.rd1 = lam1 != 0.0;
+    A.rdy = lams != 0.0;
+    A.y1r = rho1;
+    A.y1q = q1;
+    A.y1E = E1;
+    A.yr = rho;
+    A.yq = q;
+    A.yE = E;
+    A.Rm = Rmass;
+    A.Rq = Rmom;
+    A.Re = Rener;
+    if (next) {
+        A.nx_l1 = next->nl1;
+        A.nx_gdt = next->ngdt;
+        A.nxr = next->rho;
+        A.nxq = next->q;
+        A.nxE = next->E;
+    }
+    A.geo16 = ((uintptr_t)He % 16 == 0) && ((uintptr_t)gpvol % 16 == 0);
+    // XCD-aware: every XCD's eighth of the clusters split over its blocks,
+    // at most kClusterBlockMax clusters per block
+    const long chunk = ((long)cl->ncl + 7) / 8;
+    const long per = std::max<long>((chunk + kClusterBlockMax - 1) / kClusterBlockMax, 1);
+    const int grid = (int)(8 * per);
+    int tok = prof_begin(ctx, 0);
+    if (A.geo16)
+        k_rhs_split<3, T, true, true><<<grid, 128, 0, ctx->stream>>>(A);
+    else
+        k_rhs_split<3, T, false, true><<<grid, 128, 0, ctx->stream>>>(A);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
 template int rhs_fused_elements<double>(sodhip_ctx, int, const MeshCache *, const double *,
                                         const double *, const double *, const int *, const int *,
                                         const int *, const int *, const double *, const double *,

@@ -208,22 +208,24 @@ __global__ __launch_bounds__(256) void k_gather_rk4(
     }
 }
 
-// the gather/update of one stage from the element records scr (5 values per
-// element-node, type TS) over the working incidence of m
+// the gather/update of one stage from the records scr (rw values per record,
+// type TS) of the working nodes list[0..nw) (1-based), node i owning the
+// records src[row[i]..row[i+1])
 template <typename TS>
-int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
-                  const int *lpoin_w, const TS *scr, int rw, size_t EN, const double *invMl,
-                  double fact, double dt, double lam1, double lams, double gam,
-                  const double *rho1, const double *q1, const double *E1, double *rho, double *q,
-                  double *E, double *Rmass, double *Rmom, double *Rener, const NextStage *next)
+int gather_update_on(sodhip_ctx ctx, int npoin, int nw, const int *list, const int *row,
+                     const int *src, const TS *scr, int rw, const double *invMl, double fact,
+                     double dt, double lam1, double lams, double gam, const double *rho1,
+                     const double *q1, const double *E1, double *rho, double *q, double *E,
+                     double *Rmass, double *Rmom, double *Rener, const NextStage *next)
 {
     if (rw != 5 && rw != 6) return set_error(SODHIP_ERR_ARG, "record width %d", rw);
+    if (nw <= 0) return SODHIP_OK;
     int tok = prof_begin(ctx, 1);
-    const int grid = ((npoin_w + 255) / 256 + 7) / 8 * 8;
+    const int grid = ((nw + 255) / 256 + 7) / 8 * 8;
     const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
 #define SODHIP_GU3(R1, RY, NX, RWV)                                                            \
     k_gather_update<TS, R1, RY, NX, RWV><<<grid, 256, 0, ctx->stream>>>(                       \
-        npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, EN, invMl, fact, lam1, lams, gam * dt,  \
+        npoin, nw, list, row, src, scr, 0, invMl, fact, lam1, lams, gam * dt,                   \
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
 #define SODHIP_GU2(R1, RY, NX)                                                                 \
     if (rw == 6)                                                                               \
@@ -250,6 +252,26 @@ int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
 }
+template int gather_update_on<double>(sodhip_ctx, int, int, const int *, const int *, const int *,
+                                      const double *, int, const double *, double, double,
+                                      double, double, double, const double *, const double *,
+                                      const double *, double *, double *, double *, double *,
+                                      double *, double *, const NextStage *);
+
+// the gather/update of one stage from the element records scr (5 values per
+// element-node, type TS) over the working incidence of m
+template <typename TS>
+int gather_update(sodhip_ctx ctx, const MeshCache *m, int npoin, int npoin_w,
+                  const int *lpoin_w, const TS *scr, int rw, size_t EN, const double *invMl,
+                  double fact, double dt, double lam1, double lams, double gam,
+                  const double *rho1, const double *q1, const double *E1, double *rho, double *q,
+                  double *E, double *Rmass, double *Rmom, double *Rener, const NextStage *next)
+{
+    (void)EN;
+    return gather_update_on<TS>(ctx, npoin, npoin_w, lpoin_w, m->wrow, m->wsrc, scr, rw, invMl,
+                                fact, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom,
+                                Rener, next);
+}
 template int gather_update<double>(sodhip_ctx, const MeshCache *, int, int, const int *,
                                    const double *, int, size_t, const double *, double, double,
                                    double, double, double, const double *, const double *,
@@ -266,9 +288,30 @@ static int rk_stage_on(sodhip_ctx ctx, int porder, const MeshCache *m, int npoin
                        double lam1, double lams, double gam, const double *rho1, const double *q1,
                        const double *E1, double *rho, double *q, double *E, double *Rmass,
                        double *Rmom, double *Rener, const NextStage *next, void *scr_mem,
-                       size_t scr_bytes)
+                       size_t scr_bytes, const ClusterTables *cl)
 {
     const size_t EN = (size_t)m->nelem * nnode_of(porder);
+    if constexpr (sizeof(T) == 8) {
+        if (cl && porder == 3) {
+            // brick-local assembly: interior nodes finished in the element
+            // kernel, surface nodes by the gather over their partial records
+            if ((size_t)cl->nloc * 5 * sizeof(double) > scr_bytes)
+                return set_error(SODHIP_ERR_ARG,
+                                 "the stage needs %zu bytes of record scratch, %zu available",
+                                 (size_t)cl->nloc * 5 * sizeof(double), scr_bytes);
+            double *rec = (double *)scr_mem;
+            int rc = rhs_cluster_stage(ctx, m, cl, He, gpvol, dl, inv, gI, gJ, gK, mu_e,
+                                       prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr,
+                                       prm->mu_factor, prm->c_rho, prm->c_ener, prm->p_ref, invMl,
+                                       prm->fact, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E,
+                                       Rmass, Rmom, Rener, next, rec);
+            if (rc) return rc;
+            return gather_update_on<double>(ctx, npoin, cl->ns, cl->snode, cl->srow, cl->ssrc,
+                                            rec, 5, invMl, prm->fact, dt, lam1, lams, gam, rho1,
+                                            q1, E1, rho, q, E, Rmass, Rmom, Rener, next);
+        }
+    }
+    (void)npoin_w;
     const int rw = fused_record_width(porder, (int)sizeof(T));
     if (rw * EN * sizeof(T) > scr_bytes)
         return set_error(SODHIP_ERR_ARG, "the stage needs %zu bytes of record scratch, %zu available",
@@ -298,10 +341,17 @@ static int rk_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     if ((rc = get_working(ctx, m, npoin_w, lpoin_w))) return rc;
     const size_t EN = (size_t)nelem * nnode_of(porder);
     const int rw = fused_record_width(porder, (int)sizeof(T));
-    if ((rc = ensure_buf(ctx->scratch, rw * EN * sizeof(T)))) return rc;
+    const ClusterTables *cl = nullptr;
+    if (sizeof(T) == 8 && porder == 3 && ctx->cluster_stage) {
+        // a mesh without clusters (an element listing a node twice) keeps
+        // the node-ordered gather
+        if (get_clusters(ctx, m, &cl) != SODHIP_OK) cl = nullptr;
+    }
+    const size_t need = cl ? (size_t)cl->nloc * 5 * sizeof(double) : rw * EN * sizeof(T);
+    if ((rc = ensure_buf(ctx->scratch, need))) return rc;
     return rk_stage_on<T>(ctx, porder, m, npoin, npoin_w, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK,
                           mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass,
-                          Rmom, Rener, next, ctx->scratch.ptr, ctx->scratch.bytes);
+                          Rmom, Rener, next, ctx->scratch.ptr, ctx->scratch.bytes, cl);
 }
 
 // One classical RK4 stage (see k_gather_rk4): the fused RHS at the stage state
@@ -721,13 +771,14 @@ extern "C" int sodhip_rk_stage_noalloc_f64(
                                  (const double *)gpvol, (const double *)dlxigp_ip, invAtoIJK,
                                  gmshAtoI, gmshAtoJ, gmshAtoK, (const double *)mu_e, invMl, prm,
                                  dt, lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
-                                 nullptr, scr, kNoAllocScratchBytes);
+                                 nullptr, scr, kNoAllocScratchBytes,
+                                 ctx->cluster_stage && porder == 3 ? m->clusters.get() : nullptr);
     else
         rc = rk_stage_on<float>(ctx, porder, m, npoin, npoin_w, lpoin_w, (const float *)He,
                                 (const float *)gpvol, (const float *)dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, (const float *)mu_e, invMl, prm, dt,
                                 lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,
-                                nullptr, scr, kNoAllocScratchBytes);
+                                nullptr, scr, kNoAllocScratchBytes, nullptr);
     ctx->prof = prof;
     return rc;
 }

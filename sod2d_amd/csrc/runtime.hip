@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -58,6 +59,7 @@ int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu)
 static void free_mesh(MeshCache *m)
 {
     if (!m) return;
+    free_clusters(m);
     (void)hipFree(m->connecT);
     (void)hipFree(m->row);
     (void)hipFree(m->src);
@@ -136,6 +138,7 @@ int get_mesh(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
     m->h_row = std::move(row);
     m->h_src = std::move(src);
+    m->h_cT = std::move(cT);
     *out = m.get();
     ctx->meshes.push_back(std::move(m));
     return SODHIP_OK;
@@ -180,6 +183,7 @@ int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
         std::copy(m->h_src.begin() + m->h_row[n], m->h_src.begin() + m->h_row[(size_t)n + 1],
                   wsrc.begin() + wrow[i]);
     }
+    free_clusters(m);
     (void)hipFree(m->wrow);
     (void)hipFree(m->wsrc);
     (void)hipFree(m->wflag);
@@ -200,6 +204,7 @@ int get_working(sodhip_ctx ctx, MeshCache *m, int npoin_w, const int *lpoin_w)
     SODHIP_TRY_HIP(hipStreamSynchronize(ctx->stream));
     m->lpw_key = lpoin_w;
     m->npoin_w = npoin_w;
+    m->h_wflag = std::move(wflag);
     return SODHIP_OK;
 }
 
@@ -272,6 +277,7 @@ extern "C" int sodhip_create(int device, sodhip_ctx *out)
     c->device = device;
     c->stream = nullptr;
     c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (const char *v = getenv("SODHIP_STAGE_CLUSTERS")) c->cluster_stage = atoi(v) != 0;
     *out = c;
     return SODHIP_OK;
 }
@@ -319,6 +325,7 @@ static int forget_range(sodhip_ctx ctx, uintptr_t lo, uintptr_t hi)
         const uintptr_t w = (uintptr_t)m->lpw_key;
         if (m->wrow && w >= lo && w < hi) {
             if ((rc = sync())) return rc;
+            free_clusters(m);
             (void)hipFree(m->wrow);
             (void)hipFree(m->wsrc);
             (void)hipFree(m->wflag);
