@@ -429,6 +429,79 @@ int sodhip_entropy_convec_residual_f64(sodhip_ctx ctx, int porder, int nelem, in
                                        double *Reta, double *auxReta);
 
 /* ------------------------------------------------------------------------ */
+/* MLP surrogate of the entropy-viscosity sensor (no counterpart in the       */
+/* reference; it stands in for smart_visc_spectral, which stays its          */
+/* per-element fallback)                                                     */
+/* ------------------------------------------------------------------------ */
+/* The MLP 12 -> 64 -> 64 -> 1 with SiLU activations.  w: the 5057 FP32
+ * parameters (W1 12x64 row = feature, b1 64, W2 64x64 row = input, b2 64,
+ * w3 64, b3); wq: their BF16 operand copy (5184 values: W1 padded to 16 rows,
+ * W2, w3), refreshed by sodhip_mlp_quantize and by every training step; m, v:
+ * Adam moments (5057 each, training only); lo/hi: the range of each feature
+ * over the training data (the deployed sensor's fallback test).  Device
+ * pointers except lo/hi. */
+typedef struct {
+    float *w;
+    void *wq;
+    float *m, *v;
+    float lo[12], hi[12];
+} sodhip_mlp;
+/* sizes of w and wq, and the number of features (host) */
+int sodhip_mlp_sizes(int *nparams, int *nq, int *nfeatures);
+int sodhip_mlp_quantize(sodhip_ctx ctx, const sodhip_mlp *model);
+/* The twelve features of every element (p = 3) from the inputs of
+ * smart_visc_spectral (same arguments; the entropy normalisation as that call
+ * forms it): X(nelem,16) row-major FP32, features 0..11 then zeros, and
+ * beta(nelem) the element's cap estimate.  The features read 16 of the
+ * element's 64 nodes, its 8 vertices and its 8 interior nodes.  Over these,
+ * with b = rho helem_k cmax/p (|u| + c), beta_e = max b, r = ce rho|Reta|/norm
+ * h^2 / beta_e, m = min(r,1) and the log scale g(x) = 1 + log10(x + 1e-10)/10:
+ * g of the max, mean, min and standard deviation of m and of its mean over
+ * the vertices and over the interior nodes; share of nodes with r >= 1;
+ * log10(1 + max r)/4; largest Mach number; mean of |u|/(|u|+c);
+ * 10 (max-min)/mean of rho and of T. */
+int sodhip_sensor_features_f64(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                               const int *connec, const int *lpoin_w, const double *Reta,
+                               double gamma_gas, const double *rho, const double *u,
+                               const double *Tem, const double *eta, const double *helem,
+                               const double *helem_k, const int *invAtoIJK,
+                               const sodhip_visc_params *prm, float *X, double *beta);
+/* y(n) = MLP(X(n,16)) on the matrix cores (BF16 operands, FP32 accumulation) */
+int sodhip_mlp_forward(sodhip_ctx ctx, const sodhip_mlp *model, int n, const float *X, float *y);
+/* Gradient of L = (1/n) sum (y - t)^2 over the batch: forward (y, and the
+ * pre-activations into Z(2*n*64)), then the backward pass; gpart receives
+ * nwaves partial gradients of 5057 values (nwaves a multiple of 4), whose sum
+ * in order is the gradient. */
+int sodhip_mlp_gradient(sodhip_ctx ctx, const sodhip_mlp *model, int n, const float *X,
+                        const float *t, float *y, float *Z, float *gpart, int nwaves);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step = 1, 2, ...) on the
+ * gradient of sodhip_mlp_gradient, then the BF16 copy. */
+int sodhip_mlp_train_step(sodhip_ctx ctx, sodhip_mlp *model, int n, const float *X,
+                          const float *t, int step, float lr, float *y, float *Z, float *gpart,
+                          int nwaves);
+/* The surrogate sensor: smart_visc_spectral's arguments and outputs (p = 3).
+ * Per element mu_e(e,:) = cglob * beta_e * (10^(10 (y_e - 1)) - 1e-10) with
+ * y_e the MLP of the element's features (the mean of mu_e/(cglob beta_e) over
+ * the element on the log scale g), unless a feature lies outside [lo, hi] or
+ * y_e outside [0, 1]: such an element is flagged and takes smart_visc_spectral's values
+ * (mu_e and mue_l; mue_l is written for those elements only).  One kernel
+ * forms features, MLP output and mu_e; a second one runs the plain sensor on
+ * the listed fallback elements.  flag(nelem) (device) receives 1 for a
+ * fallback element, else 0; X(nelem,16), beta(nelem) and y(nelem) (device,
+ * each may be NULL) receive the features, cap estimates and MLP outputs as
+ * sodhip_sensor_features_f64 and sodhip_mlp_forward form them; nfallback
+ * (device, may be NULL) receives the number of fallback elements. */
+int sodhip_surrogate_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                       int npoin_w, const int *connec, const int *lpoin_w,
+                                       const double *Reta, double gamma_gas, const double *rho,
+                                       const double *u, const double *Tem, const double *eta,
+                                       const double *helem, const double *helem_k,
+                                       const int *invAtoIJK, const sodhip_visc_params *prm,
+                                       const sodhip_mlp *model, double *mu_e, double *mue_l,
+                                       float *X, double *beta, float *y, int *flag,
+                                       int *nfallback);
+
+/* ------------------------------------------------------------------------ */
 /* Fused Runge-Kutta stage                                                   */
 /* ------------------------------------------------------------------------ */
 typedef struct {

@@ -28,6 +28,12 @@ class ViscParams(C.Structure):
                 ("flag_normalise_entropy", I), ("flag_drop_c_in_envit", I)]
 
 
+class MLP(C.Structure):
+    """sodhip_mlp: device pointers of the surrogate MLP and its feature range."""
+    _fields_ = [("w", C.c_void_p), ("wq", C.c_void_p), ("m", C.c_void_p), ("v", C.c_void_p),
+                ("lo", C.c_float * 12), ("hi", C.c_float * 12)]
+
+
 class StageParams(C.Structure):
     """sodhip_stage_params: physical constants of the fused RK stage."""
     _fields_ = [("gamma_gas", D), ("Rgas", D), ("Cp", D), ("Pr", D), ("mu_factor", D),
@@ -88,6 +94,15 @@ _SIGS = {
     "sodhip_rk_stage_next_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                                  + [C.POINTER(StageParams), D, D, D, D] + [P] * 9
                                  + [D, P, P, P]),
+    "sodhip_mlp_sizes": (I, [P, P, P]),
+    "sodhip_mlp_quantize": (I, [P, C.POINTER(MLP)]),
+    "sodhip_sensor_features_f64": (I, [P, I, I, I, I, P, P, P, D] + [P] * 7
+                                   + [C.POINTER(ViscParams), P, P]),
+    "sodhip_mlp_forward": (I, [P, C.POINTER(MLP), I, P, P]),
+    "sodhip_mlp_gradient": (I, [P, C.POINTER(MLP), I, P, P, P, P, P, I]),
+    "sodhip_mlp_train_step": (I, [P, C.POINTER(MLP), I, P, P, I, F, P, P, P, I]),
+    "sodhip_surrogate_visc_spectral_f64": (I, [P, I, I, I, I, P, P, P, D] + [P] * 7
+                                           + [C.POINTER(ViscParams), C.POINTER(MLP)] + [P] * 7),
     "sodhip_rk4_stage_update_f64": (I, [P, I, I, P, P, D, D, D, I, I] + [P] * 12),
     "sodhip_rk4_stage_f64": (I, [P, I, I, I, I, P, P, I] + [P] * 9
                              + [C.POINTER(StageParams), D, D, D, I, I] + [P] * 15),

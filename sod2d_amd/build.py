@@ -21,7 +21,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("SODHIP_ARCH", "gfx950")
 
 SOURCES = ["runtime.hip", "ops_element.hip", "ops_lps.hip", "ops_node.hip", "ops_sensor.hip", "ops_stage.hip",
-           "setup.cpp", "clusters.cpp"]
+           "ops_mlp.hip", "setup.cpp", "clusters.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
 # Packed FP32 VALU forms (v_pk_*_f32) are disabled: all FP32 work is issued as

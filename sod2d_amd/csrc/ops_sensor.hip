@@ -14,10 +14,10 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "sensor.hpp"
 
 namespace sodhip {
 
-constexpr int kRedBlocks = 512;
 constexpr int kRedThreads = 256;
 
 template <typename T> __device__ __forceinline__ T block_sum(T v, T *sh)
@@ -284,37 +284,6 @@ __global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int npoin,
     }
 }
 
-template <int P> struct SG {
-    static constexpr int N1 = P + 1;
-    static constexpr int NN = N1 * N1 * N1;
-    static constexpr int NT = (NN + 63) / 64 * 64;
-};
-
-struct SensorArgs {
-    int nelem, npoin;
-    const int *connecT;
-    const double *Reta, *rho, *u, *Tem, *helem, *helem_k;
-    const int *inv;
-    double gamma, Rgas, ce, cmax, cglob, fact_low_mach;
-    const double *pmax; // partial maxima (nparts) or nullptr when not normalising
-    int nparts;
-    double *mu_e, *mue_l;
-};
-
-// sod2d 1-D index s (1-based) -> geometric position along the line (1-based)
-__device__ __forceinline__ int geo_of(int s, int N1) { return s == 1 ? 1 : (s == 2 ? N1 : s - 1); }
-
-// convertIJK(0:p+2): geometric position (with one ghost each side) -> sod2d
-// index, reflecting 0 -> 3 and p+2 -> p (mod_filters.f90:44-50)
-__device__ __forceinline__ int convert_ijk(int g, int P)
-{
-    if (g == 0) return 3;
-    if (g == 1) return 1;
-    if (g == P + 2) return P;
-    if (g == P + 1) return 2;
-    return g + 1; // 2..P -> 3..P+1
-}
-
 // Chunked variant: the workgroup (NT = roundup((p+1)^3, 64) lanes, one or two
 // waves) walks CH consecutive elements at a time so that the element-fastest
 // arrays helem(nelem,nnode), mue_l and mu_e move as whole 128-byte lines: the
@@ -338,27 +307,8 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgs A)
 #pragma unroll
         for (int s = 32; s > 0; s >>= 1) norm = fmax(norm, __shfl_xor(norm, s, 64));
     }
-    int I = 1, J = 1, K = 1;
-    for (int k = 0; k < N1; ++k)
-        for (int j = 0; j < N1; ++j)
-            for (int i = 0; i < N1; ++i)
-                if (A.inv[i + N1 * (j + N1 * k)] - 1 == a) {
-                    I = i + 1;
-                    J = j + 1;
-                    K = k + 1;
-                }
-    const int ii = geo_of(I, N1), jj = geo_of(J, N1), kk = geo_of(K, N1);
     int src[27];
-#pragma unroll
-    for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const int ci = convert_ijk(ii + l - 1, P) - 1, cj = convert_ijk(jj + m - 1, P) - 1,
-                          ck = convert_ijk(kk + n - 1, P) - 1;
-                src[9 * l + 3 * m + n] = min(max(A.inv[ci + N1 * (cj + N1 * ck)] - 1, 0), NN - 1);
-            }
+    filter_sources<P>(a, A.inv, src);
     const double w[3] = {0.25, 0.5, 0.25};
     // XCD-aware: the blocks of one XCD walk chunks of one contiguous eighth
     const int per = gridDim.x >> 3, xcd = blockIdx.x & 7, rk = blockIdx.x >> 3;
@@ -441,6 +391,26 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgs A)
         }
         lds_barrier(); // the next chunk overwrites lb and ob
     }
+}
+
+// The entropy normalisation's partial maxima of |eta - mean| over the working
+// nodes (kRedBlocks values in ctx->scratch2), or nullptr when the sensor does
+// not normalise; for kernels that fold them as the sensor does.
+int entropy_norm_partials(sodhip_ctx ctx, int npoin_w, const int *lpoin_w, const double *eta,
+                          bool normalise, const double **pmax, int *nparts)
+{
+    *pmax = nullptr;
+    *nparts = kRedBlocks;
+    if (!normalise) return SODHIP_OK;
+    int rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->scratch2.ptr;
+    k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+    k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+                                                              kRedBlocks, part + kRedBlocks);
+    SODHIP_TRY_HIP(hipGetLastError());
+    *pmax = part + kRedBlocks;
+    return SODHIP_OK;
 }
 
 } // namespace sodhip

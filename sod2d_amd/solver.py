@@ -122,7 +122,8 @@ class TGVRun:
                  op_precision: int = 8, parity: bool = True, dt_mode: str = "adaptive",
                  uniform_rho: bool = False, sensor: bool = True, ctx: Context | None = None,
                  geometry=None, initial=None, sensor_path: str = "spectral", lps: bool = False,
-                 high_mach: bool = False, integrator: str = "ls"):
+                 high_mach: bool = False, integrator: str = "ls", surrogate=None,
+                 judge_every: int = 10):
         self.t = params or TGVParams()
         if integrator not in ("ls", "rk4"):
             raise SodhipError(f"unknown integrator {integrator!r}")
@@ -135,8 +136,15 @@ class TGVRun:
         self.dt_mode = dt_mode
         self.op_precision = op_precision
         self.sensor = sensor
-        if sensor_path not in ("spectral", "live"):
+        if sensor_path not in ("spectral", "live", "surrogate"):
             raise SodhipError(f"unknown sensor_path {sensor_path!r}")
+        if sensor_path == "surrogate" and (surrogate is None or porder != 3):
+            raise SodhipError("the surrogate sensor needs a trained SurrogateMLP and porder 3")
+        # the MLP surrogate of smart_visc_spectral; the plain sensor runs
+        # instead every judge_every steps (and per element as its fallback)
+        self.surrogate = surrogate
+        self.judge_every = judge_every
+        self.judge_log = []  # (step, relative error of the surrogate's mu_e, fallback elements)
         self.sensor_path = sensor_path
         # LPS stabilisation (flag_lps_stab) runs updateF through the drop-in
         # operators (FP64): convection, full_proj_ijk and full_diff_stab_ijk
@@ -167,6 +175,12 @@ class TGVRun:
         self.helem = _dev(g.helem)
         self.helem_l = _dev(g.helem_l)
         self.vol_tot = g.vol_tot
+        if sensor_path == "surrogate":
+            self._sx = torch.empty(m.nelem * 16, dtype=F32, device="cuda")
+            self._sbeta = torch.empty(m.nelem, dtype=F64, device="cuda")
+            self._sy = torch.empty(m.nelem, dtype=F32, device="cuda")
+            self._sflag = torch.zeros(m.nelem, dtype=I32, device="cuda")
+            self._snfall = torch.zeros(1, dtype=I32, device="cuda")
         if sensor_path == "live":
             self.Ml = _dev(g.Ml)
             self.lelpn = _dev(case_mod.elem_per_node(m.connec, m.npoin))
@@ -247,14 +261,38 @@ class TGVRun:
                                                   _ptr(self.lpoin_w), _ptr(self.invMl), _ptr(out)),
               "lumped_solver_scal")
 
+    def _surrogate_visc(self, Reta, rho, u, Tem, eta, mu_e):
+        h, p, ne, n, nw = self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w
+        check(lib().sodhip_surrogate_visc_spectral_f64(
+            h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta), self.t.gamma_gas,
+            _ptr(rho), _ptr(u), _ptr(Tem), _ptr(eta), _ptr(self.helem_l), _ptr(self.helem),
+            _ptr(self.tabs[0]), C.byref(self.vp), C.byref(self.surrogate.struct()), _ptr(mu_e),
+            _ptr(self.mue_l), _ptr(self._sx), _ptr(self._sbeta), _ptr(self._sy),
+            _ptr(self._sflag), _ptr(self._snfall)), "surrogate_visc_spectral")
+
     def _smart_visc(self, Reta, rho, u, Tem, eta, ducros=False):
         h, p, ne, n, nw = self.ctx.h, self.p, self.nelem, self.npoin, self.npoin_w
-        if self.sensor_path == "spectral":
+        if self.sensor_path == "surrogate":
+            judge = self.judge_every > 0 and self.istep % self.judge_every == 0
+            if not judge:
+                self._surrogate_visc(Reta, rho, u, Tem, eta, self.mu_e)
+                if self.op_precision == 4:
+                    self.mu_e_op.copy_(self.mu_e)
+                return
+            # judge step: the plain sensor sets mu_e; the surrogate's values
+            # beside it are logged (relative error, fallback count)
+            mu_s = torch.empty_like(self.mu_e)
+            self._surrogate_visc(Reta, rho, u, Tem, eta, mu_s)
+        if self.sensor_path in ("spectral", "surrogate"):
             check(lib().sodhip_smart_visc_spectral_f64(
                 h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta),
                 self.t.gamma_gas, _ptr(rho), _ptr(u), _ptr(Tem), _ptr(eta), _ptr(self.helem_l),
                 _ptr(self.helem), _ptr(self.tabs[0]), C.byref(self.vp), _ptr(self.mu_e),
                 _ptr(self.mue_l)), "smart_visc_spectral")
+            if self.sensor_path == "surrogate":
+                err = float(torch.linalg.norm(mu_s - self.mu_e) /
+                            torch.clamp(torch.linalg.norm(self.mu_e), min=1e-300))
+                self.judge_log.append((self.istep, err, int(self._snfall.item())))
         else:
             check(lib().sodhip_smart_visc_spectral_imex_f64(
                 h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta), _ptr(rho),
