@@ -18,6 +18,12 @@ The JSON line also carries:
                   with HIP events on the launch stream over up to 3 steps run
                   right after the timed ones (the timed steps carry no events);
                   HBM traffic per stage from the committed rocprofv3 PMC summary
+  surrogate    -- the learned entropy-viscosity sensor (per-element MLP, BF16 on
+                  the matrix cores, plain sensor as fallback and judge every 10
+                  steps) against the plain sensor on two fresh runs of the same
+                  TGV: ms/step of each and their ratio, the sensor's own ms/step,
+                  the kinetic-energy and dissipation-rate errors, mu_e error and
+                  fallback count on the judge steps, training cost.
   cpu_baseline -- the CPU restatement of updateF (oracle, OpenMP, all host cores)
                   on a bounded sample at 16^3 elements, same GDOF/s definition.
 
@@ -189,6 +195,82 @@ def run_operators(a) -> dict:
     }
 
 
+def run_surrogate(a, steps: int = 20, window: int = 40) -> dict:
+    """The learned sensor beside the plain one on the bench's TGV: train the
+    MLP (standard training set, 1500 Adam steps), then two fresh runs, plain
+    smart_visc_spectral and the surrogate (plain fallback, judged every 10
+    steps), each timed over ``steps`` steps after 3 warmup steps, then both
+    advanced ``window`` more steps with the kinetic energy read every step."""
+    import numpy as np
+    import torch
+    from sod2d_amd.solver import TGVRun
+    from sod2d_amd.surrogate import trained_surrogate
+
+    from sod2d_amd.device import Context
+
+    ctx = Context(torch.cuda.current_device())
+    t1 = time.perf_counter()
+    net, hist, nsamp = trained_surrogate(ctx, steps=1500)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    runs = {}
+    t0 = time.perf_counter()
+    for kind in ("spectral", "surrogate"):
+        runs[kind] = TGVRun(porder=a.porder, nelem_dir=a.n, op_precision=a.precision,
+                            parity=not a.clean, dt_mode="fixed", sensor_path=kind,
+                            surrogate=net if kind == "surrogate" else None, judge_every=10,
+                            ctx=ctx)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    ms, sensor_ms = {}, {}
+    for kind, r in runs.items():
+        for _ in range(3):
+            r.step()
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(steps):
+            r.step()
+        torch.cuda.synchronize()
+        ms[kind] = (time.perf_counter() - ta) / steps * 1e3
+        # the sensor's share from 10 further steps bracketed by HIP events
+        ctx.prof_reset()
+        ctx.prof_enable(True)
+        for _ in range(10):
+            r.step()
+        torch.cuda.synchronize()
+        sensor_ms[kind] = ctx.prof_get(2)[0] / 10
+        ctx.prof_enable(False)
+    ek = {k: [] for k in runs}
+    for _ in range(window):
+        for kind, r in runs.items():
+            r.step()
+            ek[kind].append(r.diagnostics()["EK"])
+    ekp, eks = np.array(ek["spectral"]), np.array(ek["surrogate"])
+    dt = runs["spectral"].dt
+    dp, ds = -np.diff(ekp) / dt, -np.diff(eks) / dt
+    judged = [j for j in runs["surrogate"].judge_log if j[0] >= 10]
+    out = {
+        "plain_ms_per_step": round(ms["spectral"], 4),
+        "surrogate_ms_per_step": round(ms["surrogate"], 4),
+        "ratio": round(ms["surrogate"] / ms["spectral"], 4),
+        "sensor_ms_per_step": {"plain": round(sensor_ms["spectral"], 4),
+                               "surrogate": round(sensor_ms["surrogate"], 4)},
+        "ke_dissipation_rel_err": float(abs(ds[-10:].mean() - dp[-10:].mean())
+                                        / abs(dp[-10:].mean())),
+        "ke_max_rel_err": float(np.abs(eks - ekp).max() / np.abs(ekp).max()),
+        "mu_e_rel_err_judged": float(np.mean([j[1] for j in judged])) if judged else None,
+        "fallback_elements_judged": float(np.mean([j[2] for j in judged])) if judged else None,
+        "elements": runs["spectral"].nelem,
+        "judge_every": 10,
+        "steps_compared": 3 + steps + 10 + window,
+        "train": {"samples": nsamp, "adam_steps": 1500, "batch": 4096, "s": round(t_train, 2),
+                  "loss_first": hist[0], "loss_last": hist[-1]},
+        "setup_s": round(t_setup, 2),
+    }
+    del runs
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +287,8 @@ def main(argv=None):
                          "stabilisation and the live sensor path (smart_visc_spectral_imex + "
                          "Ducros limiter), through the drop-in operators (FP64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-sensor comparison (surrogate vs plain ms/step)")
     ap.add_argument("--graph", action="store_true",
                     help="time the steps as replays of a HIP graph of two captured steps "
                          "(--steps rounded up to even)")
@@ -345,6 +429,10 @@ def main(argv=None):
                         "other": round(prof[3][0] / prof_steps, 4)},
         "diagnostics": {k: diag[k] for k in ("time", "EK", "eps_T", "maxmach")},
     }
+    if not a.no_surrogate and not a.lps:
+        del run
+        torch.cuda.empty_cache()
+        out["surrogate"] = run_surrogate(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -145,6 +145,12 @@ int sodhip_update_device(sodhip_ctx ctx, const void *host, size_t bytes);
 int sodhip_update_host(sodhip_ctx ctx, void *host, size_t bytes);
 int sodhip_exit_data(sodhip_ctx ctx, const void *host, int copyout);
 void *sodhip_deviceptr(sodhip_ctx ctx, const void *host);
+/* Device-side copy of `bytes` from the mirror of src_host to the mirror of
+ * dst_host, on the context stream: the array assignments the reference runs
+ * as `!$acc kernels` on present data, such as the level copy
+ * eta(:,1) = eta(:,2) of CFDSolverBase_evalTimeIteration
+ * (CFDSolverBase.f90:1929-2164). */
+int sodhip_copy_present(sodhip_ctx ctx, const void *src_host, void *dst_host, size_t bytes);
 
 /* Kernel timing with HIP events on the context stream: when enabled, every
  * launch of kernel class `kind` is bracketed by events; sodhip_prof_get returns
@@ -436,7 +442,8 @@ int sodhip_entropy_convec_residual_f64(sodhip_ctx ctx, int porder, int nelem, in
 /* The MLP 12 -> 64 -> 64 -> 1 with SiLU activations.  w: the 5057 FP32
  * parameters (W1 12x64 row = feature, b1 64, W2 64x64 row = input, b2 64,
  * w3 64, b3); wq: their BF16 operand copy (5184 values: W1 padded to 16 rows,
- * W2, w3), refreshed by sodhip_mlp_quantize and by every training step; m, v:
+ * W2, w3; 16-byte aligned), refreshed by sodhip_mlp_quantize and by every
+ * training step; m, v:
  * Adam moments (5057 each, training only); lo/hi: the range of each feature
  * over the training data (the deployed sensor's fallback test).  Device
  * pointers except lo/hi. */

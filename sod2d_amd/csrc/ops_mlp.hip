@@ -153,21 +153,33 @@ __device__ __forceinline__ int subset_node(const int *inv, int h, int q)
 // nodes with r >= 1; log10(1 + max r)/4; the largest Mach number; the mean of
 // |u|/(|u| + c); ten times the relative spread (max - min)/mean of density
 // and of temperature.
+// a double at a 32-bit byte offset from a wave-uniform base (one offset
+// register serves every field of a node)
+__device__ __forceinline__ double ld_at(const double *base, unsigned off)
+{
+    return *(const double *)((const char *)base + off);
+}
+
 __device__ __forceinline__ void pair_features(const FeatArgs &A, int e, int h, const int sn[8],
                                               double norm, float f[kNF], double &beta)
 {
     const double hk = A.helem_k[e];
     const double ccoef = A.cmax / 3.0;
+    // byte offsets below 2^32: the host checks nelem * 64 * 8 and npoin * 8
+    const double *u1p = A.u + A.npoin, *u2p = A.u + 2 * (size_t)A.npoin;
+    int nd[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        nd[q] = *(const int *)((const char *)A.connecT + ((unsigned)e * 256u + (unsigned)sn[q] * 4u));
     double Ve[8];
     double bmax = 0.0, mach = 0.0, uc = 0.0;
     double rx = -DBL_MAX, rn = DBL_MAX, rs = 0.0, tx = -DBL_MAX, tn = DBL_MAX, ts = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int a = sn[q];
-        const int n = A.connecT[(size_t)e * 64 + a];
-        const double u0 = A.u[n], u1 = A.u[n + A.npoin], u2 = A.u[n + 2 * (size_t)A.npoin];
-        const double T = A.Tem[n], rho = A.rho[n], Reta = A.Reta[n];
-        const double hn = A.helem[e + (size_t)A.nelem * a];
+        const unsigned o = (unsigned)nd[q] * 8u;
+        const double u0 = ld_at(A.u, o), u1 = ld_at(u1p, o), u2 = ld_at(u2p, o);
+        const double T = ld_at(A.Tem, o), rho = ld_at(A.rho, o), Reta = ld_at(A.Reta, o);
+        const double hn = ld_at(A.helem, ((unsigned)e + (unsigned)A.nelem * (unsigned)sn[q]) * 8u);
         const double um = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
         const double c = sqrt(A.Rgas * A.gamma * T);
         bmax = fmax(bmax, (rho * hk) * ccoef * (um + A.fact_low_mach * c));
@@ -302,7 +314,10 @@ __device__ __forceinline__ f32x16 mlp_tile(const bf16x8 &xa, const __bf16 *sW, c
 // the BF16 weights and FP32 biases into LDS (block-wide)
 __device__ __forceinline__ void load_weights(const __bf16 *wq, const float *w, __bf16 *sW, float *sb)
 {
-    for (int i = threadIdx.x; i < kNQ; i += blockDim.x) sW[i] = wq[i];
+    // 16-byte pieces (kNQ = 5184 BF16 = 648 pieces; wq and sW 16-byte aligned)
+    static_assert(kNQ % 8 == 0, "BF16 copy in 16-byte pieces");
+    for (int i = threadIdx.x; i < kNQ / 8; i += blockDim.x)
+        ((uint4 *)sW)[i] = ((const uint4 *)wq)[i];
     for (int i = threadIdx.x; i < 2 * kNH + 1; i += blockDim.x)
         sb[i] = i < kNH ? w[kB1 + i] : (i < 2 * kNH ? w[kB2 + i - kNH] : w[kB3]);
     __syncthreads();
@@ -317,7 +332,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(int n, const float *__restr
                                                      float *__restrict__ y, float *__restrict__ Z1,
                                                      float *__restrict__ Z2)
 {
-    __shared__ __bf16 sW[kNQ];
+    __shared__ __align__(16) __bf16 sW[kNQ];
     __shared__ float sb[2 * kNH + 1];
     __shared__ __bf16 sX[4][32 * kNFP];
     __shared__ __bf16 sH[4][32 * kNH];
@@ -523,9 +538,9 @@ struct SurArgs {
 // mu_e.  An element whose features leave [lo, hi] or whose output leaves
 // [0, 1] is flagged and appended to the fallback list (one atomic per wave;
 // the list's order varies between runs, its set does not).
-__global__ __launch_bounds__(256) void k_surrogate(SurArgs A)
+__global__ __launch_bounds__(256, 2) void k_surrogate(SurArgs A)
 {
-    __shared__ __bf16 sW[kNQ];
+    __shared__ __align__(16) __bf16 sW[kNQ];
     __shared__ float sb[2 * kNH + 1];
     __shared__ __bf16 sH[4][32 * kNH];
     __shared__ float sy[4][32];
@@ -647,6 +662,7 @@ __global__ void k_zero_int(int *p) { *p = 0; }
 static int mlp_check(const sodhip_mlp *M)
 {
     SODHIP_REQUIRE(M && M->w && M->wq, "null model");
+    SODHIP_REQUIRE(((uintptr_t)M->wq & 15) == 0, "the BF16 copy wq must be 16-byte aligned");
     return SODHIP_OK;
 }
 
@@ -720,6 +736,9 @@ static int check_sensor_args(sodhip_ctx ctx, int porder, int nelem, int npoin, i
     SODHIP_REQUIRE(connec && lpoin_w && Reta && rho && u && Tem && eta && helem && helem_k &&
                        invAtoIJK,
                    "null array argument");
+    // the feature kernels address node and element-node arrays with 32-bit byte offsets
+    SODHIP_REQUIRE((long long)nelem * 64 * 8 < (1LL << 32) && (long long)npoin * 8 < (1LL << 32),
+                   "mesh too large for the feature kernels (nelem < 2^23, npoin < 2^29)");
     return SODHIP_OK;
 }
 

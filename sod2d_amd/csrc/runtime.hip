@@ -455,6 +455,23 @@ extern "C" int sodhip_exit_data(sodhip_ctx ctx, const void *host, int copyout)
     return SODHIP_OK;
 }
 
+extern "C" int sodhip_copy_present(sodhip_ctx ctx, const void *src_host, void *dst_host,
+                                   size_t bytes)
+{
+    size_t so = 0, d_off = 0;
+    PresentEntry *s = ctx ? find_present(ctx, src_host, &so) : nullptr;
+    PresentEntry *d = ctx ? find_present(ctx, dst_host, &d_off) : nullptr;
+    if (!s || !d)
+        return set_error(SODHIP_ERR_NOT_PRESENT, "host address %p is not present",
+                         s ? dst_host : src_host);
+    SODHIP_REQUIRE(so + bytes <= s->bytes && d_off + bytes <= d->bytes,
+                   "copy_present beyond a present range");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_TRY_HIP(hipMemcpyAsync((char *)d->dev + d_off, (const char *)s->dev + so, bytes,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    return SODHIP_OK;
+}
+
 extern "C" void *sodhip_deviceptr(sodhip_ctx ctx, const void *host)
 {
     if (!ctx || !host) return nullptr;

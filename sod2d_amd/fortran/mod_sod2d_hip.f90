@@ -31,18 +31,25 @@ module mod_sod2d_hip
 
    public :: sod2d_hip_init, sod2d_hip_finalize, sod2d_hip_synchronize
    public :: sod2d_hip_enter_data, sod2d_hip_update_device, sod2d_hip_update_host
-   public :: sod2d_hip_exit_data, sod2d_hip_set_visc_params
+   public :: sod2d_hip_exit_data, sod2d_hip_set_visc_params, sod2d_hip_copy
    public :: full_convec_ijk, full_diffusion_ijk, generic_scalar_convec_ijk
    public :: smart_visc_spectral, lumped_solver_scal_opt, lumped_solver_vect_opt
    public :: rk_stage_update
    public :: smart_visc_spectral_imex, eval_divergence, compute_vorticity, ducros_visc_limit
    public :: sod2d_hip_set_live_params
    public :: comp_tau, full_proj_ijk, full_diff_stab_ijk
+   public :: rk_stage_fused, rk_stage_lps_fused, eos_entropy_step, entropy_convec_residual
+   public :: rk4_stage_fused, rk4_stage_update
 
    type, bind(C) :: visc_params_t
       real(c_double) :: ce, cmax, cglob, Rgas
       integer(c_int) :: flag_normalise_entropy, flag_drop_c_in_envit
    end type visc_params_t
+
+   ! sodhip_stage_params: the physical constants of a fused stage
+   type, bind(C) :: stage_params_t
+      real(c_double) :: gamma_gas, Rgas, Cp, Pr, mu_factor, c_rho, c_ener, fact, p_ref
+   end type stage_params_t
 
    type(c_ptr), save :: ctx = c_null_ptr
    ! module globals the reference's sensor reads from mod_numerical_params,
@@ -110,6 +117,11 @@ module mod_sod2d_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: c, host
          integer(c_int), value :: copyout
+      end function
+      integer(c_int) function c_copy_present(c, src, dst, bytes) bind(C, name='sodhip_copy_present')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: c, src, dst
+         integer(c_size_t), value :: bytes
       end function
       type(c_ptr) function c_deviceptr(c, host) bind(C, name='sodhip_deviceptr')
          import :: c_ptr
@@ -231,6 +243,78 @@ module mod_sod2d_hip
          integer(c_int), value :: p, nelem, npoin
          real(c_double), value :: dmin
       end function
+      integer(c_int) function c_rk_stage(c, p, nelem, npoin, npoin_w, connec, lpoin_w, opp, He, &
+            gpvol, dl, inv, gI, gJ, gK, mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1, E1, &
+            rho, q, E, Rmass, Rmom, Rener) bind(C, name='sodhip_rk_stage_f64')
+         import :: c_int, c_ptr, c_double, stage_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, mu_e, invMl
+         type(c_ptr), value :: rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener
+         integer(c_int), value :: p, nelem, npoin, npoin_w, opp
+         type(stage_params_t), intent(in) :: prm
+         real(c_double), value :: dt, lam1, lams, gam
+      end function
+      integer(c_int) function c_rk_stage_next(c, p, nelem, npoin, npoin_w, connec, lpoin_w, opp, &
+            He, gpvol, dl, inv, gI, gJ, gK, mu_e, invMl, prm, dt, lam1, lams, gam, rho1, q1, E1, &
+            rho, q, E, Rmass, Rmom, Rener, gam_next, rhoN, qN, EN) &
+            bind(C, name='sodhip_rk_stage_next_f64')
+         import :: c_int, c_ptr, c_double, stage_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, mu_e, invMl
+         type(c_ptr), value :: rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, rhoN, qN, EN
+         integer(c_int), value :: p, nelem, npoin, npoin_w, opp
+         type(stage_params_t), intent(in) :: prm
+         real(c_double), value :: dt, lam1, lams, gam, gam_next
+      end function
+      integer(c_int) function c_rk_stage_lps(c, p, nelem, npoin, npoin_w, connec, lpoin_w, He, &
+            gpvol, dl, inv, gI, gJ, gK, mu_e, invMl, tau, prm, c_lps, dt, lam1, lams, gam, rho1, &
+            q1, E1, rho, q, E, u, e_int, pr, aux_h, Tem, mu_fluid, PM, PE, PX, PY, PZ, Rmass, &
+            Rmom, Rener, gam_next, rhoN, qN, EN) bind(C, name='sodhip_rk_stage_lps_f64')
+         import :: c_int, c_ptr, c_double, stage_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, mu_e, invMl, tau
+         type(c_ptr), value :: rho1, q1, E1, rho, q, E, u, e_int, pr, aux_h, Tem, mu_fluid
+         type(c_ptr), value :: PM, PE, PX, PY, PZ, Rmass, Rmom, Rener, rhoN, qN, EN
+         integer(c_int), value :: p, nelem, npoin, npoin_w
+         type(stage_params_t), intent(in) :: prm
+         real(c_double), value :: c_lps, dt, lam1, lams, gam, gam_next
+      end function
+      integer(c_int) function c_eos_entropy_step(c, npoin, npoin_w, lpoin_w, g, Rgas, p_inf, &
+            rho_inf, mu_factor, rho, q, E, u, e_int, pr, csound, machno, Tem, eta, mu_fluid) &
+            bind(C, name='sodhip_eos_entropy_step_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, lpoin_w, rho, q, E, u, e_int, pr, csound, machno, Tem, eta
+         type(c_ptr), value :: mu_fluid
+         integer(c_int), value :: npoin, npoin_w
+         real(c_double), value :: g, Rgas, p_inf, rho_inf, mu_factor
+      end function
+      integer(c_int) function c_entropy_convec_residual(c, p, nelem, npoin, npoin_w, connec, &
+            lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, u, eta, invMl, factor_comp, dt, Reta, &
+            auxReta) bind(C, name='sodhip_entropy_convec_residual_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, u, eta, invMl
+         type(c_ptr), value :: Reta, auxReta
+         integer(c_int), value :: p, nelem, npoin, npoin_w
+         real(c_double), value :: factor_comp, dt
+      end function
+      integer(c_int) function c_rk4_stage(c, p, nelem, npoin, npoin_w, connec, lpoin_w, opp, He, &
+            gpvol, dl, inv, gI, gJ, gK, mu_e, invMl, prm, dt, a_next, b, first, last, rho, q, E, &
+            srho, sq, sE, arho, aq, aE, Smass, Smom, Sener, Rmass, Rmom, Rener) &
+            bind(C, name='sodhip_rk4_stage_f64')
+         import :: c_int, c_ptr, c_double, stage_params_t
+         type(c_ptr), value :: c, connec, lpoin_w, He, gpvol, dl, inv, gI, gJ, gK, mu_e, invMl
+         type(c_ptr), value :: rho, q, E, srho, sq, sE, arho, aq, aE, Smass, Smom, Sener
+         type(c_ptr), value :: Rmass, Rmom, Rener
+         integer(c_int), value :: p, nelem, npoin, npoin_w, opp, first, last
+         type(stage_params_t), intent(in) :: prm
+         real(c_double), value :: dt, a_next, b
+      end function
+      integer(c_int) function c_rk4_update(c, npoin, npoin_w, lpoin_w, invMl, dt, a_next, b, &
+            first, last, rho, q, E, arho, aq, aE, Rmass, Rmom, Rener, Smass, Smom, Sener) &
+            bind(C, name='sodhip_rk4_stage_update_f64')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: c, lpoin_w, invMl, rho, q, E, arho, aq, aE, Rmass, Rmom, Rener
+         type(c_ptr), value :: Smass, Smom, Sener
+         integer(c_int), value :: npoin, npoin_w, first, last
+         real(c_double), value :: dt, a_next, b
+      end function
    end interface
 
 contains
@@ -312,6 +396,19 @@ contains
       call check(c_update_host(ctx, c_loc(a), size(a, kind=c_size_t) * 8_c_size_t), &
                  'sodhip_update_host')
    end subroutine upd_host_r8
+
+   ! dst = src on the device for mapped arrays of equal size (an `!$acc
+   ! kernels` assignment on present data, e.g. eta(:,1) = eta(:,2))
+   subroutine sod2d_hip_copy(src, dst)
+      real(c_double), dimension(..), intent(in), target :: src
+      real(c_double), dimension(..), intent(inout), target :: dst
+      if (size(src) /= size(dst)) then
+         write(*,'(a)') 'mod_sod2d_hip: sod2d_hip_copy of arrays of different sizes'
+         error stop 1
+      end if
+      call check(c_copy_present(ctx, c_loc(src), c_loc(dst), size(src, kind=c_size_t) * 8_c_size_t), &
+                 'sodhip_copy_present')
+   end subroutine sod2d_hip_copy
 
    subroutine upd_host_i4(a)
       integer(c_int), dimension(..), intent(inout), target :: a
@@ -690,5 +787,209 @@ contains
          gam, dev(rho1,'rho1'), dev(q1,'q1'), dev(E1,'E1'), dev(rho,'rho'), dev(q,'q'), &
          dev(E,'E'), dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener')), 'rk_stage_update')
    end subroutine rk_stage_update
+
+   ! ------------------------------------------------------------ fused stages
+   ! One stage of rk_4_ls_main (time_integ_ls.f90:339-361) with updateF
+   ! (:489-691, flag_lps_stab = .false.) fused in: primitives and Sutherland
+   ! at (rho, q, E), R = -(full_convec_ijk + full_diffusion_ijk), the lumped
+   ! inverse, y <- lam1*y1 + lams*y + gam*dt*R.  R*invMl goes to Rmass/Rmom/
+   ! Rener when they are present.  With gam_next and rho_next/q_next/E_next
+   ! the next step's first stage at fixed dt is stored as well
+   ! (sodhip_rk_stage_next_f64).  Operator precision follows rp.
+   subroutine rk_stage_fused(nelem,npoin,npoin_w,connec,lpoin_w,He,gpvol,dlxigp_ip,invAtoIJK, &
+         gmshAtoI,gmshAtoJ,gmshAtoK,mu_e,invMl,gamma_gas,Rgas,Cp,Prt,mu_factor,dt,lam1,lams, &
+         gam,rho1,q1,E1,rho,q,E,Rmass,Rmom,Rener,gam_next,rho_next,q_next,E_next)
+      integer(4), intent(in)    :: nelem, npoin, npoin_w
+      integer(4), intent(in)    :: connec(nelem,nnode), lpoin_w(npoin_w)
+      real(rp),   intent(in)    :: He(ndime,ndime,ngaus,nelem), gpvol(1,ngaus,nelem)
+      real(rp),   intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
+      integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
+      integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)    :: mu_e(nelem,ngaus), invMl(npoin)
+      real(rp),   intent(in)    :: gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam
+      real(8),    intent(in)    :: rho1(npoin), q1(npoin,ndime), E1(npoin)
+      real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(8),    intent(inout), optional :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp),   intent(in),    optional :: gam_next
+      real(8),    intent(inout), optional :: rho_next(npoin), q_next(npoin,ndime), E_next(npoin)
+      type(stage_params_t) :: prm
+      type(c_ptr) :: pm, pq, pe
+      prm = stage_params_t(real(gamma_gas,8), real(Rgas,8), real(Cp,8), real(Prt,8), &
+         real(mu_factor,8), 1.0d0, 1.0d0, -1.0d0, 0.0d0)
+      pm = c_null_ptr; pq = c_null_ptr; pe = c_null_ptr
+      if (present(Rmass)) then
+         pm = dev(Rmass,'Rmass'); pq = dev(Rmom,'Rmom'); pe = dev(Rener,'Rener')
+      end if
+      if (present(gam_next)) then
+         call check(c_rk_stage_next(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+            dev(lpoin_w,'lpoin_w'), rp, dev(He,'He'), dev(gpvol,'gpvol'), &
+            dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), &
+            dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), dev(mu_e,'mu_e'), &
+            dev(invMl,'invMl'), prm, real(dt,8), real(lam1,8), real(lams,8), real(gam,8), &
+            dev(rho1,'rho1'), dev(q1,'q1'), dev(E1,'E1'), dev(rho,'rho'), dev(q,'q'), &
+            dev(E,'E'), pm, pq, pe, real(gam_next,8), dev(rho_next,'rho_next'), &
+            dev(q_next,'q_next'), dev(E_next,'E_next')), 'rk_stage_fused (next)')
+      else
+         call check(c_rk_stage(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+            dev(lpoin_w,'lpoin_w'), rp, dev(He,'He'), dev(gpvol,'gpvol'), &
+            dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), &
+            dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), dev(mu_e,'mu_e'), &
+            dev(invMl,'invMl'), prm, real(dt,8), real(lam1,8), real(lams,8), real(gam,8), &
+            dev(rho1,'rho1'), dev(q1,'q1'), dev(E1,'E1'), dev(rho,'rho'), dev(q,'q'), &
+            dev(E,'E'), pm, pq, pe), 'rk_stage_fused')
+      end if
+   end subroutine rk_stage_fused
+
+   ! One stage of the LPS path (flag_lps_stab, time_integ_ls.f90:607-611):
+   ! updateF with full_proj_ijk and full_diff_stab_ijk, the lumped solve and
+   ! the stage combine (sodhip_rk_stage_lps_f64, FP64).
+   subroutine rk_stage_lps_fused(nelem,npoin,npoin_w,connec,lpoin_w,He,gpvol,dlxigp_ip, &
+         invAtoIJK,gmshAtoI,gmshAtoJ,gmshAtoK,mu_e,invMl,tau_stab,gamma_gas,Rgas,Cp,Prt, &
+         mu_factor,dt,lam1,lams,gam,rho1,q1,E1,rho,q,E,u,e_int,pr,aux_h,Tem,mu_fluid,ProjMass, &
+         ProjEner,ProjMX,ProjMY,ProjMZ,Rmass,Rmom,Rener,gam_next,rho_next,q_next,E_next)
+      integer(4), intent(in)    :: nelem, npoin, npoin_w
+      integer(4), intent(in)    :: connec(nelem,nnode), lpoin_w(npoin_w)
+      real(8),    intent(in)    :: He(ndime,ndime,ngaus,nelem), gpvol(1,ngaus,nelem)
+      real(8),    intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
+      integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
+      integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(8),    intent(in)    :: mu_e(nelem,ngaus), invMl(npoin), tau_stab(nelem)
+      real(8),    intent(in)    :: gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam
+      real(8),    intent(in)    :: rho1(npoin), q1(npoin,ndime), E1(npoin)
+      real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(8),    intent(inout) :: u(npoin,ndime), e_int(npoin), pr(npoin), aux_h(npoin)
+      real(8),    intent(inout) :: Tem(npoin), mu_fluid(npoin)
+      real(8),    intent(inout) :: ProjMass(npoin,ndime), ProjEner(npoin,ndime)
+      real(8),    intent(inout) :: ProjMX(npoin,ndime), ProjMY(npoin,ndime), ProjMZ(npoin,ndime)
+      real(8),    intent(inout), optional :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(8),    intent(in),    optional :: gam_next
+      real(8),    intent(inout), optional :: rho_next(npoin), q_next(npoin,ndime), E_next(npoin)
+      type(stage_params_t) :: prm
+      type(c_ptr) :: pm, pq, pe, nr, nq, ne
+      real(8) :: gn
+      prm = stage_params_t(gamma_gas, Rgas, Cp, Prt, mu_factor, 1.0d0, 1.0d0, -1.0d0, 0.0d0)
+      pm = c_null_ptr; pq = c_null_ptr; pe = c_null_ptr
+      nr = c_null_ptr; nq = c_null_ptr; ne = c_null_ptr; gn = 0.0d0
+      if (present(Rmass)) then
+         pm = dev(Rmass,'Rmass'); pq = dev(Rmom,'Rmom'); pe = dev(Rener,'Rener')
+      end if
+      if (present(gam_next)) then
+         gn = gam_next
+         nr = dev(rho_next,'rho_next'); nq = dev(q_next,'q_next'); ne = dev(E_next,'E_next')
+      end if
+      call check(c_rk_stage_lps(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+         dev(lpoin_w,'lpoin_w'), dev(He,'He'), dev(gpvol,'gpvol'), dev(dlxigp_ip,'dlxigp_ip'), &
+         dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), dev(gmshAtoJ,'gmshAtoJ'), &
+         dev(gmshAtoK,'gmshAtoK'), dev(mu_e,'mu_e'), dev(invMl,'invMl'), &
+         dev(tau_stab,'tau_stab'), prm, real(c_lps_comp,8), dt, lam1, lams, gam, &
+         dev(rho1,'rho1'), dev(q1,'q1'), dev(E1,'E1'), dev(rho,'rho'), dev(q,'q'), dev(E,'E'), &
+         dev(u,'u'), dev(e_int,'e_int'), dev(pr,'pr'), dev(aux_h,'aux_h'), dev(Tem,'Tem'), &
+         dev(mu_fluid,'mu_fluid'), dev(ProjMass,'ProjMass'), dev(ProjEner,'ProjEner'), &
+         dev(ProjMX,'ProjMX'), dev(ProjMY,'ProjMY'), dev(ProjMZ,'ProjMZ'), pm, pq, pe, gn, nr, &
+         nq, ne), 'rk_stage_lps_fused')
+   end subroutine rk_stage_lps_fused
+
+   ! rk_4_ls_main's end-of-step node updates in one pass (time_integ_ls.f90:
+   ! 391-407, thermo_entropy mod_entropy_viscosity.f90:411-433 without the
+   ! flux, sutherland_viscosity with a uniform factor): eta is eta(npoin,2),
+   ! level 2 is written.
+   subroutine eos_entropy_step(npoin,npoin_w,lpoin_w,gamma_gas,Rgas,p_inf,rho_inf,mu_factor, &
+         rho,q,E,u,e_int,pr,csound,machno,Tem,eta,mu_fluid)
+      integer(4), intent(in)    :: npoin, npoin_w, lpoin_w(npoin_w)
+      real(8),    intent(in)    :: gamma_gas, Rgas, p_inf, rho_inf, mu_factor
+      real(8),    intent(in)    :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(8),    intent(inout) :: u(npoin,ndime), e_int(npoin), pr(npoin), csound(npoin)
+      real(8),    intent(inout) :: machno(npoin), Tem(npoin), eta(npoin,2), mu_fluid(npoin)
+      call check(c_eos_entropy_step(ctx, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), gamma_gas, &
+         Rgas, p_inf, rho_inf, mu_factor, dev(rho,'rho'), dev(q,'q'), dev(E,'E'), dev(u,'u'), &
+         dev(e_int,'e_int'), dev(pr,'pr'), dev(csound,'csound'), dev(machno,'machno'), &
+         dev(Tem,'Tem'), dev(eta,'eta'), dev(mu_fluid,'mu_fluid')), 'eos_entropy_step')
+   end subroutine eos_entropy_step
+
+   ! The end-of-step entropy residual (time_integ_ls.f90:429-443): Reta(:,2) of
+   ! f_eta = u*eta(:,1), its lumped solve, auxReta = 1.5 Reta2 - 0.5 Reta1 +
+   ! factor_comp (eta2-eta1)/dt, then Reta(:,1) <- Reta(:,2).
+   subroutine entropy_convec_residual(nelem,npoin,npoin_w,connec,lpoin_w,He,gpvol,dlxigp_ip, &
+         invAtoIJK,gmshAtoI,gmshAtoJ,gmshAtoK,u,eta,invMl,factor_comp,dt,Reta,auxReta)
+      integer(4), intent(in)    :: nelem, npoin, npoin_w
+      integer(4), intent(in)    :: connec(nelem,nnode), lpoin_w(npoin_w)
+      real(8),    intent(in)    :: He(ndime,ndime,ngaus,nelem), gpvol(1,ngaus,nelem)
+      real(8),    intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
+      integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
+      integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(8),    intent(in)    :: u(npoin,ndime), eta(npoin,2), invMl(npoin)
+      real(8),    intent(in)    :: factor_comp, dt
+      real(8),    intent(inout) :: Reta(npoin,2), auxReta(npoin)
+      call check(c_entropy_convec_residual(ctx, porder, nelem, npoin, npoin_w, &
+         dev(connec,'connec'), dev(lpoin_w,'lpoin_w'), dev(He,'He'), dev(gpvol,'gpvol'), &
+         dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), &
+         dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), dev(u,'u'), dev(eta,'eta'), &
+         dev(invMl,'invMl'), factor_comp, dt, dev(Reta,'Reta'), dev(auxReta,'auxReta')), &
+         'entropy_convec_residual')
+   end subroutine entropy_convec_residual
+
+   ! One stage of the classical RK4 rk_4_main (time_integ.f90:250-470,
+   ! compiled out in the reference) with the RHS fused: R at the stage state
+   ! (srho, sq, sE), operators without fact, lumped inverse, Rsum <- Rsum + b R,
+   ! then aux = y - dt a_next R, or y <- y - dt Rsum at the last stage.
+   subroutine rk4_stage_fused(nelem,npoin,npoin_w,connec,lpoin_w,He,gpvol,dlxigp_ip,invAtoIJK, &
+         gmshAtoI,gmshAtoJ,gmshAtoK,mu_e,invMl,gamma_gas,Rgas,Cp,Prt,mu_factor,dt,a_next,b, &
+         first,last,rho,q,E,srho,sq,sE,aux_rho,aux_q,aux_E,Rmass_sum,Rmom_sum,Rener_sum)
+      integer(4), intent(in)    :: nelem, npoin, npoin_w
+      integer(4), intent(in)    :: connec(nelem,nnode), lpoin_w(npoin_w)
+      real(rp),   intent(in)    :: He(ndime,ndime,ngaus,nelem), gpvol(1,ngaus,nelem)
+      real(rp),   intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
+      integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
+      integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
+      real(rp),   intent(in)    :: mu_e(nelem,ngaus), invMl(npoin)
+      real(rp),   intent(in)    :: gamma_gas, Rgas, Cp, Prt, mu_factor, dt, a_next, b
+      logical,    intent(in)    :: first, last
+      real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(8),    intent(in)    :: srho(npoin), sq(npoin,ndime), sE(npoin)
+      real(8),    intent(inout) :: aux_rho(npoin), aux_q(npoin,ndime), aux_E(npoin)
+      real(8),    intent(inout) :: Rmass_sum(npoin), Rmom_sum(npoin,ndime), Rener_sum(npoin)
+      type(stage_params_t) :: prm
+      integer(c_int) :: fi, la
+      prm = stage_params_t(real(gamma_gas,8), real(Rgas,8), real(Cp,8), real(Prt,8), &
+         real(mu_factor,8), 1.0d0, 1.0d0, 1.0d0, 0.0d0)
+      fi = 0; la = 0
+      if (first) fi = 1
+      if (last) la = 1
+      call check(c_rk4_stage(ctx, porder, nelem, npoin, npoin_w, dev(connec,'connec'), &
+         dev(lpoin_w,'lpoin_w'), rp, dev(He,'He'), dev(gpvol,'gpvol'), &
+         dev(dlxigp_ip,'dlxigp_ip'), dev(invAtoIJK,'invAtoIJK'), dev(gmshAtoI,'gmshAtoI'), &
+         dev(gmshAtoJ,'gmshAtoJ'), dev(gmshAtoK,'gmshAtoK'), dev(mu_e,'mu_e'), &
+         dev(invMl,'invMl'), prm, real(dt,8), real(a_next,8), real(b,8), fi, la, dev(rho,'rho'), &
+         dev(q,'q'), dev(E,'E'), dev(srho,'srho'), dev(sq,'sq'), dev(sE,'sE'), &
+         dev(aux_rho,'aux_rho'), dev(aux_q,'aux_q'), dev(aux_E,'aux_E'), &
+         dev(Rmass_sum,'Rmass_sum'), dev(Rmom_sum,'Rmom_sum'), dev(Rener_sum,'Rener_sum'), &
+         c_null_ptr, c_null_ptr, c_null_ptr), 'rk4_stage_fused')
+   end subroutine rk4_stage_fused
+
+   ! The node loops of an rk_4_main stage on assembled residuals (time_integ.f90:
+   ! 266-275, 431-451, 460-468), with the lumped inverse when invMl is present.
+   subroutine rk4_stage_update(npoin,npoin_w,lpoin_w,dt,a_next,b,first,last,rho,q,E,aux_rho, &
+         aux_q,aux_E,Rmass,Rmom,Rener,Rmass_sum,Rmom_sum,Rener_sum,invMl)
+      integer(4), intent(in)    :: npoin, npoin_w, lpoin_w(npoin_w)
+      real(8),    intent(in)    :: dt, a_next, b
+      logical,    intent(in)    :: first, last
+      real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
+      real(8),    intent(inout) :: aux_rho(npoin), aux_q(npoin,ndime), aux_E(npoin)
+      real(8),    intent(inout) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(8),    intent(inout) :: Rmass_sum(npoin), Rmom_sum(npoin,ndime), Rener_sum(npoin)
+      real(8),    intent(in), optional :: invMl(npoin)
+      type(c_ptr) :: pinv
+      integer(c_int) :: fi, la
+      pinv = c_null_ptr
+      if (present(invMl)) pinv = dev(invMl,'invMl')
+      fi = 0; la = 0
+      if (first) fi = 1
+      if (last) la = 1
+      call check(c_rk4_update(ctx, npoin, npoin_w, dev(lpoin_w,'lpoin_w'), pinv, dt, a_next, b, &
+         fi, la, dev(rho,'rho'), dev(q,'q'), dev(E,'E'), dev(aux_rho,'aux_rho'), &
+         dev(aux_q,'aux_q'), dev(aux_E,'aux_E'), dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), &
+         dev(Rener,'Rener'), dev(Rmass_sum,'Rmass_sum'), dev(Rmom_sum,'Rmom_sum'), &
+         dev(Rener_sum,'Rener_sum')), 'rk4_stage_update')
+   end subroutine rk4_stage_update
 
 end module mod_sod2d_hip
