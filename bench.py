@@ -64,7 +64,7 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "64")))
     T = run.T
 
-    def rhs():
+    def rhs(cores=cores):
         lw = run.lpoin_w
         O.eos_stage(lw, t.gamma_gas, t.Rgas, run.rho[1], run.q[1], run.E[1], run.u[1],
                     run.e_int[1], run.pr[1], run.aux_h, run.Tem[1])
@@ -81,21 +81,29 @@ def cpu_baseline(budget_s: float = 15.0) -> dict:
         O.rk_ls_update(lw, 0.0, 1.0, 0.0, run.rho[0], run.q[0], run.E[0], run.rho[1], run.q[1],
                        run.E[1], run.Rmass, run.Rmom, run.Rener)
 
-    rhs()
-    t0 = time.perf_counter()
-    rhs()
-    t1 = time.perf_counter() - t0
-    reps = max(1, min(200, int(budget_s / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        rhs()
-    dt = (time.perf_counter() - t0) / reps
+    def timed(nc, budget):
+        rhs(nc)
+        t0 = time.perf_counter()
+        rhs(nc)
+        t1 = time.perf_counter() - t0
+        reps = max(1, min(200, int(budget / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rhs(nc)
+        return (time.perf_counter() - t0) / reps, reps
+
+    dt, reps = timed(cores, budget_s)
+    # the same RHS on one core (a few evaluations)
+    dt1, reps1 = timed(1, budget_s / 3)
     N = g.mesh.npoin_w
     return {"value": round(5 * N / dt / 1e9, 6), "unit": "GDOF/s per RK stage", "cores": cores,
             "kind": "port",
             "sample": f"updateF + stage combine of the CPU oracle, TGV p=3 16^3 elements "
                       f"({N} unique nodes), FP64, {reps} RHS evaluations, "
-                      f"{dt * 1e3:.2f} ms each, OpenMP element loop on {cores} threads"}
+                      f"{dt * 1e3:.2f} ms each, OpenMP element loop on {cores} threads",
+            "single_core": {"value": round(5 * N / dt1 / 1e9, 6), "cores": 1,
+                            "sample": f"the same, {reps1} RHS evaluations on 1 thread, "
+                                      f"{dt1 * 1e3:.2f} ms each"}}
 
 
 def run_operators(a) -> dict:

@@ -262,6 +262,15 @@ int sodhip_generic_scalar_convec_ijk_f64(sodhip_ctx ctx, int porder, int nelem, 
                                          const int *gmshAtoK, const double *q, const double *eta,
                                          const double *u, double *Rconvec, int initialize,
                                          double fact);
+/* The same operator on real(4) arrays (the reference's default USE_RP=4 build,
+ * CMakeLists.txt:30): FP32 arithmetic. */
+int sodhip_generic_scalar_convec_ijk_f32(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                         const int *connec, const float *He, const float *gpvol,
+                                         const float *dlxigp_ip, const int *invAtoIJK,
+                                         const int *gmshAtoI, const int *gmshAtoJ,
+                                         const int *gmshAtoK, const float *q, const float *eta,
+                                         const float *u, float *Rconvec, int initialize,
+                                         float fact);
 
 /* Module globals read by smart_visc_spectral (mod_numerical_params.f90:1-175). */
 typedef struct {
@@ -280,6 +289,14 @@ int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int np
                                    const double *Tem, const double *eta, const double *helem,
                                    const double *helem_k, const int *invAtoIJK,
                                    const sodhip_visc_params *prm, double *mu_e, double *mue_l);
+/* The same sensor on real(4) arrays (USE_RP=4): FP32 inputs and outputs, FP64
+ * arithmetic and entropy normalisation. */
+int sodhip_smart_visc_spectral_f32(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                                   const int *connec, const int *lpoin_w, const float *Reta,
+                                   float gamma_gas, const float *rho, const float *u,
+                                   const float *Tem, const float *eta, const float *helem,
+                                   const float *helem_k, const int *invAtoIJK,
+                                   const sodhip_visc_params *prm, float *mu_e, float *mue_l);
 
 /* smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227), the sensor of the
  * reference's live LS-RK path (time_integ_ls.f90:292,455): per element
@@ -343,6 +360,11 @@ int sodhip_lumped_solver_scal_f64(sodhip_ctx ctx, int npoin, int npoin_w, const 
                                   const double *invMl, double *R);
 int sodhip_lumped_solver_vect_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
                                   const double *invMl, double *R);
+/* real(4) arrays (USE_RP=4), FP32 arithmetic */
+int sodhip_lumped_solver_scal_f32(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                  const float *invMl, float *R);
+int sodhip_lumped_solver_vect_f32(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                  const float *invMl, float *R);
 
 /* RK stage update with the lumped-mass inverse fused in (the reference has these
  * as inline loops: mod_solver.f90:40-104 then time_integ_ls.f90:323-331,350-358):
@@ -355,6 +377,11 @@ int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int
                                double gam, const double *rho1, const double *q1,
                                const double *E1, double *rho, double *q, double *E,
                                double *Rmass, double *Rmom, double *Rener);
+/* real(4) arrays (USE_RP=4), FP32 arithmetic */
+int sodhip_rk_stage_update_f32(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                               const float *invMl, float dt, float lam1, float lams, float gam,
+                               const float *rho1, const float *q1, const float *E1, float *rho,
+                               float *q, float *E, float *Rmass, float *Rmom, float *Rener);
 
 /* The node update of one stage of the classical RK4 integrator rk_4_main
  * (time_integ.f90; compiled out in the reference by #if 0, restated from its

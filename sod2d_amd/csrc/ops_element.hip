@@ -2249,6 +2249,17 @@ extern "C" int sodhip_generic_scalar_convec_ijk_f64(
                                  initialize, fact);
 }
 
+extern "C" int sodhip_generic_scalar_convec_ijk_f32(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec, const float *He,
+    const float *gpvol, const float *dlxigp_ip, const int *invAtoIJK, const int *gmshAtoI,
+    const int *gmshAtoJ, const int *gmshAtoK, const float *q, const float *eta, const float *u,
+    float *Rconvec, int initialize, float fact)
+{
+    return scalar_convec<float>(ctx, porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, q, eta, u, Rconvec,
+                                initialize, fact);
+}
+
 extern "C" int sodhip_entropy_convec_residual_f64(
     sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
     const int *lpoin_w, const double *He, const double *gpvol, const double *dlxigp_ip,

@@ -5,62 +5,63 @@
 namespace sodhip {
 
 // lumped_solver_scal_opt / _vect_opt (mod_solver.f90:40-104): R <- R*invMl over
-// working nodes, for ncomp components of an (npoin,ncomp) array.
+// working nodes, for ncomp components of an (npoin,ncomp) array, in the
+// arrays' precision T.
+template <typename T>
 __global__ __launch_bounds__(256) void k_lumped(int npoin, int npoin_w,
                                                 const int *__restrict__ lpoin_w,
-                                                const double *__restrict__ invMl,
-                                                double *__restrict__ R, int ncomp)
+                                                const T *__restrict__ invMl,
+                                                T *__restrict__ R, int ncomp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
-    const double m = invMl[n];
+    const T m = invMl[n];
     for (int c = 0; c < ncomp; ++c) R[n + (size_t)npoin * c] *= m;
 }
 
 // R <- R*invMl (stored back), y <- lam1*y1 + lams*y + gdt*R over working nodes
 // for the five conserved fields (time_integ_ls.f90:323-331,350-358).  RD1 / RDY:
 // y1 / y enters (non-zero coefficient); otherwise the term is 0 * 0, the same
-// value as 0 * (finite y), without reading the field.
-template <bool RD1, bool RDY>
+// value as 0 * (finite y), without reading the field.  Arithmetic in the
+// arrays' precision T.
+template <typename T, bool RD1, bool RDY>
 __global__ __launch_bounds__(256) void k_rk_update(int npoin, int npoin_w,
                                                    const int *__restrict__ lpoin_w,
-                                                   const double *__restrict__ invMl, double lam1,
-                                                   double lams, double gdt,
-                                                   const double *__restrict__ rho1,
-                                                   const double *__restrict__ q1,
-                                                   const double *__restrict__ E1,
-                                                   double *__restrict__ rho, double *__restrict__ q,
-                                                   double *__restrict__ E,
-                                                   double *__restrict__ Rmass,
-                                                   double *__restrict__ Rmom,
-                                                   double *__restrict__ Rener)
+                                                   const T *__restrict__ invMl, T lam1, T lams,
+                                                   T gdt, const T *__restrict__ rho1,
+                                                   const T *__restrict__ q1,
+                                                   const T *__restrict__ E1, T *__restrict__ rho,
+                                                   T *__restrict__ q, T *__restrict__ E,
+                                                   T *__restrict__ Rmass, T *__restrict__ Rmom,
+                                                   T *__restrict__ Rener)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
+    const T z = T(0);
     if (invMl) {
-        const double m = invMl[n];
-        const double rm = Rmass[n] * m;
+        const T m = invMl[n];
+        const T rm = Rmass[n] * m;
         Rmass[n] = rm;
-        rho[n] = lam1 * (RD1 ? rho1[n] : 0.0) + lams * (RDY ? rho[n] : 0.0) + gdt * rm;
-        const double re = Rener[n] * m;
+        rho[n] = lam1 * (RD1 ? rho1[n] : z) + lams * (RDY ? rho[n] : z) + gdt * rm;
+        const T re = Rener[n] * m;
         Rener[n] = re;
-        E[n] = lam1 * (RD1 ? E1[n] : 0.0) + lams * (RDY ? E[n] : 0.0) + gdt * re;
+        E[n] = lam1 * (RD1 ? E1[n] : z) + lams * (RDY ? E[n] : z) + gdt * re;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t k = n + (size_t)npoin * d;
-            const double rq = Rmom[k] * m;
+            const T rq = Rmom[k] * m;
             Rmom[k] = rq;
-            q[k] = lam1 * (RD1 ? q1[k] : 0.0) + lams * (RDY ? q[k] : 0.0) + gdt * rq;
+            q[k] = lam1 * (RD1 ? q1[k] : z) + lams * (RDY ? q[k] : z) + gdt * rq;
         }
     } else { // R already carries the mass inverse (the residual kept from updateF)
-        rho[n] = lam1 * (RD1 ? rho1[n] : 0.0) + lams * (RDY ? rho[n] : 0.0) + gdt * Rmass[n];
-        E[n] = lam1 * (RD1 ? E1[n] : 0.0) + lams * (RDY ? E[n] : 0.0) + gdt * Rener[n];
+        rho[n] = lam1 * (RD1 ? rho1[n] : z) + lams * (RDY ? rho[n] : z) + gdt * Rmass[n];
+        E[n] = lam1 * (RD1 ? E1[n] : z) + lams * (RDY ? E[n] : z) + gdt * Rener[n];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const size_t k = n + (size_t)npoin * d;
-            q[k] = lam1 * (RD1 ? q1[k] : 0.0) + lams * (RDY ? q[k] : 0.0) + gdt * Rmom[k];
+            q[k] = lam1 * (RD1 ? q1[k] : z) + lams * (RDY ? q[k] : z) + gdt * Rmom[k];
         }
     }
 }
@@ -69,16 +70,17 @@ __global__ __launch_bounds__(256) void k_rk_update(int npoin, int npoin_w,
 
 using namespace sodhip;
 
-static int lumped(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w, const double *invMl,
-                  double *R, int ncomp)
+template <typename T>
+static int lumped(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w, const T *invMl,
+                  T *R, int ncomp)
 {
     SODHIP_REQUIRE(ctx && lpoin_w && invMl && R, "null argument");
     SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
     if (npoin_w == 0) return SODHIP_OK;
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     int tok = prof_begin(ctx, 3);
-    k_lumped<<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(npoin, npoin_w, lpoin_w, invMl, R,
-                                                             ncomp);
+    k_lumped<T><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(npoin, npoin_w, lpoin_w, invMl, R,
+                                                                ncomp);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
@@ -96,12 +98,22 @@ extern "C" int sodhip_lumped_solver_vect_f64(sodhip_ctx ctx, int npoin, int npoi
     return lumped(ctx, npoin, npoin_w, lpoin_w, invMl, R, 3);
 }
 
-extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w,
-                                          const int *lpoin_w, const double *invMl, double dt,
-                                          double lam1, double lams, double gam,
-                                          const double *rho1, const double *q1, const double *E1,
-                                          double *rho, double *q, double *E, double *Rmass,
-                                          double *Rmom, double *Rener)
+extern "C" int sodhip_lumped_solver_scal_f32(sodhip_ctx ctx, int npoin, int npoin_w,
+                                             const int *lpoin_w, const float *invMl, float *R)
+{
+    return lumped(ctx, npoin, npoin_w, lpoin_w, invMl, R, 1);
+}
+
+extern "C" int sodhip_lumped_solver_vect_f32(sodhip_ctx ctx, int npoin, int npoin_w,
+                                             const int *lpoin_w, const float *invMl, float *R)
+{
+    return lumped(ctx, npoin, npoin_w, lpoin_w, invMl, R, 3);
+}
+
+template <typename T>
+static int rk_stage_update(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                           const T *invMl, T dt, T lam1, T lams, T gam, const T *rho1, const T *q1,
+                           const T *E1, T *rho, T *q, T *E, T *Rmass, T *Rmom, T *Rener)
 {
     SODHIP_REQUIRE(ctx && lpoin_w && rho1 && q1 && E1 && rho && q && E && Rmass && Rmom && Rener,
                    "null argument");
@@ -110,14 +122,14 @@ extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     int tok = prof_begin(ctx, 3);
 #define SODHIP_RKU(R1, RY)                                                                     \
-    k_rk_update<R1, RY><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(                       \
+    k_rk_update<T, R1, RY><<<(npoin_w + 255) / 256, 256, 0, ctx->stream>>>(                    \
         npoin, npoin_w, lpoin_w, invMl, lam1, lams, gam * dt, rho1, q1, E1, rho, q, E, Rmass,  \
         Rmom, Rener)
-    if (lam1 != 0.0 && lams != 0.0)
+    if (lam1 != T(0) && lams != T(0))
         SODHIP_RKU(true, true);
-    else if (lam1 != 0.0)
+    else if (lam1 != T(0))
         SODHIP_RKU(true, false);
-    else if (lams != 0.0)
+    else if (lams != T(0))
         SODHIP_RKU(false, true);
     else
         SODHIP_RKU(false, false);
@@ -125,6 +137,27 @@ extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
+}
+
+extern "C" int sodhip_rk_stage_update_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                          const int *lpoin_w, const double *invMl, double dt,
+                                          double lam1, double lams, double gam,
+                                          const double *rho1, const double *q1, const double *E1,
+                                          double *rho, double *q, double *E, double *Rmass,
+                                          double *Rmom, double *Rener)
+{
+    return rk_stage_update<double>(ctx, npoin, npoin_w, lpoin_w, invMl, dt, lam1, lams, gam, rho1,
+                                   q1, E1, rho, q, E, Rmass, Rmom, Rener);
+}
+
+extern "C" int sodhip_rk_stage_update_f32(sodhip_ctx ctx, int npoin, int npoin_w,
+                                          const int *lpoin_w, const float *invMl, float dt,
+                                          float lam1, float lams, float gam, const float *rho1,
+                                          const float *q1, const float *E1, float *rho, float *q,
+                                          float *E, float *Rmass, float *Rmom, float *Rener)
+{
+    return rk_stage_update<float>(ctx, npoin, npoin_w, lpoin_w, invMl, dt, lam1, lams, gam, rho1,
+                                  q1, E1, rho, q, E, Rmass, Rmom, Rener);
 }
 
 namespace sodhip {

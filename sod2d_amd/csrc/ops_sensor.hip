@@ -49,8 +49,10 @@ template <typename T> __device__ __forceinline__ T block_max(T v, T *sh)
 }
 
 // partial sums of eta over working nodes: block b owns a contiguous slice
+// (FP64 sums for either storage type)
+template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int *__restrict__ lpoin_w,
-                                                         const double *__restrict__ eta,
+                                                         const T *__restrict__ eta,
                                                          double *__restrict__ part)
 {
     __shared__ double sh[kRedThreads];
@@ -64,22 +66,24 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int 
     for (; i + 3 * bs < i1; i += 4 * bs) {
         const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
                   n3 = lpoin_w[i + 3 * bs];
-        const double v0 = eta[n0 - 1], v1 = eta[n1 - 1], v2 = eta[n2 - 1], v3 = eta[n3 - 1];
+        const double v0 = (double)eta[n0 - 1], v1 = (double)eta[n1 - 1],
+                     v2 = (double)eta[n2 - 1], v3 = (double)eta[n3 - 1];
         s += v0;
         s += v1;
         s += v2;
         s += v3;
     }
-    for (; i < i1; i += bs) s += eta[lpoin_w[i] - 1];
+    for (; i < i1; i += bs) s += (double)eta[lpoin_w[i] - 1];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // every block: mean = (sum of partial sums)/npoin_w in a fixed order, then its
 // slice's max |eta - mean|
+template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
                                                             const int *__restrict__ lpoin_w,
-                                                            const double *__restrict__ eta,
+                                                            const T *__restrict__ eta,
                                                             const double *__restrict__ part,
                                                             int nparts, double *__restrict__ pmax)
 {
@@ -95,11 +99,12 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
     for (; i + 3 * bs < i1; i += 4 * bs) {
         const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
                   n3 = lpoin_w[i + 3 * bs];
-        const double v0 = eta[n0 - 1], v1 = eta[n1 - 1], v2 = eta[n2 - 1], v3 = eta[n3 - 1];
+        const double v0 = (double)eta[n0 - 1], v1 = (double)eta[n1 - 1],
+                     v2 = (double)eta[n2 - 1], v3 = (double)eta[n3 - 1];
         m = fmax(fmax(m, fabs(v0 - mean)), fabs(v1 - mean));
         m = fmax(fmax(m, fabs(v2 - mean)), fabs(v3 - mean));
     }
-    for (; i < i1; i += bs) m = fmax(m, fabs(eta[lpoin_w[i] - 1] - mean));
+    for (; i < i1; i += bs) m = fmax(m, fabs((double)eta[lpoin_w[i] - 1] - mean));
     m = block_max(m, sh);
     if (threadIdx.x == 0) pmax[blockIdx.x] = m;
 }
@@ -289,8 +294,8 @@ __global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int npoin,
 // arrays helem(nelem,nnode), mue_l and mu_e move as whole 128-byte lines: the
 // CH x NN block is read and written through LDS with thread t holding (node
 // t/CH, element t%CH), while the per-element arithmetic keeps lane = node.
-template <int P, int CH>
-__global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgs A)
+template <int P, int CH, typename S = double>
+__global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgsT<S> A)
 {
     constexpr int N1 = SG<P>::N1, NN = SG<P>::NN, NT = SG<P>::NT, NW = NT / 64;
     constexpr int BLK = CH * NN, ROWS = (BLK + NT - 1) / NT;
@@ -405,8 +410,8 @@ int entropy_norm_partials(sodhip_ctx ctx, int npoin_w, const int *lpoin_w, const
     int rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double));
     if (rc) return rc;
     double *part = (double *)ctx->scratch2.ptr;
-    k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
-    k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+    k_eta_sum<double><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+    k_eta_maxdev<double><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
                                                               kRedBlocks, part + kRedBlocks);
     SODHIP_TRY_HIP(hipGetLastError());
     *pmax = part + kRedBlocks;
@@ -417,14 +422,12 @@ int entropy_norm_partials(sodhip_ctx ctx, int npoin_w, const int *lpoin_w, const
 
 using namespace sodhip;
 
-extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
-                                              int npoin_w, const int *connec, const int *lpoin_w,
-                                              const double *Reta, double gamma_gas,
-                                              const double *rho, const double *u,
-                                              const double *Tem, const double *eta,
-                                              const double *helem, const double *helem_k,
-                                              const int *invAtoIJK, const sodhip_visc_params *prm,
-                                              double *mu_e, double *mue_l)
+template <typename T>
+static int smart_visc(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                      const int *connec, const int *lpoin_w, const T *Reta, double gamma_gas,
+                      const T *rho, const T *u, const T *Tem, const T *eta, const T *helem,
+                      const T *helem_k, const int *invAtoIJK, const sodhip_visc_params *prm,
+                      T *mu_e, T *mue_l)
 {
     SODHIP_REQUIRE(ctx && prm, "null argument");
     SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
@@ -443,11 +446,11 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     const bool normalise = prm->flag_normalise_entropy == 1;
     int tok = prof_begin(ctx, 2);
     if (normalise) {
-        k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
-        k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
-                                                                  kRedBlocks, pmax);
+        k_eta_sum<T><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+        k_eta_maxdev<T><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+                                                                     kRedBlocks, pmax);
     }
-    SensorArgs A{};
+    SensorArgsT<T> A{};
     A.nelem = nelem;
     A.npoin = npoin;
     A.connecT = m->connecT;
@@ -469,13 +472,40 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
     A.mu_e = mu_e;
     A.mue_l = mue_l;
     switch (porder) {
-    case 2: k_smart_visc_ch<2, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
-    case 3: k_smart_visc_ch<3, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
-    case 4: k_smart_visc_ch<4, 16><<<elem_grid(ctx, (nelem + 15) / 16, 8), SG<4>::NT, 0, ctx->stream>>>(A); break;
+    case 2: k_smart_visc_ch<2, 16, T><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
+    case 3: k_smart_visc_ch<3, 16, T><<<elem_grid(ctx, (nelem + 15) / 16, 8), 64, 0, ctx->stream>>>(A); break;
+    case 4: k_smart_visc_ch<4, 16, T><<<elem_grid(ctx, (nelem + 15) / 16, 8), SG<4>::NT, 0, ctx->stream>>>(A); break;
     }
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
+}
+
+extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                              int npoin_w, const int *connec, const int *lpoin_w,
+                                              const double *Reta, double gamma_gas,
+                                              const double *rho, const double *u,
+                                              const double *Tem, const double *eta,
+                                              const double *helem, const double *helem_k,
+                                              const int *invAtoIJK, const sodhip_visc_params *prm,
+                                              double *mu_e, double *mue_l)
+{
+    return smart_visc<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, gamma_gas,
+                              rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm, mu_e, mue_l);
+}
+
+extern "C" int sodhip_smart_visc_spectral_f32(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                              int npoin_w, const int *connec, const int *lpoin_w,
+                                              const float *Reta, float gamma_gas,
+                                              const float *rho, const float *u, const float *Tem,
+                                              const float *eta, const float *helem,
+                                              const float *helem_k, const int *invAtoIJK,
+                                              const sodhip_visc_params *prm, float *mu_e,
+                                              float *mue_l)
+{
+    return smart_visc<float>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta,
+                             (double)gamma_gas, rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm,
+                             mu_e, mue_l);
 }
 
 extern "C" int sodhip_smart_visc_spectral_imex_f64(
@@ -504,8 +534,8 @@ extern "C" int sodhip_smart_visc_spectral_imex_f64(
         // the reference assigns its module global ce (kept for later calls)
         prm->ce = ce_comp;
     } else if (normalise) {
-        k_eta_sum<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
-        k_eta_maxdev<<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
+        k_eta_sum<double><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
+        k_eta_maxdev<double><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
                                                                   kRedBlocks, pmax);
     }
     ImexArgs A{};

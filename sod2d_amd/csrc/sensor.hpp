@@ -13,16 +13,19 @@ template <int P> struct SG {
     static constexpr int NT = (NN + 63) / 64 * 64;
 };
 
-struct SensorArgs {
+// the sensor's arrays in the storage type T (double, or float for the FP32
+// interface); the arithmetic is FP64 either way
+template <typename T> struct SensorArgsT {
     int nelem, npoin;
     const int *connecT;
-    const double *Reta, *rho, *u, *Tem, *helem, *helem_k;
+    const T *Reta, *rho, *u, *Tem, *helem, *helem_k;
     const int *inv;
     double gamma, Rgas, ce, cmax, cglob, fact_low_mach;
     const double *pmax; // partial maxima (nparts) or nullptr when not normalising
     int nparts;
-    double *mu_e, *mue_l;
+    T *mu_e, *mue_l;
 };
+using SensorArgs = SensorArgsT<double>;
 
 // sod2d 1-D index s (1-based) -> geometric position along the line (1-based)
 __device__ __forceinline__ int geo_of(int s, int N1) { return s == 1 ? 1 : (s == 2 ? N1 : s - 1); }

@@ -17,13 +17,30 @@
 ! sod2d_hip_update_host.  A missing mapping or a failed launch stops the
 ! program with the library's message, as the reference stops on errors.
 !
-! The element order is fixed at compile time (-D__PORDER__, as the reference).
+! The element order is fixed at compile time (-D__PORDER__, as the reference),
+! and so is the real kind rp of the reference's real(rp) arguments
+! (-D__RP__=8 or 4, the reference's USE_RP, CMakeLists.txt:30; default 8).
+! With rp = 4 the operators, the sensor, the lumped solves and the stage
+! combine take real(4) arrays (the C-ABI's _f32 entry points); the live-path
+! and LPS operators exist in the rp = 8 build only.  The fused stages keep the
+! conserved state, the lumped-mass inverse and the combine in real(8) in both
+! builds; their operator arrays (He, gpvol, dlxigp_ip, mu_e) are real(rp).
+#ifndef __RP__
+#define __RP__ 8
+#endif
+#if __RP__ == 4
+#define RPSFX '_f32'
+#define C_RP c_float
+#else
+#define RPSFX '_f64'
+#define C_RP c_double
+#endif
 module mod_sod2d_hip
    use iso_c_binding
    implicit none
    private
 
-   integer(4), parameter :: rp = 8
+   integer(4), parameter :: rp = __RP__
    integer(4), parameter :: ndime = 3
    integer(4), parameter :: porder = __PORDER__
    integer(4), parameter :: nnode = (porder+1)**3
@@ -35,9 +52,11 @@ module mod_sod2d_hip
    public :: full_convec_ijk, full_diffusion_ijk, generic_scalar_convec_ijk
    public :: smart_visc_spectral, lumped_solver_scal_opt, lumped_solver_vect_opt
    public :: rk_stage_update
+#if __RP__ == 8
    public :: smart_visc_spectral_imex, eval_divergence, compute_vorticity, ducros_visc_limit
-   public :: sod2d_hip_set_live_params
    public :: comp_tau, full_proj_ijk, full_diff_stab_ijk
+#endif
+   public :: sod2d_hip_set_live_params
    public :: rk_stage_fused, rk_stage_lps_fused, eos_entropy_step, entropy_convec_residual
    public :: rk4_stage_fused, rk4_stage_update
 
@@ -71,16 +90,16 @@ module mod_sod2d_hip
    real(rp), save :: fact_diffstab = 1.0_rp
 
    interface sod2d_hip_enter_data
-      module procedure enter_r8, enter_i4
+      module procedure enter_r8, enter_i4, enter_r4
    end interface
    interface sod2d_hip_update_device
-      module procedure upd_dev_r8, upd_dev_i4
+      module procedure upd_dev_r8, upd_dev_i4, upd_dev_r4
    end interface
    interface sod2d_hip_update_host
-      module procedure upd_host_r8, upd_host_i4
+      module procedure upd_host_r8, upd_host_i4, upd_host_r4
    end interface
    interface sod2d_hip_exit_data
-      module procedure exit_r8, exit_i4
+      module procedure exit_r8, exit_i4, exit_r4
    end interface
 
    interface
@@ -135,21 +154,21 @@ module mod_sod2d_hip
       end function
       integer(c_int) function c_full_convec(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
             gJ, gK, u, q, rho, pr, E, Rmass, Rmom, Rener, init, fact) &
-            bind(C, name='sodhip_full_convec_ijk_f64')
-         import :: c_int, c_ptr, c_double
+            bind(C, name='sodhip_full_convec_ijk'//RPSFX)
+         import :: c_int, c_ptr, c_float, c_double
          type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, u, q, rho, pr, E
          type(c_ptr), value :: Rmass, Rmom, Rener
          integer(c_int), value :: p, nelem, npoin, init
-         real(c_double), value :: fact
+         real(C_RP), value :: fact
       end function
       integer(c_int) function c_full_diffusion(c, p, nelem, npoin, connec, He, gpvol, dl, inv, &
             gI, gJ, gK, Cp, Pr, rho_n, rho, u, Tem, mu_fluid, mu_e, mu_sgs, c_rho, c_ener, &
-            Rmass, Rmom, Rener, init, fact) bind(C, name='sodhip_full_diffusion_ijk_f64')
-         import :: c_int, c_ptr, c_double
+            Rmass, Rmom, Rener, init, fact) bind(C, name='sodhip_full_diffusion_ijk'//RPSFX)
+         import :: c_int, c_ptr, c_float, c_double
          type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, rho_n, rho, u, Tem
          type(c_ptr), value :: mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener
          integer(c_int), value :: p, nelem, npoin, init
-         real(c_double), value :: Cp, Pr, c_rho, c_ener, fact
+         real(C_RP), value :: Cp, Pr, c_rho, c_ener, fact
       end function
       integer(c_int) function c_comp_tau(c, p, nelem, npoin, connec, csound, u, helem_k, &
             c_species, drop_c, tau) bind(C, name='sodhip_comp_tau_f64')
@@ -179,41 +198,41 @@ module mod_sod2d_hip
       end function
       integer(c_int) function c_scalar_convec(c, p, nelem, npoin, connec, He, gpvol, dl, inv, gI, &
             gJ, gK, q, eta, u, Rconvec, init, fact) &
-            bind(C, name='sodhip_generic_scalar_convec_ijk_f64')
-         import :: c_int, c_ptr, c_double
+            bind(C, name='sodhip_generic_scalar_convec_ijk'//RPSFX)
+         import :: c_int, c_ptr, c_float, c_double
          type(c_ptr), value :: c, connec, He, gpvol, dl, inv, gI, gJ, gK, q, eta, u, Rconvec
          integer(c_int), value :: p, nelem, npoin, init
-         real(c_double), value :: fact
+         real(C_RP), value :: fact
       end function
       integer(c_int) function c_smart_visc(c, p, nelem, npoin, npoin_w, connec, lpoin_w, Reta, &
             gamma_gas, rho, u, Tem, eta, helem, helem_k, inv, prm, mu_e, mue_l) &
-            bind(C, name='sodhip_smart_visc_spectral_f64')
-         import :: c_int, c_ptr, c_double, visc_params_t
+            bind(C, name='sodhip_smart_visc_spectral'//RPSFX)
+         import :: c_int, c_ptr, c_float, c_double, visc_params_t
          type(c_ptr), value :: c, connec, lpoin_w, Reta, rho, u, Tem, eta, helem, helem_k, inv
          type(c_ptr), value :: mu_e, mue_l
          integer(c_int), value :: p, nelem, npoin, npoin_w
-         real(c_double), value :: gamma_gas
+         real(C_RP), value :: gamma_gas
          type(visc_params_t), intent(in) :: prm
       end function
       integer(c_int) function c_lumped_scal(c, npoin, npoin_w, lpoin_w, invMl, R) &
-            bind(C, name='sodhip_lumped_solver_scal_f64')
+            bind(C, name='sodhip_lumped_solver_scal'//RPSFX)
          import :: c_int, c_ptr
          type(c_ptr), value :: c, lpoin_w, invMl, R
          integer(c_int), value :: npoin, npoin_w
       end function
       integer(c_int) function c_lumped_vect(c, npoin, npoin_w, lpoin_w, invMl, R) &
-            bind(C, name='sodhip_lumped_solver_vect_f64')
+            bind(C, name='sodhip_lumped_solver_vect'//RPSFX)
          import :: c_int, c_ptr
          type(c_ptr), value :: c, lpoin_w, invMl, R
          integer(c_int), value :: npoin, npoin_w
       end function
       integer(c_int) function c_rk_update(c, npoin, npoin_w, lpoin_w, invMl, dt, lam1, lams, gam, &
             rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener) &
-            bind(C, name='sodhip_rk_stage_update_f64')
-         import :: c_int, c_ptr, c_double
+            bind(C, name='sodhip_rk_stage_update'//RPSFX)
+         import :: c_int, c_ptr, c_float, c_double
          type(c_ptr), value :: c, lpoin_w, invMl, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener
          integer(c_int), value :: npoin, npoin_w
-         real(c_double), value :: dt, lam1, lams, gam
+         real(C_RP), value :: dt, lam1, lams, gam
       end function
       integer(c_int) function c_visc_imex(c, p, nelem, npoin, npoin_w, connec, lpoin_w, Reta, &
             rho, u, csound, eta, helem_k, prm, ce_comp, high_mach, mu_e, mue_l) &
@@ -367,6 +386,18 @@ contains
                  'sodhip_enter_data')
    end subroutine enter_r8
 
+   subroutine enter_r4(a, copyin)
+      real(c_float), dimension(..), intent(in), target :: a
+      logical, intent(in), optional :: copyin
+      integer(c_int) :: ci
+      ci = 1
+      if (present(copyin)) then
+         if (.not. copyin) ci = 0
+      end if
+      call check(c_enter_data(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t, ci), &
+                 'sodhip_enter_data')
+   end subroutine enter_r4
+
    subroutine enter_i4(a, copyin)
       integer(c_int), dimension(..), intent(in), target :: a
       logical, intent(in), optional :: copyin
@@ -384,6 +415,12 @@ contains
       call check(c_update_device(ctx, c_loc(a), size(a, kind=c_size_t) * 8_c_size_t), &
                  'sodhip_update_device')
    end subroutine upd_dev_r8
+
+   subroutine upd_dev_r4(a)
+      real(c_float), dimension(..), intent(in), target :: a
+      call check(c_update_device(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t), &
+                 'sodhip_update_device')
+   end subroutine upd_dev_r4
 
    subroutine upd_dev_i4(a)
       integer(c_int), dimension(..), intent(in), target :: a
@@ -410,6 +447,12 @@ contains
                  'sodhip_copy_present')
    end subroutine sod2d_hip_copy
 
+   subroutine upd_host_r4(a)
+      real(c_float), dimension(..), intent(inout), target :: a
+      call check(c_update_host(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t), &
+                 'sodhip_update_host')
+   end subroutine upd_host_r4
+
    subroutine upd_host_i4(a)
       integer(c_int), dimension(..), intent(inout), target :: a
       call check(c_update_host(ctx, c_loc(a), size(a, kind=c_size_t) * 4_c_size_t), &
@@ -426,6 +469,17 @@ contains
       end if
       call check(c_exit_data(ctx, c_loc(a), co), 'sodhip_exit_data')
    end subroutine exit_r8
+
+   subroutine exit_r4(a, copyout)
+      real(c_float), dimension(..), intent(inout), target :: a
+      logical, intent(in), optional :: copyout
+      integer(c_int) :: co
+      co = 0
+      if (present(copyout)) then
+         if (copyout) co = 1
+      end if
+      call check(c_exit_data(ctx, c_loc(a), co), 'sodhip_exit_data')
+   end subroutine exit_r4
 
    subroutine exit_i4(a, copyout)
       integer(c_int), dimension(..), intent(inout), target :: a
@@ -546,6 +600,7 @@ contains
          fact_diffu), 'full_diffusion_ijk')
    end subroutine full_diffusion_ijk
 
+#if __RP__ == 8
    ! elem_stab.f90:589-616.  dt is part of the reference's interface and is not
    ! read; c_species_stab and flag_drop_c_in_envit are module globals.
    subroutine comp_tau(nelem,npoin,connec,csound,u,helem_k,dt,tau)
@@ -614,6 +669,7 @@ contains
          real(c_lps_comp, rp), dev(Rmass,'Rmass'), dev(Rmom,'Rmom'), dev(Rener,'Rener'), &
          init_flag(initialze), fact_diffstab), 'full_diff_stab_ijk')
    end subroutine full_diff_stab_ijk
+#endif
 
    ! elem_convec.f90:472-594
    subroutine generic_scalar_convec_ijk(nelem,npoin,connec,Ngp,dNgp,He,gpvol,dlxigp_ip,xgp, &
@@ -665,6 +721,7 @@ contains
          'smart_visc_spectral')
    end subroutine smart_visc_spectral
 
+#if __RP__ == 8
    ! mod_entropy_viscosity.f90:113-227, the live path's sensor.  Ngp, coord,
    ! dNgp, gpvol, wgp, Tem, helem, Ml, invAtoIJK and gmshAtoI/J/K are part of
    ! the reference's interface and are not read.
@@ -749,6 +806,7 @@ contains
          dev(divU,'divU'), dev(vorti,'vorti'), dev(Ml,'Ml'), dev(helem,'helem'), &
          ducros_min_val, dev(mu_e,'mu_e')), 'ducros_visc_limit')
    end subroutine ducros_visc_limit
+#endif
 
    ! mod_solver.f90:40-59
    subroutine lumped_solver_scal_opt(npoin,npoin_w,lpoin_w,invMl,R)
@@ -805,7 +863,8 @@ contains
       real(rp),   intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
       integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
       integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
-      real(rp),   intent(in)    :: mu_e(nelem,ngaus), invMl(npoin)
+      real(rp),   intent(in)    :: mu_e(nelem,ngaus)
+      real(8),    intent(in)    :: invMl(npoin)
       real(rp),   intent(in)    :: gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam
       real(8),    intent(in)    :: rho1(npoin), q1(npoin,ndime), E1(npoin)
       real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)
@@ -941,7 +1000,8 @@ contains
       real(rp),   intent(in)    :: dlxigp_ip(ngaus,ndime,porder+1)
       integer(4), intent(in)    :: invAtoIJK(porder+1,porder+1,porder+1), gmshAtoI(nnode)
       integer(4), intent(in)    :: gmshAtoJ(nnode), gmshAtoK(nnode)
-      real(rp),   intent(in)    :: mu_e(nelem,ngaus), invMl(npoin)
+      real(rp),   intent(in)    :: mu_e(nelem,ngaus)
+      real(8),    intent(in)    :: invMl(npoin)
       real(rp),   intent(in)    :: gamma_gas, Rgas, Cp, Prt, mu_factor, dt, a_next, b
       logical,    intent(in)    :: first, last
       real(8),    intent(inout) :: rho(npoin), q(npoin,ndime), E(npoin)

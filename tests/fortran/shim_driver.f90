@@ -1,28 +1,34 @@
 ! shim_driver.f90 -- test program for the Fortran layer mod_sod2d_hip: reads a
-! case written by tests/test_gpu_fortran_shim.py (raw little-endian arrays in
-! the reference's layouts), maps the arrays to the device, calls the operators
-! through their reference interfaces and writes the results back as raw
-! arrays.  Usage: shim_driver <directory>
+! case written by tests/test_gpu_fortran_shim.py (raw little-endian FP64 arrays
+! in the reference's layouts), maps the arrays to the device, calls the
+! operators through their reference interfaces and writes the results back
+! as raw FP64 arrays.  The arrays are real(__RP__), the module's rp: with
+! -D__RP__=4 the real(4) interfaces are exercised (the live-path and LPS
+! operators exist in the real(8) build only).  Usage: shim_driver <directory>
+#ifndef __RP__
+#define __RP__ 8
+#endif
 program shim_driver
    use iso_c_binding
    use mod_sod2d_hip
    implicit none
    integer, parameter :: porder = __PORDER__, nnode = (porder+1)**3, ngaus = nnode
+   integer, parameter :: rk = __RP__
    character(len=512) :: dir
    integer(4) :: nelem, npoin, npoin_w, u_meta
    integer(4), allocatable :: connec(:,:), lpoin_w(:), inv(:,:,:), gI(:), gJ(:), gK(:)
-   real(8), allocatable :: He(:,:,:,:), gpvol(:,:,:), dl(:,:,:), Ngp(:,:), dNgp(:,:,:), xgp(:,:)
-   real(8), allocatable :: wgp(:), coord(:,:)
-   real(8), allocatable :: u(:,:), q(:,:), rho(:), pr(:), h(:), Tem(:), mu_fluid(:), mu_e(:,:)
-   real(8), allocatable :: mu_sgs(:,:), Ml(:), invMl(:), Reta(:), eta(:), helem_l(:,:), helem(:)
-   real(8), allocatable :: csound(:), Rmass(:), Rmom(:,:), Rener(:), Rconv(:), mue_out(:,:)
-   real(8), allocatable :: mue_l(:,:), rho1(:), q1(:,:), E1(:), rhoS(:), qS(:,:), ES(:)
-   real(8), allocatable :: mu_imex(:,:), mue_l2(:,:), divU(:), curlU(:,:)
+   real(rk), allocatable :: He(:,:,:,:), gpvol(:,:,:), dl(:,:,:), Ngp(:,:), dNgp(:,:,:), xgp(:,:)
+   real(rk), allocatable :: wgp(:), coord(:,:)
+   real(rk), allocatable :: u(:,:), q(:,:), rho(:), pr(:), h(:), Tem(:), mu_fluid(:), mu_e(:,:)
+   real(rk), allocatable :: mu_sgs(:,:), Ml(:), invMl(:), Reta(:), eta(:), helem_l(:,:), helem(:)
+   real(rk), allocatable :: csound(:), Rmass(:), Rmom(:,:), Rener(:), Rconv(:), mue_out(:,:)
+   real(rk), allocatable :: mue_l(:,:), rho1(:), q1(:,:), E1(:), rhoS(:), qS(:,:), ES(:)
+   real(rk), allocatable :: mu_imex(:,:), mue_l2(:,:), divU(:), curlU(:,:)
    integer(4), allocatable :: lelpn(:), atoIJK(:)
-   real(8), allocatable :: tau(:), PMass(:,:), PEner(:,:), PMX(:,:), PMY(:,:), PMZ(:,:)
-   real(8), allocatable :: Rsm(:), Rsq(:,:), Rse(:)
-   real(8) :: leviCivi(3,3,3)
-   real(8) :: Cp, Prt, gamma_gas, ce, cmax, cglob, Rgas, dt, lam1, lams, gam
+   real(rk), allocatable :: tau(:), PMass(:,:), PEner(:,:), PMX(:,:), PMY(:,:), PMZ(:,:)
+   real(rk), allocatable :: Rsm(:), Rsq(:,:), Rse(:)
+   real(rk) :: leviCivi(3,3,3)
+   real(rk) :: Cp, Prt, gamma_gas, ce, cmax, cglob, Rgas, dt, lam1, lams, gam
    integer(4) :: flag_norm
 
    call get_command_argument(1, dir)
@@ -46,9 +52,9 @@ program shim_driver
    allocate(lelpn(npoin), atoIJK(nnode))
    allocate(tau(nelem), PMass(npoin,3), PEner(npoin,3), PMX(npoin,3), PMY(npoin,3), PMZ(npoin,3))
    allocate(Rsm(npoin), Rsq(npoin,3), Rse(npoin))
-   leviCivi = 0.0d0; atoIJK = 0; mu_imex = 0.0d0; mue_l2 = 0.0d0
-   Ngp = 0.0d0; dNgp = 0.0d0; xgp = 0.0d0; wgp = 0.0d0; coord = 0.0d0; Ml = 0.0d0
-   csound = 0.0d0; mu_sgs = 0.0d0; mue_out = 0.0d0; mue_l = 0.0d0
+   leviCivi = 0.0_rk; atoIJK = 0; mu_imex = 0.0_rk; mue_l2 = 0.0_rk
+   Ngp = 0.0_rk; dNgp = 0.0_rk; xgp = 0.0_rk; wgp = 0.0_rk; coord = 0.0_rk; Ml = 0.0_rk
+   csound = 0.0_rk; mu_sgs = 0.0_rk; mue_out = 0.0_rk; mue_l = 0.0_rk
 
    call rd_i(connec, 'connec'); call rd_i(lpoin_w, 'lpoin_w'); call rd_i(inv, 'invAtoIJK')
    call rd_i(gI, 'gI'); call rd_i(gJ, 'gJ'); call rd_i(gK, 'gK')
@@ -91,9 +97,9 @@ program shim_driver
    ! the RHS of updateF: convection (initialising) then diffusion (accumulating),
    ! both with fact = -1 (time_integ_ls.f90:605-640)
    call full_convec_ijk(nelem,npoin,connec,Ngp,dNgp,He,gpvol,dl,xgp,inv,gI,gJ,gK,u,q,rho,pr,h, &
-                        Rmass,Rmom,Rener,.true.,-1.0d0)
+                        Rmass,Rmom,Rener,.true.,-1.0_rk)
    call full_diffusion_ijk(nelem,npoin,connec,Ngp,He,gpvol,dl,inv,gI,gJ,gK,Cp,Prt,rho,rho,u,Tem, &
-                           mu_fluid,mu_e,mu_sgs,Ml,Rmass,Rmom,Rener,.false.,-1.0d0)
+                           mu_fluid,mu_e,mu_sgs,Ml,Rmass,Rmom,Rener,.false.,-1.0_rk)
    call sod2d_hip_update_host(Rmass); call sod2d_hip_update_host(Rmom)
    call sod2d_hip_update_host(Rener)
    call wr(Rmass, 'out_Rmass'); call wr(Rmom, 'out_Rmom'); call wr(Rener, 'out_Rener')
@@ -120,6 +126,7 @@ program shim_driver
    call sod2d_hip_update_host(mue_out)
    call wr(mue_out, 'out_mu_e')
 
+#if __RP__ == 8
    ! the live path's sensor and limiter (time_integ_ls.f90:455-463)
    call smart_visc_spectral_imex(nelem,npoin,npoin_w,connec,lpoin_w,Reta,Ngp,coord,dNgp,gpvol, &
                                  wgp,gamma_gas,rho,u,csound,Tem,eta,helem_l,helem,Ml,mu_imex,inv, &
@@ -142,12 +149,13 @@ program shim_driver
                       rho,u,Tem,Ml,invMl,PMass,PEner,PMX,PMY,PMZ)
    call full_diff_stab_ijk(nelem,npoin,connec,Ngp,He,gpvol,dl,inv,gI,gJ,gK,Cp,Prt,rho,rho,u,Tem, &
                            mu_fluid,mu_e,mu_sgs,Ml,PMass,PEner,PMX,PMY,PMZ,tau,Rsm,Rsq,Rse, &
-                           .true.,-1.0d0)
+                           .true.,-1.0_rk)
    call sod2d_hip_update_host(tau); call sod2d_hip_update_host(PMass)
    call sod2d_hip_update_host(PMX); call sod2d_hip_update_host(Rsm)
    call sod2d_hip_update_host(Rsq); call sod2d_hip_update_host(Rse)
    call wr(tau, 'out_tau'); call wr(PMass, 'out_PMass'); call wr(PMX, 'out_PMX')
    call wr(Rsm, 'out_Rsm'); call wr(Rsq, 'out_Rsq'); call wr(Rse, 'out_Rse')
+#endif
 
    call sod2d_hip_synchronize()
    call sod2d_hip_finalize()
@@ -156,14 +164,17 @@ program shim_driver
 contains
 
    subroutine rd_r(a, name)
-      real(8), intent(out) :: a(*)
+      real(rk), intent(out) :: a(*)
       character(*), intent(in) :: name
       integer :: un, n
+      real(8), allocatable :: t(:)
       inquire(file=trim(dir)//'/'//name//'.bin', size=n)
+      allocate(t(n/8))
       open(newunit=un, file=trim(dir)//'/'//name//'.bin', access='stream', form='unformatted', &
            status='old', action='read')
-      read(un) a(1:n/8)
+      read(un) t
       close(un)
+      a(1:n/8) = real(t, rk)
    end subroutine rd_r
 
    subroutine rd_i(a, name)
@@ -178,16 +189,16 @@ contains
    end subroutine rd_i
 
    subroutine wr(a, name)
-      real(8), intent(in) :: a(..)
+      real(rk), intent(in) :: a(..)
       character(*), intent(in) :: name
       integer :: un
       open(newunit=un, file=trim(dir)//'/'//name//'.bin', access='stream', form='unformatted', &
            status='replace', action='write')
       select rank (a)
       rank (1)
-         write(un) a
+         write(un) real(a, 8)
       rank (2)
-         write(un) a
+         write(un) real(a, 8)
       end select
       close(un)
    end subroutine wr

@@ -6,7 +6,9 @@
   with per-field bounds.
 * 64^3 elements, p=3 (7,077,888 unique nodes), the first two steps (the first
   one carries the extra start-up RHS and sensor call), FP64 device against the
-  FP64 oracle at full size.
+  FP64 oracle at full size; and 20 steps of the FP32-operator mode against the
+  FP64 device run (pinned to the oracle by the test before it) at full size,
+  with per-field bounds.
 
 Bars: conserved fields within 1e-12 normwise relative per step (after k steps
 the bound is k * 1e-12), dt and the diagnostics within 1e-10, mu_e within
@@ -103,3 +105,35 @@ def test_tgv64_first_two_steps_fp64(oracle):
             assert e[k] < step * 1e-12, (step, k, e[k])
         assert e["mu_e"] < 1e-10, (step, e["mu_e"])
     _diag_close(dev.diagnostics(), orc.diagnostics(), 1e-10)
+
+
+def test_tgv64_fp32_operators_vs_fp64():
+    """Full size, FP32 operator arithmetic (FP64 state, gather, mass inverse,
+    combine and reductions) against the FP64 path over 20 steps, per field
+    every 5 steps.  Bounds: rho 2e-7, rho*u 1e-5, E 2e-7, mu_e 0.3; after 20
+    steps EK 1e-9 and the dissipation rate eps_T 5e-5.  The sensor output is a
+    limiter of the entropy residual, a small difference of nearly equal
+    entropies, which amplifies the state's rounding; on the finer mesh the
+    residual is smaller and the amplification larger than at 16^3.  Measured
+    over the 20 steps: rho <= 5.0e-8, rho*u <= 4.6e-7, E <= 3.5e-8, mu_e
+    5-6e-2; EK 7e-12, eps_T 5.4e-6."""
+    from sod2d_amd.solver import TGVParams, TGVRun
+    g = make_case(3, 64, periodic=True, lengths=(2 * np.pi,) * 3)
+    runs = {k: TGVRun(3, geometry=g, params=TGVParams(), op_precision=k, dt_mode="fixed")
+            for k in (8, 4)}
+    assert runs[4].dt == runs[8].dt
+    w = g.mesh.lpoin_w - 1
+    for step in range(1, 21):
+        for r in runs.values():
+            r.step()
+        if step % 5 == 0:
+            a, b = runs[4].state(), runs[8].state()
+            e = {k: relerr(a[k][w], b[k][w]) for k in ("rho", "q", "E")}
+            e["mu_e"] = relerr(a["mu_e"], b["mu_e"])
+            print(f"tgv64 fp32 operators vs fp64, step {step}:", e, flush=True)
+            assert e["rho"] < 2e-7 and e["q"] < 1e-5 and e["E"] < 2e-7, (step, e)
+            assert e["mu_e"] < 0.3, (step, e)
+    da, db = runs[4].diagnostics(), runs[8].diagnostics()
+    print("tgv64 diagnostics fp32 operators / fp64:", da, db, flush=True)
+    for k, tol in (("EK", 1e-9), ("eps_T", 5e-5)):
+        assert abs(da[k] - db[k]) <= tol * abs(db[k]), (k, da[k], db[k])
