@@ -18,6 +18,9 @@ The JSON line also carries:
                   with HIP events on the launch stream over up to 3 steps run
                   right after the timed ones (the timed steps carry no events);
                   HBM traffic per stage from the committed rocprofv3 PMC summary
+  ensemble     -- 32 uneven TGV instances (n^3 elements, n ~ U{8..32}, Ma and Re
+                  varied) joined into one mesh: GDOF/s and ms per step of HIP-graph
+                  replayed steps, the autotuner's launch shape, the feature dump.
   surrogate    -- the learned entropy-viscosity sensor (per-element MLP, BF16 on
                   the matrix cores, plain sensor as fallback and judge every 10
                   steps) against the plain sensor on two fresh runs of the same
@@ -279,6 +282,71 @@ def run_surrogate(a, steps: int = 20, window: int = 40) -> dict:
     return out
 
 
+def run_ensemble(a, count: int = 32, steps: int = 10) -> dict:
+    """The batched ensemble: ``count`` periodic TGV instances, p=3, with n^3
+    elements, n ~ U{8..32}, Ma ~ U(0.1, 0.5), Re ~ U(400, 3200) (seeded),
+    joined into one mesh and advanced at the smallest instance dt; one element
+    launch and one gather per RK stage cover every instance.  After the first
+    step the autotuner picks the stage's launch shape, two steps are captured
+    in a HIP graph, and ``steps`` steps (rounded up to even) are timed as graph
+    replays.  The features and mu_e targets of every element are dumped once.
+    GDOF/s per RK stage as for the single run, over all instances' nodes."""
+    import torch
+    from sod2d_amd import autotune
+    from sod2d_amd.ensemble import EnsembleRun, config4_cases
+
+    cases = config4_cases(count)
+    t0 = time.perf_counter()
+    ens = EnsembleRun(cases)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    ens.step()  # the first step builds the library's tables
+    tune = autotune.tune_stage(ens, reps=3)
+    ens.step()
+    graph = ens.capture()
+    graph.replay()
+    torch.cuda.synchronize()
+    nrep = (steps + 1) // 2
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        graph.replay()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / (2 * nrep)
+    graph.release()
+    t1 = time.perf_counter()
+    dump = ens.dump_features()
+    torch.cuda.synchronize()
+    t_dump = time.perf_counter() - t1
+    N = ens.npoin_w
+    ns = [c.n for c in cases]
+    ch = tune["choice"]
+    return {
+        "value": round(5 * N * ens.rhs_per_step / t_step / 1e9, 4),
+        "unit": "GDOF/s",
+        "ms_per_step": round(t_step * 1e3, 4),
+        "steps": 2 * nrep,
+        "instances": count,
+        "elements": ens.nelem,
+        "unique_nodes": N,
+        "n_range": [min(ns), max(ns)],
+        "mach_range": [round(min(c.M for c in cases), 3), round(max(c.M for c in cases), 3)],
+        "re_range": [round(min(c.Re for c in cases), 1), round(max(c.Re for c in cases), 1)],
+        "dt": ens.dt,
+        "graph": True,
+        "autotune": {"elem_blocks_per_cu": ch["elem_blocks_per_cu"],
+                     "geo_rows16": ch["geo_rows16"], "gather_threads": ch["gather_threads"],
+                     "ms_stage": round(ch["ms_stage"], 5),
+                     "ms_stage_default": round(next(
+                         r["ms_stage"] for r in tune["table"]
+                         if (r["elem_blocks_per_cu"], r["geo_rows16"], r["gather_threads"])
+                         == (16, 1, 256)), 5),
+                     "candidates": len(tune["table"])},
+        "feature_dump": {"elements": int(dump["X"].shape[0]), "features": int(dump["X"].shape[1]),
+                         "s": round(t_dump, 3)},
+        "setup_s": round(t_setup, 2),
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,9 +368,12 @@ def main(argv=None):
     ap.add_argument("--graph", action="store_true",
                     help="time the steps as replays of a HIP graph of two captured steps "
                          "(--steps rounded up to even)")
-    ap.add_argument("--workload", choices=("tgv", "operators"), default="tgv",
+    ap.add_argument("--workload", choices=("tgv", "operators", "ensemble"), default="tgv",
                     help="tgv: the TGV time step (default); operators: the isolated "
-                         "full_convec_ijk / full_diffusion_ijk / smart_visc_spectral calls")
+                         "full_convec_ijk / full_diffusion_ijk / smart_visc_spectral calls; "
+                         "ensemble: 32 uneven TGV instances joined into one mesh")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble summary in the default line")
     ap.add_argument("--op-n", type=int, default=128,
                     help="elements per direction of the operator workload's box")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -318,6 +389,19 @@ def main(argv=None):
         return 2
     if a.workload == "operators":
         print(json.dumps(run_operators(a)), flush=True)
+        return 0
+    if a.workload == "ensemble":
+        sys.path.insert(0, ROOT)
+        e = run_ensemble(a, steps=max(a.steps, 2))
+        out = {"metric": "GDOF/s per RK stage, batched ensemble of uneven TGV instances",
+               "value": e["value"], "unit": "GDOF/s", "n_gpus": 1, "steps": e["steps"],
+               "warmup": 3, "ms_per_step": e["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": "32 periodic TGV instances p=3, n^3 elements n~U{8..32}, "
+                                      "Ma~U(0.1,0.5), Re~U(400,3200), joined into one mesh, "
+                                      "common fixed dt, HIP-graph replayed steps"},
+               "ensemble": e}
+        print(json.dumps(out), flush=True)
         return 0
 
     import torch
@@ -441,6 +525,9 @@ def main(argv=None):
         del run
         torch.cuda.empty_cache()
         out["surrogate"] = run_surrogate(a)
+    if not a.no_ensemble and not a.lps and a.precision == 8:
+        torch.cuda.empty_cache()
+        out["ensemble"] = run_ensemble(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -157,9 +157,44 @@ int sodhip_copy_present(sodhip_ctx ctx, const void *src_host, void *dst_host, si
  * the summed duration (ms) and the number of launches since the last reset.
  * kind: 0 element operator kernel, 1 node gather/update kernel, 2 sensor,
  * 3 other nodal kernels. */
+/* Launch shape of the fused RK stage on this context: blocks per CU of the
+ * element kernel (1..64; with a grid-stride walk this sets the elements each
+ * workgroup takes), whether the geometry moves through LDS in 16-byte lane
+ * rows when He and gpvol are aligned for it (1) or in 4-byte rows (0), and
+ * the gather/update block size (64, 128 or 256).  Defaults 16, 1, 256.  The
+ * results do not depend on the shape.  No counterpart in the reference: the
+ * autotuner (sod2d_amd/autotune.py) sets it. */
+int sodhip_set_stage_config(sodhip_ctx ctx, int elem_blocks_per_cu, int geo_rows16,
+                            int gather_threads);
+int sodhip_get_stage_config(sodhip_ctx ctx, int *elem_blocks_per_cu, int *geo_rows16,
+                            int *gather_threads);
+
 int sodhip_prof_enable(sodhip_ctx ctx, int enable);
 int sodhip_prof_get(sodhip_ctx ctx, int kind, double *total_ms, long long *count);
 int sodhip_prof_reset(sodhip_ctx ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Joined instances                                                          */
+/* ------------------------------------------------------------------------ */
+/* Independent meshes joined into one disconnected mesh (an ensemble of runs:
+ * elements, nodes and working-node lists concatenated with offsets), so that
+ * one launch per kernel covers every instance.  No counterpart in the
+ * reference, which runs one mesh.  The fields that differ between instances
+ * (device arrays):
+ *   wseg(ninst+1)      instance i owns the lpoin_w positions wseg[i]..wseg[i+1]-1
+ *                      (its entropy normalisation runs over them);
+ *   elem_inst(nelem)   the instance of each element;
+ *   p_inf, rho_inf, mu_factor (npoin): the reference state of the entropy and
+ *                      the Sutherland factor of each node's instance (the
+ *                      reference's mu_factor(npoin)).
+ * The fused stage takes the Sutherland factor per element through
+ * sodhip_stage_params.mu_factor_e. */
+typedef struct {
+    int ninst;
+    const int *wseg;
+    const int *elem_inst;
+    const double *p_inf, *rho_inf, *mu_factor;
+} sodhip_instances;
 
 /* ------------------------------------------------------------------------ */
 /* Drop-in operators (device pointers, reference layouts)                    */
@@ -289,6 +324,17 @@ int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int nelem, int np
                                    const double *Tem, const double *eta, const double *helem,
                                    const double *helem_k, const int *invAtoIJK,
                                    const sodhip_visc_params *prm, double *mu_e, double *mue_l);
+/* The same sensor on joined instances: each instance's entropy normalisation
+ * over its own working nodes, with the blocking of a single mesh (the bits of
+ * the instance run alone). */
+int sodhip_smart_visc_spectral_ens_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                       int npoin_w, const int *connec, const int *lpoin_w,
+                                       const sodhip_instances *ens, const double *Reta,
+                                       double gamma_gas, const double *rho, const double *u,
+                                       const double *Tem, const double *eta,
+                                       const double *helem, const double *helem_k,
+                                       const int *invAtoIJK, const sodhip_visc_params *prm,
+                                       double *mu_e, double *mue_l);
 /* The same sensor on real(4) arrays (USE_RP=4): FP32 inputs and outputs, FP64
  * arithmetic and entropy normalisation. */
 int sodhip_smart_visc_spectral_f32(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
@@ -444,6 +490,13 @@ int sodhip_eos_entropy_step_f64(sodhip_ctx ctx, int npoin, int npoin_w, const in
                                 const double *E, double *u, double *e_int, double *pr,
                                 double *csound, double *machno, double *Tem, double *eta,
                                 double *mu_fluid);
+/* sodhip_eos_entropy_step_f64 on joined instances: p_inf, rho_inf and
+ * mu_factor per node from ens. */
+int sodhip_eos_entropy_step_ens_f64(sodhip_ctx ctx, int npoin, int npoin_w, const int *lpoin_w,
+                                    double gamma_gas, double Rgas, const sodhip_instances *ens,
+                                    const double *rho, const double *q, const double *E,
+                                    double *u, double *e_int, double *pr, double *csound,
+                                    double *machno, double *Tem, double *eta, double *mu_fluid);
 /* The entropy residual of rk_4_ls_main (time_integ_ls.f90:429-443) in one
  * call: sodhip_generic_scalar_convec_ijk_f64 with q = NULL (f = u * eta(:,1)),
  * initialize = 1 and fact = 1 into Reta(:,2), sodhip_lumped_solver_scal_f64 on
@@ -500,6 +553,15 @@ int sodhip_sensor_features_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
                                const double *Tem, const double *eta, const double *helem,
                                const double *helem_k, const int *invAtoIJK,
                                const sodhip_visc_params *prm, float *X, double *beta);
+/* The same features on joined instances (each instance's entropy
+ * normalisation), for dumping training data from an ensemble run. */
+int sodhip_sensor_features_ens_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                   int npoin_w, const int *connec, const int *lpoin_w,
+                                   const sodhip_instances *ens, const double *Reta,
+                                   double gamma_gas, const double *rho, const double *u,
+                                   const double *Tem, const double *eta, const double *helem,
+                                   const double *helem_k, const int *invAtoIJK,
+                                   const sodhip_visc_params *prm, float *X, double *beta);
 /* y(n) = MLP(X(n,16)) on the matrix cores (BF16 operands, FP32 accumulation) */
 int sodhip_mlp_forward(sodhip_ctx ctx, const sodhip_mlp *model, int n, const float *X, float *y);
 /* Gradient of L = (1/n) sum (y - t)^2 over the batch: forward (y, and the
@@ -545,6 +607,10 @@ typedef struct {
     double fact;                    /* residual sign: updateF passes -1 */
     double p_ref;                   /* uniform pressure removed before the operator
                                        arithmetic (only grad p enters); 0 = none */
+    const double *mu_factor_e;      /* NULL, or a Sutherland factor per element (device,
+                                       nelem): the reference's mu_factor(npoin) when it is
+                                       constant over each element, as on a mesh joining
+                                       instances of different Reynolds numbers */
 } sodhip_stage_params;
 
 /* One stage of rk_4_ls_main (time_integ_ls.f90:339-361) with updateF

@@ -37,7 +37,14 @@ class MLP(C.Structure):
 class StageParams(C.Structure):
     """sodhip_stage_params: physical constants of the fused RK stage."""
     _fields_ = [("gamma_gas", D), ("Rgas", D), ("Cp", D), ("Pr", D), ("mu_factor", D),
-                ("c_rho", D), ("c_ener", D), ("fact", D), ("p_ref", D)]
+                ("c_rho", D), ("c_ener", D), ("fact", D), ("p_ref", D),
+                ("mu_factor_e", C.c_void_p)]
+
+
+class Instances(C.Structure):
+    """sodhip_instances: the per-instance fields of joined meshes (device pointers)."""
+    _fields_ = [("ninst", I), ("wseg", P), ("elem_inst", P), ("p_inf", P), ("rho_inf", P),
+                ("mu_factor", P)]
 
 
 # name -> (restype, argtypes)
@@ -66,6 +73,14 @@ _SIGS = {
     "sodhip_prof_enable": (I, [P, I]),
     "sodhip_prof_get": (I, [P, I, C.POINTER(D), C.POINTER(C.c_longlong)]),
     "sodhip_prof_reset": (I, [P]),
+    "sodhip_copy_present": (I, [P, P, P, SZ]),
+    "sodhip_set_stage_config": (I, [P, I, I, I]),
+    "sodhip_get_stage_config": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+    "sodhip_smart_visc_spectral_ens_f64": (I, [P, I, I, I, I, P, P, C.POINTER(Instances), P, D]
+                                           + [P] * 7 + [C.POINTER(ViscParams), P, P]),
+    "sodhip_eos_entropy_step_ens_f64": (I, [P, I, I, P, D, D, C.POINTER(Instances)] + [P] * 11),
+    "sodhip_sensor_features_ens_f64": (I, [P, I, I, I, I, P, P, C.POINTER(Instances), P, D]
+                                       + [P] * 7 + [C.POINTER(ViscParams), P, P]),
     "sodhip_full_convec_ijk_f64": (I, [P, I, I, I] + [P] * 16 + [I, D]),
     "sodhip_full_convec_ijk_f32": (I, [P, I, I, I] + [P] * 16 + [I, F]),
     "sodhip_full_diffusion_ijk_f64": (I, [P, I, I, I] + [P] * 8 + [D, D] + [P] * 7 + [D, D]

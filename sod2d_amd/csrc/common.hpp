@@ -140,6 +140,11 @@ struct sodhip_ctx_s {
     // more than the smaller gather saves (measured 0.79 -> 1.50 ms element,
     // 0.37 -> 0.29 ms gather per stage at 64^3)
     bool cluster_stage = false;
+    // launch shape of the fused stage (sodhip_set_stage_config): element-kernel
+    // blocks per CU, 16-byte geometry rows when aligned, gather block size
+    int stage_bpc = 16;
+    int stage_geo16 = 1;
+    int gather_threads = 256;
 };
 
 namespace sodhip {
@@ -198,11 +203,11 @@ int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *c
                       const double *He, const double *gpvol, const double *dl, const int *inv,
                       const int *gI, const int *gJ, const int *gK, const double *mu_e,
                       double gamma, double Rgas, double Cp, double Pr, double mu_factor,
-                      double c_rho, double c_ener, double p_ref, const double *invMl, double fact,
-                      double dt, double lam1, double lams, double gam, const double *rho1,
-                      const double *q1, const double *E1, double *rho, double *q, double *E,
-                      double *Rmass, double *Rmom, double *Rener, const NextStage *next,
-                      double *rec);
+                      const double *mu_factor_e, double c_rho, double c_ener, double p_ref,
+                      const double *invMl, double fact, double dt, double lam1, double lams,
+                      double gam, const double *rho1, const double *q1, const double *E1,
+                      double *rho, double *q, double *E, double *Rmass, double *Rmom,
+                      double *Rener, const NextStage *next, double *rec);
 // gather/update of one stage from element records of rw (5 or 6) values
 // (ops_stage.hip)
 template <typename TS>

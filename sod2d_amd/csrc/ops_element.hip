@@ -98,6 +98,9 @@ template <typename T> struct RhsArgs {
     // conserved inputs (fused stage): FP64 state
     const double *cr, *cq, *cE;
     double gamma, Rgas, mu_factor, p_ref;
+    // per-element Sutherland factor (an ensemble of meshes whose instances
+    // differ in it), or nullptr for the uniform mu_factor
+    const double *mu_factor_e;
     const int *perm; // element-local scratch position of each node
     T *out;          // records of 5 (mass, mom1, mom2, mom3, energy) per element-node;
                      // (e,a) at record e*NN + perm[a]
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(EG<P>::NT, 2) void k_rhs(RhsArgs<T> A)
                 rhoH = T(r * (hd + 0.5 * umag));
                 Tk = T(Td);
                 rhon = rho;
-                muf = T(sutherland<double>(A.mu_factor, Td));
+                muf = T(sutherland<double>(A.mu_factor_e ? A.mu_factor_e[e] : A.mu_factor, Td));
             } else {
                 rho = cur.rho;
 #pragma unroll
@@ -1536,7 +1539,8 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
 #pragma unroll
                 for (int d = 0; d < 3; ++d) u[d] = T(ud[d]);
                 Tk = T(Td);
-                muf = T(sutherland<double>(A.mu_factor, Td));
+                const double mfe = A.mu_factor_e ? A.mu_factor_e[elem_of(t)] : A.mu_factor;
+                muf = T(sutherland<double>(mfe, Td));
                 if (act) s1[(own + 10) >> 1] = V2{Tk, T(0)};
             }
             lds_barrier(); // B2
@@ -1895,7 +1899,8 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
                        const int *gK, const double *rho, const double *q, const double *E,
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
-                       double mu_factor, double c_rho, double c_ener, double p_ref, T *out)
+                       double mu_factor, const double *mu_factor_e, double c_rho, double c_ener,
+                       double p_ref, T *out)
 {
     RhsArgs<T> A{};
     A.nelem = m->nelem;
@@ -1917,6 +1922,7 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     A.gamma = gamma;
     A.Rgas = Rgas;
     A.mu_factor = mu_factor;
+    A.mu_factor_e = mu_factor_e;
     A.p_ref = p_ref;
     A.Cp = T(Cp);
     A.Pr = T(Pr);
@@ -1933,9 +1939,10 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     {
         if (porder == 3) {
             const size_t nn = (size_t)nnode_of(porder);
-            A.geo16 = ((uintptr_t)A.He % 16 == 0) && ((uintptr_t)A.gpvol % 16 == 0) &&
-                      (9 * nn * sizeof(T)) % 16 == 0 && (nn * sizeof(T)) % 16 == 0;
-            const int grid = elem_grid(ctx, m->nelem, 16);
+            A.geo16 = ctx->stage_geo16 && ((uintptr_t)A.He % 16 == 0) &&
+                      ((uintptr_t)A.gpvol % 16 == 0) && (9 * nn * sizeof(T)) % 16 == 0 &&
+                      (nn * sizeof(T)) % 16 == 0;
+            const int grid = elem_grid(ctx, m->nelem, ctx->stage_bpc);
             if (A.geo16)
                 k_rhs_split<3, T, true><<<grid, 128, 0, ctx->stream>>>(A);
             else
@@ -1955,11 +1962,11 @@ int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *c
                       const double *He, const double *gpvol, const double *dl, const int *inv,
                       const int *gI, const int *gJ, const int *gK, const double *mu_e,
                       double gamma, double Rgas, double Cp, double Pr, double mu_factor,
-                      double c_rho, double c_ener, double p_ref, const double *invMl, double fact,
-                      double dt, double lam1, double lams, double gam, const double *rho1,
-                      const double *q1, const double *E1, double *rho, double *q, double *E,
-                      double *Rmass, double *Rmom, double *Rener, const NextStage *next,
-                      double *rec)
+                      const double *mu_factor_e, double c_rho, double c_ener, double p_ref,
+                      const double *invMl, double fact, double dt, double lam1, double lams,
+                      double gam, const double *rho1, const double *q1, const double *E1,
+                      double *rho, double *q, double *E, double *Rmass, double *Rmom,
+                      double *Rener, const NextStage *next, double *rec)
 {
     using T = double;
     RhsArgs<T> A{};
@@ -1982,6 +1989,7 @@ int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *c
     A.gamma = gamma;
     A.Rgas = Rgas;
     A.mu_factor = mu_factor;
+    A.mu_factor_e = mu_factor_e;
     A.p_ref = p_ref;
     A.Cp = Cp;
     A.Pr = Pr;
@@ -2038,12 +2046,14 @@ template int rhs_fused_elements<double>(sodhip_ctx, int, const MeshCache *, cons
                                         const double *, const double *, const int *, const int *,
                                         const int *, const int *, const double *, const double *,
                                         const double *, const double *, double, double, double,
-                                        double, double, double, double, double, double *);
+                                        double, double, const double *, double, double, double,
+                                        double *);
 template int rhs_fused_elements<float>(sodhip_ctx, int, const MeshCache *, const float *,
                                        const float *, const float *, const int *, const int *,
                                        const int *, const int *, const double *, const double *,
                                        const double *, const float *, double, double, double,
-                                       double, double, double, double, double, float *);
+                                       double, double, const double *, double, double, double,
+                                       float *);
 
 template <typename T>
 static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const int *connec,

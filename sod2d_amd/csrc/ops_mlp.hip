@@ -110,6 +110,9 @@ struct FeatArgs {
     double gamma, Rgas, ce, cmax, fact_low_mach;
     const double *pmax;
     int nparts;
+    // joined instances: the normalisation of element e is norm_i[elem_inst[e]]
+    const double *norm_i;
+    const int *elem_inst;
 };
 
 // the scale of the viscosity ratios and of the MLP's output: a ratio x in
@@ -240,10 +243,10 @@ __global__ __launch_bounds__(256) void k_sensor_features(FeatArgs A, float *__re
     for (int q = 0; q < 8; ++q) sn[q] = subset_node(A.inv, h, q);
     const int ntile = (A.nelem + 31) / 32;
     for (int tile = blockIdx.x * 4 + wv; tile < ntile; tile += gridDim.x * 4) {
-        const int e = tile * 32 + (l >> 1);
+        const int e = tile * 32 + (l >> 1), ec = min(e, A.nelem - 1);
         float f[kNF];
         double be;
-        pair_features(A, min(e, A.nelem - 1), h, sn, norm, f, be);
+        pair_features(A, ec, h, sn, A.norm_i ? A.norm_i[A.elem_inst[ec]] : norm, f, be);
         if (e < A.nelem) { // lane h writes features 8h .. 8h+7
             float4 *o = (float4 *)(X + (size_t)e * kNFP + 8 * h);
             const float g[8] = {h ? f[8] : f[0], h ? f[9] : f[1], h ? f[10] : f[2],
@@ -742,13 +745,12 @@ static int check_sensor_args(sodhip_ctx ctx, int porder, int nelem, int npoin, i
     return SODHIP_OK;
 }
 
-extern "C" int sodhip_sensor_features_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
-                                          int npoin_w, const int *connec, const int *lpoin_w,
-                                          const double *Reta, double gamma_gas, const double *rho,
-                                          const double *u, const double *Tem, const double *eta,
-                                          const double *helem, const double *helem_k,
-                                          const int *invAtoIJK, const sodhip_visc_params *prm,
-                                          float *X, double *beta)
+static int sensor_features(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w,
+                           const int *connec, const int *lpoin_w, const double *Reta,
+                           double gamma_gas, const double *rho, const double *u, const double *Tem,
+                           const double *eta, const double *helem, const double *helem_k,
+                           const int *invAtoIJK, const sodhip_visc_params *prm, float *X,
+                           double *beta, const sodhip_instances *ens)
 {
     int rc = check_sensor_args(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, rho, u,
                                Tem, eta, helem, helem_k, invAtoIJK, prm);
@@ -757,18 +759,48 @@ extern "C" int sodhip_sensor_features_f64(sodhip_ctx ctx, int porder, int nelem,
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     MeshCache *m = nullptr;
     if ((rc = get_mesh(ctx, porder, nelem, npoin, connec, &m))) return rc;
-    const double *pmax = nullptr;
+    const double *pmax = nullptr, *norm_i = nullptr;
     int nparts = 0;
+    const bool normalise = prm->flag_normalise_entropy == 1;
     int tok = prof_begin(ctx, 2);
-    if ((rc = entropy_norm_partials(ctx, npoin_w, lpoin_w, eta, prm->flag_normalise_entropy == 1,
-                                    &pmax, &nparts)))
+    if (ens && normalise) {
+        if ((rc = entropy_norm_instances(ctx, ens, lpoin_w, eta, &norm_i))) return rc;
+    } else if ((rc = entropy_norm_partials(ctx, npoin_w, lpoin_w, eta, normalise, &pmax,
+                                           &nparts))) {
         return rc;
-    const FeatArgs A = feat_args(m, nelem, npoin, Reta, gamma_gas, rho, u, Tem, helem, helem_k,
-                                 invAtoIJK, prm, pmax, nparts);
+    }
+    FeatArgs A = feat_args(m, nelem, npoin, Reta, gamma_gas, rho, u, Tem, helem, helem_k,
+                           invAtoIJK, prm, pmax, nparts);
+    A.norm_i = norm_i;
+    A.elem_inst = norm_i ? ens->elem_inst : nullptr;
     k_sensor_features<<<tile_grid(ctx, nelem), 256, 0, ctx->stream>>>(A, X, beta);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
+}
+
+extern "C" int sodhip_sensor_features_f64(sodhip_ctx ctx, int porder, int nelem, int npoin,
+                                          int npoin_w, const int *connec, const int *lpoin_w,
+                                          const double *Reta, double gamma_gas, const double *rho,
+                                          const double *u, const double *Tem, const double *eta,
+                                          const double *helem, const double *helem_k,
+                                          const int *invAtoIJK, const sodhip_visc_params *prm,
+                                          float *X, double *beta)
+{
+    return sensor_features(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, gamma_gas,
+                           rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm, X, beta, nullptr);
+}
+
+extern "C" int sodhip_sensor_features_ens_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, const sodhip_instances *ens, const double *Reta, double gamma_gas,
+    const double *rho, const double *u, const double *Tem, const double *eta,
+    const double *helem, const double *helem_k, const int *invAtoIJK,
+    const sodhip_visc_params *prm, float *X, double *beta)
+{
+    SODHIP_REQUIRE(ens && ens->ninst > 0 && ens->wseg && ens->elem_inst, "invalid instances");
+    return sensor_features(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, gamma_gas,
+                           rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm, X, beta, ens);
 }
 
 extern "C" int sodhip_mlp_forward(sodhip_ctx ctx, const sodhip_mlp *M, int n, const float *X,

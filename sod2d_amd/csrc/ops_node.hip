@@ -360,17 +360,23 @@ __global__ __launch_bounds__(256) void k_eos_step(int npoin, int npoin_w,
 // thermo_entropy (eta(:,1) <- eta(:,2), eta(:,2) from the new rho and p) and
 // sutherland_viscosity from the new T, each with the arithmetic of its own
 // kernel, so every output has the same bits as the three kernels give
+// The per-node arrays p_inf_n, rho_inf_n, mu_factor_n (each may be nullptr)
+// replace the scalars on a mesh joining instances that differ in them.
 __global__ __launch_bounds__(256) void k_end_state(
-    int npoin, int npoin_w, const int *__restrict__ lpoin_w, double g, double Rgas, double p_inf,
-    double rho_inf, double mu_factor, const double *__restrict__ rho,
-    const double *__restrict__ q, const double *__restrict__ E, double *__restrict__ u,
-    double *__restrict__ e_int, double *__restrict__ pr, double *__restrict__ csound,
-    double *__restrict__ machno, double *__restrict__ Tem, double *__restrict__ eta,
-    double *__restrict__ mu_fluid)
+    int npoin, int npoin_w, const int *__restrict__ lpoin_w, double g, double Rgas,
+    double p_inf_s, double rho_inf_s, double mu_factor_s, const double *__restrict__ p_inf_n,
+    const double *__restrict__ rho_inf_n, const double *__restrict__ mu_factor_n,
+    const double *__restrict__ rho, const double *__restrict__ q, const double *__restrict__ E,
+    double *__restrict__ u, double *__restrict__ e_int, double *__restrict__ pr,
+    double *__restrict__ csound, double *__restrict__ machno, double *__restrict__ Tem,
+    double *__restrict__ eta, double *__restrict__ mu_fluid)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npoin_w) return;
     const int n = lpoin_w[i] - 1;
+    const double p_inf = p_inf_n ? p_inf_n[n] : p_inf_s;
+    const double rho_inf = rho_inf_n ? rho_inf_n[n] : rho_inf_s;
+    const double mu_factor = mu_factor_n ? mu_factor_n[n] : mu_factor_s;
     const double r = rho[n];
     const double e2 = eta[n + (size_t)npoin];
     double umag = 0.0;
@@ -501,8 +507,28 @@ extern "C" int sodhip_eos_entropy_step_f64(sodhip_ctx ctx, int npoin, int npoin_
     SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     SODHIP_NODE_LAUNCH(3, npoin_w, k_end_state, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, p_inf,
-                       rho_inf, mu_factor, rho, q, E, u, e_int, pr, csound, machno, Tem, eta,
+                       rho_inf, mu_factor, (const double *)nullptr, (const double *)nullptr,
+                       (const double *)nullptr, rho, q, E, u, e_int, pr, csound, machno, Tem, eta,
                        mu_fluid);
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_eos_entropy_step_ens_f64(sodhip_ctx ctx, int npoin, int npoin_w,
+                                               const int *lpoin_w, double gamma_gas, double Rgas,
+                                               const sodhip_instances *ens, const double *rho,
+                                               const double *q, const double *E, double *u,
+                                               double *e_int, double *pr, double *csound,
+                                               double *machno, double *Tem, double *eta,
+                                               double *mu_fluid)
+{
+    SODHIP_REQUIRE(ctx && ens && ens->p_inf && ens->rho_inf && ens->mu_factor && lpoin_w && rho &&
+                       q && E && u && e_int && pr && csound && machno && Tem && eta && mu_fluid,
+                   "null argument");
+    SODHIP_REQUIRE(npoin_w >= 0 && npoin_w <= npoin, "npoin_w outside 0..npoin");
+    SODHIP_TRY_HIP(hipSetDevice(ctx->device));
+    SODHIP_NODE_LAUNCH(3, npoin_w, k_end_state, npoin, npoin_w, lpoin_w, gamma_gas, Rgas, 0.0, 0.0,
+                       0.0, ens->p_inf, ens->rho_inf, ens->mu_factor, rho, q, E, u, e_int, pr,
+                       csound, machno, Tem, eta, mu_fluid);
     return SODHIP_OK;
 }
 

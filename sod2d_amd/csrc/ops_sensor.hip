@@ -48,18 +48,14 @@ template <typename T> __device__ __forceinline__ T block_max(T v, T *sh)
     return r;
 }
 
-// partial sums of eta over working nodes: block b owns a contiguous slice
-// (FP64 sums for either storage type)
+// Block-wide sum of eta over the lpoin_w positions [i0, i1).  Four strided
+// values are loaded ahead and added in the order of the plain strided loop,
+// so the partial sums do not depend on the unrolling.
 template <typename T>
-__global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int *__restrict__ lpoin_w,
-                                                         const T *__restrict__ eta,
-                                                         double *__restrict__ part)
+__device__ __forceinline__ double eta_slice_sum(const int *__restrict__ lpoin_w,
+                                                const T *__restrict__ eta, int i0, int i1,
+                                                double *sh)
 {
-    __shared__ double sh[kRedThreads];
-    const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
-    const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
-    // four strided values are loaded ahead and added in the order of the
-    // plain strided loop, so the partial sums do not depend on the unrolling
     const int bs = blockDim.x;
     double s = 0.0;
     int i = i0 + threadIdx.x;
@@ -74,7 +70,46 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int 
         s += v3;
     }
     for (; i < i1; i += bs) s += (double)eta[lpoin_w[i] - 1];
-    s = block_sum(s, sh);
+    return block_sum(s, sh);
+}
+
+// Block-wide max |eta - mean| over the lpoin_w positions [i0, i1), with mean
+// the sum of the nparts partial sums part[] (in a fixed order) over nw.
+template <typename T>
+__device__ __forceinline__ double eta_slice_maxdev(const int *__restrict__ lpoin_w,
+                                                   const T *__restrict__ eta,
+                                                   const double *__restrict__ part, int nparts,
+                                                   int nw, int i0, int i1, double *sh)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    const double mean = block_sum(s, sh) / (double)nw;
+    const int bs = blockDim.x;
+    double m = 0.0;
+    int i = i0 + threadIdx.x;
+    for (; i + 3 * bs < i1; i += 4 * bs) {
+        const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
+                  n3 = lpoin_w[i + 3 * bs];
+        const double v0 = (double)eta[n0 - 1], v1 = (double)eta[n1 - 1],
+                     v2 = (double)eta[n2 - 1], v3 = (double)eta[n3 - 1];
+        m = fmax(fmax(m, fabs(v0 - mean)), fabs(v1 - mean));
+        m = fmax(fmax(m, fabs(v2 - mean)), fabs(v3 - mean));
+    }
+    for (; i < i1; i += bs) m = fmax(m, fabs((double)eta[lpoin_w[i] - 1] - mean));
+    return block_max(m, sh);
+}
+
+// partial sums of eta over working nodes: block b owns a contiguous slice
+// (FP64 sums for either storage type)
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_eta_sum(int npoin_w, const int *__restrict__ lpoin_w,
+                                                         const T *__restrict__ eta,
+                                                         double *__restrict__ part)
+{
+    __shared__ double sh[kRedThreads];
+    const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
+    const double s = eta_slice_sum(lpoin_w, eta, i0, i1, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -88,25 +123,58 @@ __global__ __launch_bounds__(kRedThreads) void k_eta_maxdev(int npoin_w,
                                                             int nparts, double *__restrict__ pmax)
 {
     __shared__ double sh[kRedThreads];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
-    const double mean = block_sum(s, sh) / (double)npoin_w;
     const int per = (npoin_w + gridDim.x - 1) / gridDim.x;
     const int i0 = blockIdx.x * per, i1 = min(npoin_w, i0 + per);
-    const int bs = blockDim.x;
-    double m = 0.0;
-    int i = i0 + threadIdx.x;
-    for (; i + 3 * bs < i1; i += 4 * bs) {
-        const int n0 = lpoin_w[i], n1 = lpoin_w[i + bs], n2 = lpoin_w[i + 2 * bs],
-                  n3 = lpoin_w[i + 3 * bs];
-        const double v0 = (double)eta[n0 - 1], v1 = (double)eta[n1 - 1],
-                     v2 = (double)eta[n2 - 1], v3 = (double)eta[n3 - 1];
-        m = fmax(fmax(m, fabs(v0 - mean)), fabs(v1 - mean));
-        m = fmax(fmax(m, fabs(v2 - mean)), fabs(v3 - mean));
-    }
-    for (; i < i1; i += bs) m = fmax(m, fabs((double)eta[lpoin_w[i] - 1] - mean));
-    m = block_max(m, sh);
+    const double m = eta_slice_maxdev(lpoin_w, eta, part, nparts, npoin_w, i0, i1, sh);
     if (threadIdx.x == 0) pmax[blockIdx.x] = m;
+}
+
+// The same two passes per instance of a joined mesh: instance i owns the
+// lpoin_w positions wseg[i]..wseg[i+1]-1 and the blocks i*nb..i*nb+nb-1,
+// which split its segment as nb blocks split a single mesh's list (so the
+// partials, mean and maximum have the bits of the instance run alone).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_eta_sum_seg(const int *__restrict__ wseg, int nb,
+                                                             const int *__restrict__ lpoin_w,
+                                                             const T *__restrict__ eta,
+                                                             double *__restrict__ part)
+{
+    __shared__ double sh[kRedThreads];
+    const int inst = blockIdx.x / nb, b = blockIdx.x % nb;
+    const int w0 = wseg[inst], nw = wseg[inst + 1] - w0;
+    const int per = (nw + nb - 1) / nb;
+    const int i0 = w0 + b * per, i1 = min(w0 + nw, i0 + per);
+    const double s = eta_slice_sum(lpoin_w, eta, i0, i1, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_eta_maxdev_seg(const int *__restrict__ wseg,
+                                                                int nb,
+                                                                const int *__restrict__ lpoin_w,
+                                                                const T *__restrict__ eta,
+                                                                const double *__restrict__ part,
+                                                                double *__restrict__ pmax)
+{
+    __shared__ double sh[kRedThreads];
+    const int inst = blockIdx.x / nb, b = blockIdx.x % nb;
+    const int w0 = wseg[inst], nw = wseg[inst + 1] - w0;
+    const int per = (nw + nb - 1) / nb;
+    const int i0 = w0 + b * per, i1 = min(w0 + nw, i0 + per);
+    const double m = eta_slice_maxdev(lpoin_w, eta, part + (size_t)inst * nb, nb, nw, i0, i1, sh);
+    if (threadIdx.x == 0) pmax[blockIdx.x] = m;
+}
+
+// per instance: the maximum of its nb partial maxima (any order gives the
+// same maximum)
+__global__ __launch_bounds__(64) void k_norm_seg(int nb, const double *__restrict__ pmax,
+                                                 double *__restrict__ norm_i)
+{
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 64) m = fmax(m, pmax[(size_t)blockIdx.x * nb + i]);
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s, 64));
+    if (threadIdx.x == 0) norm_i[blockIdx.x] = m;
 }
 
 template <typename T> __device__ __forceinline__ T block_min(T v, T *sh)
@@ -364,7 +432,8 @@ __global__ __launch_bounds__(SG<P>::NT) void k_smart_visc_ch(SensorArgsT<S> A)
 #pragma unroll
                 for (int w2 = 1; w2 < NW; ++w2) be = fmax(be, wmax[w2]);
             }
-            const double R1 = cur.rho * fabs(cur.Reta) / norm;
+            const double nrm = A.norm_i ? A.norm_i[A.elem_inst[e]] : norm;
+            const double R1 = cur.rho * fabs(cur.Reta) / nrm;
             const double h = lb[c * NN + a];
             const double Ve = A.ce * R1 * (h * h);
             const double ml = A.cglob * fmin(Ve, be);
@@ -418,6 +487,25 @@ int entropy_norm_partials(sodhip_ctx ctx, int npoin_w, const int *lpoin_w, const
     return SODHIP_OK;
 }
 
+int entropy_norm_instances(sodhip_ctx ctx, const sodhip_instances *ens, const int *lpoin_w,
+                           const double *eta, const double **norm_i)
+{
+    SODHIP_REQUIRE(ens && ens->ninst > 0 && ens->wseg && ens->elem_inst, "invalid instances");
+    const size_t np = (size_t)ens->ninst * kRedBlocks;
+    int rc = ensure_buf(ctx->scratch2, (2 * np + ens->ninst) * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->scratch2.ptr, *pmax = part + np, *nrm = pmax + np;
+    const int grid = ens->ninst * kRedBlocks;
+    k_eta_sum_seg<double><<<grid, kRedThreads, 0, ctx->stream>>>(ens->wseg, kRedBlocks, lpoin_w,
+                                                                 eta, part);
+    k_eta_maxdev_seg<double><<<grid, kRedThreads, 0, ctx->stream>>>(ens->wseg, kRedBlocks,
+                                                                    lpoin_w, eta, part, pmax);
+    k_norm_seg<<<ens->ninst, 64, 0, ctx->stream>>>(kRedBlocks, pmax, nrm);
+    SODHIP_TRY_HIP(hipGetLastError());
+    *norm_i = nrm;
+    return SODHIP_OK;
+}
+
 } // namespace sodhip
 
 using namespace sodhip;
@@ -427,7 +515,7 @@ static int smart_visc(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoi
                       const int *connec, const int *lpoin_w, const T *Reta, double gamma_gas,
                       const T *rho, const T *u, const T *Tem, const T *eta, const T *helem,
                       const T *helem_k, const int *invAtoIJK, const sodhip_visc_params *prm,
-                      T *mu_e, T *mue_l)
+                      T *mu_e, T *mue_l, const sodhip_instances *ens = nullptr)
 {
     SODHIP_REQUIRE(ctx && prm, "null argument");
     SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
@@ -445,7 +533,12 @@ static int smart_visc(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoi
     double *pmax = part + kRedBlocks;
     const bool normalise = prm->flag_normalise_entropy == 1;
     int tok = prof_begin(ctx, 2);
-    if (normalise) {
+    const double *norm_i = nullptr;
+    if (normalise && ens) {
+        if constexpr (sizeof(T) == 8) {
+            if ((rc = entropy_norm_instances(ctx, ens, lpoin_w, eta, &norm_i))) return rc;
+        }
+    } else if (normalise) {
         k_eta_sum<T><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part);
         k_eta_maxdev<T><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(npoin_w, lpoin_w, eta, part,
                                                                      kRedBlocks, pmax);
@@ -467,8 +560,10 @@ static int smart_visc(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoi
     A.cmax = prm->cmax;
     A.cglob = prm->cglob;
     A.fact_low_mach = prm->flag_drop_c_in_envit ? 0.0 : 1.0;
-    A.pmax = normalise ? pmax : nullptr;
+    A.pmax = normalise && !ens ? pmax : nullptr;
     A.nparts = kRedBlocks;
+    A.norm_i = norm_i;
+    A.elem_inst = norm_i ? ens->elem_inst : nullptr;
     A.mu_e = mu_e;
     A.mue_l = mue_l;
     switch (porder) {
@@ -492,6 +587,18 @@ extern "C" int sodhip_smart_visc_spectral_f64(sodhip_ctx ctx, int porder, int ne
 {
     return smart_visc<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, gamma_gas,
                               rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm, mu_e, mue_l);
+}
+
+extern "C" int sodhip_smart_visc_spectral_ens_f64(
+    sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_w, const int *connec,
+    const int *lpoin_w, const sodhip_instances *ens, const double *Reta, double gamma_gas,
+    const double *rho, const double *u, const double *Tem, const double *eta,
+    const double *helem, const double *helem_k, const int *invAtoIJK,
+    const sodhip_visc_params *prm, double *mu_e, double *mue_l)
+{
+    SODHIP_REQUIRE(ens && ens->ninst > 0 && ens->wseg && ens->elem_inst, "invalid instances");
+    return smart_visc<double>(ctx, porder, nelem, npoin, npoin_w, connec, lpoin_w, Reta, gamma_gas,
+                              rho, u, Tem, eta, helem, helem_k, invAtoIJK, prm, mu_e, mue_l, ens);
 }
 
 extern "C" int sodhip_smart_visc_spectral_f32(sodhip_ctx ctx, int porder, int nelem, int npoin,

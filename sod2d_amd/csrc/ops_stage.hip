@@ -28,7 +28,8 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
                        const T *gpvol, const T *dl, const int *inv, const int *gI, const int *gJ,
                        const int *gK, const double *rho, const double *q, const double *E,
                        const T *mu_e, double gamma, double Rgas, double Cp, double Pr,
-                       double mu_factor, double c_rho, double c_ener, double p_ref, T *out);
+                       double mu_factor, const double *mu_factor_e, double c_rho, double c_ener,
+                       double p_ref, T *out);
 int fused_record_width(int porder, int op_bytes);
 
 // RD1 / RDY: the base state y1 / the current state y enters the combine (its
@@ -221,10 +222,11 @@ int gather_update_on(sodhip_ctx ctx, int npoin, int nw, const int *list, const i
     if (rw != 5 && rw != 6) return set_error(SODHIP_ERR_ARG, "record width %d", rw);
     if (nw <= 0) return SODHIP_OK;
     int tok = prof_begin(ctx, 1);
-    const int grid = ((nw + 255) / 256 + 7) / 8 * 8;
+    const int gt = ctx->gather_threads;
+    const int grid = ((nw + gt - 1) / gt + 7) / 8 * 8;
     const NextStage nx = next ? *next : NextStage{0.0, 0.0, nullptr, nullptr, nullptr};
 #define SODHIP_GU3(R1, RY, NX, RWV)                                                            \
-    k_gather_update<TS, R1, RY, NX, RWV><<<grid, 256, 0, ctx->stream>>>(                       \
+    k_gather_update<TS, R1, RY, NX, RWV><<<grid, gt, 0, ctx->stream>>>(                        \
         npoin, nw, list, row, src, scr, 0, invMl, fact, lam1, lams, gam * dt,                   \
         rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener, nx)
 #define SODHIP_GU2(R1, RY, NX)                                                                 \
@@ -302,7 +304,8 @@ static int rk_stage_on(sodhip_ctx ctx, int porder, const MeshCache *m, int npoin
             double *rec = (double *)scr_mem;
             int rc = rhs_cluster_stage(ctx, m, cl, He, gpvol, dl, inv, gI, gJ, gK, mu_e,
                                        prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr,
-                                       prm->mu_factor, prm->c_rho, prm->c_ener, prm->p_ref, invMl,
+                                       prm->mu_factor, prm->mu_factor_e, prm->c_rho,
+                                       prm->c_ener, prm->p_ref, invMl,
                                        prm->fact, dt, lam1, lams, gam, rho1, q1, E1, rho, q, E,
                                        Rmass, Rmom, Rener, next, rec);
             if (rc) return rc;
@@ -318,7 +321,7 @@ static int rk_stage_on(sodhip_ctx ctx, int porder, const MeshCache *m, int npoin
                          rw * EN * sizeof(T), scr_bytes);
     T *scr = (T *)scr_mem;
     int rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, rho, q, E, mu_e,
-                                   prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
+                                   prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor, prm->mu_factor_e,
                                    prm->c_rho, prm->c_ener, prm->p_ref, scr);
     if (rc) return rc;
     SODHIP_TRY_HIP(hipGetLastError());
@@ -375,7 +378,7 @@ static int rk4_stage(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin
     if ((rc = ensure_buf(ctx->scratch, rw * EN * sizeof(T)))) return rc;
     T *scr = (T *)ctx->scratch.ptr;
     rc = rhs_fused_elements<T>(ctx, porder, m, He, gpvol, dl, inv, gI, gJ, gK, srho, sq, sE, mu_e,
-                               prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor,
+                               prm->gamma_gas, prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor, prm->mu_factor_e,
                                prm->c_rho, prm->c_ener, prm->p_ref, scr);
     if (rc) return rc;
     int tok = prof_begin(ctx, 1);

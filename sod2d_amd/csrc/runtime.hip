@@ -364,6 +364,31 @@ extern "C" int sodhip_pin_scratch(sodhip_ctx ctx, int pin)
     return SODHIP_OK;
 }
 
+extern "C" int sodhip_set_stage_config(sodhip_ctx ctx, int elem_blocks_per_cu, int geo_rows16,
+                                       int gather_threads)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    SODHIP_REQUIRE(elem_blocks_per_cu >= 1 && elem_blocks_per_cu <= 64,
+                   "elem_blocks_per_cu %d outside 1..64", elem_blocks_per_cu);
+    SODHIP_REQUIRE(geo_rows16 == 0 || geo_rows16 == 1, "geo_rows16 must be 0 or 1");
+    SODHIP_REQUIRE(gather_threads == 64 || gather_threads == 128 || gather_threads == 256,
+                   "gather_threads %d not 64, 128 or 256", gather_threads);
+    ctx->stage_bpc = elem_blocks_per_cu;
+    ctx->stage_geo16 = geo_rows16;
+    ctx->gather_threads = gather_threads;
+    return SODHIP_OK;
+}
+
+extern "C" int sodhip_get_stage_config(sodhip_ctx ctx, int *elem_blocks_per_cu, int *geo_rows16,
+                                       int *gather_threads)
+{
+    SODHIP_REQUIRE(ctx, "null context");
+    if (elem_blocks_per_cu) *elem_blocks_per_cu = ctx->stage_bpc;
+    if (geo_rows16) *geo_rows16 = ctx->stage_geo16;
+    if (gather_threads) *gather_threads = ctx->gather_threads;
+    return SODHIP_OK;
+}
+
 extern "C" int sodhip_set_stream(sodhip_ctx ctx, void *stream)
 {
     SODHIP_REQUIRE(ctx, "null context");

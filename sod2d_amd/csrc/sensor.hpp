@@ -23,6 +23,10 @@ template <typename T> struct SensorArgsT {
     double gamma, Rgas, ce, cmax, cglob, fact_low_mach;
     const double *pmax; // partial maxima (nparts) or nullptr when not normalising
     int nparts;
+    // joined instances: the normalisation of element e is norm_i[elem_inst[e]]
+    // (norm_i nullptr: the one normalisation above)
+    const double *norm_i;
+    const int *elem_inst;
     T *mu_e, *mue_l;
 };
 using SensorArgs = SensorArgsT<double>;
@@ -77,5 +81,9 @@ constexpr int kRedBlocks = 512;
 // launches the entropy normalisation's reductions (ops_sensor.hip)
 int entropy_norm_partials(sodhip_ctx ctx, int npoin_w, const int *lpoin_w, const double *eta,
                           bool normalise, const double **pmax, int *nparts);
+// the normalisation of every instance of a joined mesh into *norm_i (ninst
+// doubles in ctx->scratch2 after the partials); ops_sensor.hip
+int entropy_norm_instances(sodhip_ctx ctx, const sodhip_instances *ens, const int *lpoin_w,
+                           const double *eta, const double **norm_i);
 
 } // namespace sodhip

@@ -68,6 +68,8 @@ module mod_sod2d_hip
    ! sodhip_stage_params: the physical constants of a fused stage
    type, bind(C) :: stage_params_t
       real(c_double) :: gamma_gas, Rgas, Cp, Pr, mu_factor, c_rho, c_ener, fact, p_ref
+      ! per-element Sutherland factor (device), or none: the uniform mu_factor
+      type(c_ptr) :: mu_factor_e = c_null_ptr
    end type stage_params_t
 
    type(c_ptr), save :: ctx = c_null_ptr

@@ -123,8 +123,14 @@ class TGVRun:
                  uniform_rho: bool = False, sensor: bool = True, ctx: Context | None = None,
                  geometry=None, initial=None, sensor_path: str = "spectral", lps: bool = False,
                  high_mach: bool = False, integrator: str = "ls", surrogate=None,
-                 judge_every: int = 10):
+                 judge_every: int = 10, instances=None):
         self.t = params or TGVParams()
+        # joined instances (sod2d_amd.ensemble): per-instance Sutherland factor
+        # and entropy reference state, per-instance entropy normalisation
+        self.ens = instances
+        if instances is not None and (op_precision != 8 or lps or integrator != "ls" or
+                                      sensor_path != "spectral"):
+            raise SodhipError("joined instances run the FP64 plain path with the spectral sensor")
         if integrator not in ("ls", "rk4"):
             raise SodhipError(f"unknown integrator {integrator!r}")
         if integrator == "rk4" and lps:
@@ -231,6 +237,8 @@ class TGVRun:
         self.sp = StageParams(self.t.gamma_gas, self.t.Rgas, self.t.Cp, self.t.Prt,
                               self.t.mu_factor, 1.0, 1.0, -1.0,
                               self.t.po if op_precision == 4 else 0.0)
+        if self.ens is not None:
+            self.sp.mu_factor_e = self.ens.mu_factor_e.data_ptr()
         self.vp = ViscParams(0.1, 0.5, 1.0, self.t.Rgas, 1, 0)
         self.ctx.sync_stream()
         # evalInitialViscosity + evalInitialDt (CFDSolverBase.f90:1384-1460)
@@ -239,7 +247,8 @@ class TGVRun:
 
     # ------------------------------------------------------------ helpers
     def _sutherland(self):
-        check(lib().sodhip_sutherland_f64(self.ctx.h, self.npoin, _ptr(self.Tem[1]), None,
+        muf = _ptr(self.ens.mu_factor) if self.ens is not None else None
+        check(lib().sodhip_sutherland_f64(self.ctx.h, self.npoin, _ptr(self.Tem[1]), muf,
                                           self.t.mu_factor, _ptr(self.mu_fluid)), "sutherland")
 
     def adapt_dt(self) -> float:
@@ -283,7 +292,13 @@ class TGVRun:
             # beside it are logged (relative error, fallback count)
             mu_s = torch.empty_like(self.mu_e)
             self._surrogate_visc(Reta, rho, u, Tem, eta, mu_s)
-        if self.sensor_path in ("spectral", "surrogate"):
+        if self.ens is not None:
+            check(lib().sodhip_smart_visc_spectral_ens_f64(
+                h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), C.byref(self.ens.struct),
+                _ptr(Reta), self.t.gamma_gas, _ptr(rho), _ptr(u), _ptr(Tem), _ptr(eta),
+                _ptr(self.helem_l), _ptr(self.helem), _ptr(self.tabs[0]), C.byref(self.vp),
+                _ptr(self.mu_e), _ptr(self.mue_l)), "smart_visc_spectral_ens")
+        elif self.sensor_path in ("spectral", "surrogate"):
             check(lib().sodhip_smart_visc_spectral_f64(
                 h, p, ne, n, nw, _ptr(self.connec), _ptr(self.lpoin_w), _ptr(Reta),
                 self.t.gamma_gas, _ptr(rho), _ptr(u), _ptr(Tem), _ptr(eta), _ptr(self.helem_l),
@@ -448,7 +463,15 @@ class TGVRun:
         """EOS, entropy, entropy residual, sensor and Sutherland after the stages
         (time_integ_ls.f90:389-473)."""
         h, n, t = self.ctx.h, self.npoin, self.t
-        if self.fuse_end_state:
+        if self.ens is not None:
+            # per node: the instance's entropy reference state and Sutherland factor
+            check(lib().sodhip_eos_entropy_step_ens_f64(
+                h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas,
+                C.byref(self.ens.struct), _ptr(self.rho[1]), _ptr(self.q[1]), _ptr(self.E[1]),
+                _ptr(self.u[1]), _ptr(self.e_int[1]), _ptr(self.pr[1]), _ptr(self.csound),
+                _ptr(self.machno), _ptr(self.Tem[1]), _ptr(self.eta), _ptr(self.mu_fluid)),
+                "eos_entropy_step_ens")
+        elif self.fuse_end_state:
             # Sutherland moves ahead of the sensor, which does not read mu_fluid
             check(lib().sodhip_eos_entropy_step_f64(
                 h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.Rgas, t.po, t.rho0,
@@ -466,7 +489,7 @@ class TGVRun:
                 h, n, self.npoin_w, _ptr(self.lpoin_w), t.gamma_gas, t.po, t.rho0,
                 _ptr(self.rho[1]), _ptr(self.pr[1]), None, _ptr(self.eta), None),
                 "thermo_entropy")
-        if self.fuse_end_state:
+        if self.fuse_end_state or self.ens is not None:
             # Reta(:,2) of f_eta = u(:,:,1)*eta(:,1), its lumped solve and the
             # residual in one element pass and one node pass
             check(lib().sodhip_entropy_convec_residual_f64(
@@ -482,7 +505,7 @@ class TGVRun:
         if self.sensor:
             self._smart_visc(self.auxReta, self.rho[1], self.u[1], self.Tem[1], self.eta[n:],
                              ducros=True)
-        if not self.fuse_end_state:
+        if not self.fuse_end_state and self.ens is None:
             self._sutherland()
 
     # ------------------------------------------------------------ diagnostics
