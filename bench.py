@@ -374,6 +374,13 @@ def main(argv=None):
                          "ensemble: 32 uneven TGV instances joined into one mesh")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble summary in the default line")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape of the fused stage instead of the "
+                         "autotuner's (sod2d_amd/autotune.py, run after the warm-up steps)")
+    ap.add_argument("--stage-config", default=None, metavar="B,G,T",
+                    help="use this launch shape of the fused stage (element blocks per CU, "
+                         "16-byte geometry rows 0/1, gather threads) instead of autotuning, "
+                         "e.g. to profile the shape a tuned run chose")
     ap.add_argument("--op-n", type=int, default=128,
                     help="elements per direction of the operator workload's box")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -417,9 +424,21 @@ def main(argv=None):
                  dt_mode="fixed", lps=a.lps, sensor_path="live" if a.lps else "spectral")
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
-    for _ in range(max(a.warmup, 1 if a.graph else 0)):
+    for _ in range(max(a.warmup, 1 if (a.graph or not a.no_autotune) else 0)):
         run.step()
     torch.cuda.synchronize()
+    tune = None
+    if a.stage_config and not a.lps:
+        from sod2d_amd import autotune
+        cfg = tuple(int(x) for x in a.stage_config.split(","))
+        autotune.set_config(run.ctx, cfg)
+        tune = {"choice": {"elem_blocks_per_cu": cfg[0], "geo_rows16": cfg[1],
+                           "gather_threads": cfg[2]}, "given": True}
+    elif not a.no_autotune and not a.lps:
+        from sod2d_amd import autotune
+        tune = autotune.tune_stage(run, reps=3)
+        run.step()
+        torch.cuda.synchronize()
     if a.graph:
         graph = run.capture()
         a.steps += a.steps % 2
@@ -503,6 +522,15 @@ def main(argv=None):
             "porder": a.porder, "elements": run.nelem, "unique_nodes": N,
             "rhs_per_step": nrhs, "operator_precision": a.precision, "dt": run.dt,
             "setup_s": round(t_setup, 2),
+            "stage_launch": (None if tune is None else {
+                k: tune["choice"][k] for k in ("elem_blocks_per_cu", "geo_rows16",
+                                               "gather_threads")} | ({"given": True}
+                if tune.get("given") else {
+                "autotuned": True, "ms_stage_tuned": round(tune["choice"]["ms_stage"], 5),
+                "ms_stage_default": round(next(
+                    r["ms_stage"] for r in tune["table"]
+                    if (r["elem_blocks_per_cu"], r["geo_rows16"], r["gather_threads"])
+                    == (16, 1, 256)), 5)})),
         },
         "roofline": {
             "kernel": ("LPS stage (sodhip_rk_stage_lps_f64): element kernels (convection, "
