@@ -464,6 +464,20 @@ def main(argv=None):
     prof = {k: run.ctx.prof_get(k) for k in range(4)}
     run.ctx.prof_enable(False)
     diag = run.diagnostics()
+    # the same run with the adaptive time step (CFL dt reduced and read back
+    # every step, stage 1 then evaluated at the start of each step)
+    adaptive = None
+    if not a.lps and not a.graph:
+        run.dt_mode = "adaptive"
+        run.step()
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(5):
+            run.step()
+        torch.cuda.synchronize()
+        adaptive = {"ms_per_step": round((time.perf_counter() - ta) / 5 * 1e3, 4), "steps": 5,
+                    "dt": run.dt}
+        run.dt_mode = "fixed"
     if a.copy_probe > 0:
         n = a.copy_probe * (1 << 20) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -548,6 +562,7 @@ def main(argv=None):
                         "sensor": round(prof[2][0] / prof_steps, 4),
                         "other": round(prof[3][0] / prof_steps, 4)},
         "diagnostics": {k: diag[k] for k in ("time", "EK", "eps_T", "maxmach")},
+        "adaptive_dt": adaptive,
     }
     if not a.no_surrogate and not a.lps:
         del run

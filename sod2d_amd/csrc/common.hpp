@@ -181,6 +181,9 @@ void prof_end(sodhip_ctx ctx, int token);
 
 // XCD-aware grid: number of blocks for a grid-stride element loop
 int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu);
+// grid of the lane-per-node element loops (xcd_elems): a multiple of 8, at
+// most blocks_per_cu blocks per CU
+int diag_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu = 32);
 
 
 // first stage of the next step stored by a stage's gather/update (ops_stage.hip):

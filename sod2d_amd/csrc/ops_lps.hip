@@ -268,37 +268,42 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
     }
 }
 
-// comp_tau (elem_stab.f90:589-616): one thread per element over the
-// reference's element-fastest connec.
+// comp_tau (elem_stab.f90:589-616): tau_e = max over the element's nodes of
+// h (fact_c c + |u|), h = helem_k (c_species_stab / p); one thread per
+// element over the reference's element-fastest connec, the nodes visited an
+// i-line at a time (build_alex) with all loads of a line issued together
+// (fmax is exact).
 __global__ __launch_bounds__(256) void k_comp_tau(int nelem, int nnode, int porder,
                                                   const int *__restrict__ connec,
+                                                  const int *__restrict__ perm,
                                                   const double *__restrict__ csound,
                                                   const double *__restrict__ u, int npoin,
                                                   const double *__restrict__ helem_k,
                                                   double c_species_stab, double fact_c,
                                                   double *__restrict__ tau)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int alex[125];
+    build_alex(perm, nnode, alex);
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= nelem) return;
     const double h = helem_k[e] * (c_species_stab / (double)porder);
-    // nodes KA at a time with all loads of a group issued first (fmax is
-    // exact); a group past nnode repeats the last node
-    constexpr int KA = 4;
+    const int KA = porder + 1;
     double taul = 0.0;
-    for (int a0 = 0; a0 < nnode; a0 += KA) {
-        int n[KA];
+    for (int l0 = 0; l0 < nnode; l0 += KA) {
+        int n[5];
 #pragma unroll
-        for (int j = 0; j < KA; ++j) n[j] = connec[e + (size_t)nelem * min(a0 + j, nnode - 1)] - 1;
-        double u0[KA], u1[KA], u2[KA], cs[KA];
+        for (int j = 0; j < 5; ++j)
+            n[j] = connec[e + (size_t)nelem * alex[min(l0 + min(j, KA - 1), nnode - 1)]] - 1;
+        double u0[5], u1[5], u2[5], cs[5];
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
+        for (int j = 0; j < 5; ++j) {
             u0[j] = u[n[j]];
             u1[j] = u[n[j] + (size_t)npoin];
             u2[j] = u[n[j] + 2 * (size_t)npoin];
             cs[j] = csound[n[j]];
         }
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
+        for (int j = 0; j < 5; ++j) {
             const double aux1 = fact_c * cs[j] + sqrt(u0[j] * u0[j] + u1[j] * u1[j] + u2[j] * u2[j]);
             taul = fmax(taul, h * aux1);
         }
@@ -467,13 +472,16 @@ extern "C" int sodhip_comp_tau_f64(sodhip_ctx ctx, int porder, int nelem, int np
                                    int flag_drop_c_in_envit, double *tau)
 {
     SODHIP_REQUIRE(ctx, "null context");
-    SODHIP_REQUIRE(porder >= 1 && porder <= kMaxP, "porder %d not supported", porder);
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
     SODHIP_REQUIRE(nelem > 0 && npoin > 0, "invalid mesh arguments");
     SODHIP_REQUIRE(connec && csound && u && helem_k && tau, "null array argument");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
     const int nn = nnode_of(porder);
-    k_comp_tau<<<(nelem + 255) / 256, 256, 0, ctx->stream>>>(
-        nelem, nn, porder, connec, csound, u, npoin, helem_k, c_species_stab,
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    k_comp_tau<<<tpe_grid(nelem), 256, 0, ctx->stream>>>(
+        nelem, nn, porder, connec, m->perm, csound, u, npoin, helem_k, c_species_stab,
         flag_drop_c_in_envit ? 0.0 : 1.0, tau);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;

@@ -207,14 +207,16 @@ __global__ __launch_bounds__(kRedThreads) void k_rho_min(int npoin_w, const int 
 }
 
 // smart_visc_spectral_imex (mod_entropy_viscosity.f90:113-227), one thread per
-// element (connec in the reference layout: consecutive threads read
-// consecutive words): mue_l <- incoming mu_e; beta_e = max over the nodes of
-// rho helem_k (cmax/p) (|u| + fact c); mu_e = cglob min(ce rho |Reta|/norm
-// (helem_k/p)^2, beta_e).  norm: 1, or the max of the partials (max |eta -
-// mean|), or the min of the partials (min rho), folded per block.
+// element over the reference's element-fastest connec, mu_e and mue_l (for a
+// node position the lanes of a wave read and write consecutive entries):
+// mue_l <- incoming mu_e; beta_e = max over the nodes of rho helem_k (cmax/p)
+// (|u| + fact c); mu_e = cglob min(ce rho |Reta|/norm (helem_k/p)^2, beta_e).
+// norm: 1, or the max of the partials (max |eta - mean|), or the min of the
+// partials (min rho), folded per block.  The nodes are visited an i-line at a
+// time (build_alex) with all loads of a line issued together.
 struct ImexArgs {
     int nelem, nnode, porder;
-    const int *connec;
+    const int *connec, *perm;
     const double *Reta, *rho, *u, *csound, *helem_k;
     int npoin;
     double ce, cmax, cglob, fact_low_mach;
@@ -226,6 +228,8 @@ struct ImexArgs {
 __global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
 {
     __shared__ double sh[256];
+    __shared__ int alex[125];
+    build_alex(A.perm, A.nnode, alex);
     double norm = 1.0;
     if (A.part) {
         double v = A.norm_min ? 100000.0 : 0.0;
@@ -233,28 +237,25 @@ __global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
             v = A.norm_min ? fmin(v, A.part[i]) : fmax(v, A.part[i]);
         norm = A.norm_min ? block_min(v, sh) : block_max(v, sh);
     }
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= A.nelem) return;
     const size_t NE = (size_t)A.nelem;
     const double *u1 = A.u + A.npoin, *u2 = A.u + 2 * (size_t)A.npoin;
     const double hk = A.helem_k[e], cp = A.cmax / (double)A.porder;
-    // the nodes are taken KA at a time with all their loads issued before the
-    // stores and the arithmetic (in node order, as the plain loop), so that a
-    // lane has KA gathers in flight; a group past nnode repeats the last node
-    constexpr int KA = 4;
+    const int KA = A.porder + 1;
     double betae = 0.0;
-    for (int a0 = 0; a0 < A.nnode; a0 += KA) {
-        size_t ea[KA];
-        int n[KA];
-        double mu[KA], v0[KA], v1[KA], v2[KA], cs[KA], rh[KA];
+    for (int l0 = 0; l0 < A.nnode; l0 += KA) {
+        size_t ea[5];
+        int n[5];
+        double mu[5], v0[5], v1[5], v2[5], cs[5], rh[5];
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
-            ea[j] = e + NE * min(a0 + j, A.nnode - 1);
+        for (int j = 0; j < 5; ++j) {
+            ea[j] = e + NE * alex[min(l0 + min(j, KA - 1), A.nnode - 1)];
             n[j] = A.connec[ea[j]] - 1;
             mu[j] = A.mu_e[ea[j]];
         }
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
+        for (int j = 0; j < 5; ++j) {
             v0[j] = A.u[n[j]];
             v1[j] = u1[n[j]];
             v2[j] = u2[n[j]];
@@ -262,32 +263,31 @@ __global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
             rh[j] = A.rho[n[j]];
         }
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
-            if (a0 + j >= A.nnode) break;
-            A.mue_l[ea[j]] = mu[j];
+        for (int j = 0; j < 5; ++j) {
+            if (j < KA) A.mue_l[ea[j]] = mu[j];
             const double speed =
                 sqrt(v0[j] * v0[j] + v1[j] * v1[j] + v2[j] * v2[j]) + A.fact_low_mach * cs[j];
             betae = fmax(betae, (rh[j] * hk) * cp * speed);
         }
     }
     const double hp = hk / (double)A.porder;
-    for (int a0 = 0; a0 < A.nnode; a0 += KA) {
-        size_t ea[KA];
-        int n[KA];
-        double rh[KA], re[KA];
+    for (int l0 = 0; l0 < A.nnode; l0 += KA) {
+        size_t ea[5];
+        int n[5];
+        double rh[5], re[5];
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
-            ea[j] = e + NE * min(a0 + j, A.nnode - 1);
+        for (int j = 0; j < 5; ++j) {
+            ea[j] = e + NE * alex[min(l0 + min(j, KA - 1), A.nnode - 1)];
             n[j] = A.connec[ea[j]] - 1;
         }
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
+        for (int j = 0; j < 5; ++j) {
             rh[j] = A.rho[n[j]];
             re[j] = A.Reta[n[j]];
         }
 #pragma unroll
-        for (int j = 0; j < KA; ++j) {
-            if (a0 + j >= A.nnode) break;
+        for (int j = 0; j < 5; ++j) {
+            if (j >= KA) break;
             const double R1 = rh[j] * fabs(re[j]) / norm;
             const double Ve = A.ce * R1 * (hp * hp);
             A.mu_e[ea[j]] = A.cglob * fmin(Ve, betae);
@@ -295,65 +295,72 @@ __global__ __launch_bounds__(256) void k_visc_imex(ImexArgs A)
     }
 }
 
-// ducros_visc_limit (mod_entropy_viscosity.f90:277-314), one thread per
-// element-node (mu_e(nelem,ngaus) and connec in the reference layout)
-__global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int npoin,
+// ducros_visc_limit (mod_entropy_viscosity.f90:277-314) in two passes.  Of
+// the limiter's inputs only helem and mu_e are per element-node; the rest
+// depends on the node alone.  A node pass forms, per node, d2 = (divU/Ml)^2,
+// phi = min(4/3 d2 / (d2 + |curl u|^2 + eps), 1) and |u| (three arrays, read
+// and written in streams); the element pass gathers those three instead of
+// the eight node fields (eight gather streams per node overran the caches:
+// about 3.7 times the algorithmic bytes were fetched).  Same expressions,
+// same bits as one pass.
+__global__ __launch_bounds__(256) void k_ducros_node(int npoin, const double *__restrict__ u,
+                                                     const double *__restrict__ divU,
+                                                     const double *__restrict__ vorti,
+                                                     const double *__restrict__ Ml,
+                                                     double *__restrict__ nq)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npoin) return;
+    const size_t P1 = (size_t)npoin, P2 = 2 * (size_t)npoin;
+    const double dv = divU[n] / Ml[n];
+    const double w0 = vorti[n], w1 = vorti[n + P1], w2 = vorti[n + P2];
+    const double v0 = u[n], v1 = u[n + P1], v2 = u[n + P2];
+    const double d2 = dv * dv;
+    nq[n] = fmin((4.0 / 3.0) * d2 / (d2 + (w0 * w0 + w1 * w1 + w2 * w2) + DBL_EPSILON), 1.0);
+    nq[n + P1] = d2;
+    nq[n + P2] = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+}
+
+// the element pass: one thread per element over the reference's
+// element-fastest connec, helem and mu_e, the nodes visited an i-line at a
+// time (build_alex) with all loads of a line issued together
+__global__ __launch_bounds__(256) void k_ducros(int nelem, int nnode, int porder, int npoin,
                                                 const int *__restrict__ connec,
-                                                const double *__restrict__ u,
-                                                const double *__restrict__ divU,
-                                                const double *__restrict__ vorti,
-                                                const double *__restrict__ Ml,
+                                                const int *__restrict__ perm,
+                                                const double *__restrict__ nq,
                                                 const double *__restrict__ helem, double dmin,
                                                 double *__restrict__ mu_e)
 {
-    // Each thread takes KA consecutive element nodes of one element and issues
-    // all their loads before the arithmetic, so that a wave has KA gathers in
-    // flight per lane.  Threads are in tiles of 64 consecutive elements x all
-    // node groups, element fastest: a wave reads 64 consecutive entries of the
-    // element-fastest arrays, and the waves of one tile gather the nodes those
-    // elements share.
-    constexpr int KA = 4;
-    const int nag = (nnode + KA - 1) / KA;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t tile = i / (64 * (size_t)nag);
-    const int r = (int)(i - tile * 64 * (size_t)nag);
-    const size_t e = tile * 64 + (r & 63);
-    if (e >= (size_t)nelem) return;
-    const int a0 = (r >> 6) * KA;
+    __shared__ int alex[125];
+    build_alex(perm, nnode, alex);
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
     const size_t P1 = (size_t)npoin, P2 = 2 * (size_t)npoin;
-    size_t t[KA];
-    int n[KA];
+    const int KA = porder + 1;
+    for (int l0 = 0; l0 < nnode; l0 += KA) {
+        size_t t[5];
+        int n[5];
 #pragma unroll
-    for (int j = 0; j < KA; ++j) {
-        const int a = min(a0 + j, nnode - 1); // a group past nnode repeats the last node
-        t[j] = (size_t)a * nelem + e;
-        n[j] = connec[t[j]] - 1;
-    }
-    double dv[KA], w0[KA], w1[KA], w2[KA], v0[KA], v1[KA], v2[KA], he[KA], mu[KA];
+        for (int j = 0; j < 5; ++j) {
+            t[j] = (size_t)alex[min(l0 + min(j, KA - 1), nnode - 1)] * nelem + e;
+            n[j] = connec[t[j]] - 1;
+        }
+        double phi[5], d2[5], um[5], he[5], mu[5];
 #pragma unroll
-    for (int j = 0; j < KA; ++j) {
-        dv[j] = divU[n[j]] / Ml[n[j]];
-        w0[j] = vorti[n[j]];
-        w1[j] = vorti[n[j] + P1];
-        w2[j] = vorti[n[j] + P2];
-        v0[j] = u[n[j]];
-        v1[j] = u[n[j] + P1];
-        v2[j] = u[n[j] + P2];
-        he[j] = helem[t[j]];
-        mu[j] = mu_e[t[j]];
-    }
+        for (int j = 0; j < 5; ++j) {
+            phi[j] = nq[n[j]];
+            d2[j] = nq[n[j] + P1];
+            um[j] = nq[n[j] + P2];
+            he[j] = helem[t[j]];
+            mu[j] = mu_e[t[j]];
+        }
 #pragma unroll
-    for (int j = 0; j < KA; ++j) {
-        if (a0 + j >= nnode) break;
-        const double umag = sqrt(v0[j] * v0[j] + v1[j] * v1[j] + v2[j] * v2[j]);
-        const double d2 = dv[j] * dv[j];
-        const double phi = fmin((4.0 / 3.0) * d2 /
-                                    (d2 + (w0[j] * w0[j] + w1[j] * w1[j] + w2[j] * w2[j]) +
-                                     DBL_EPSILON),
-                                1.0);
-        const double rr = 1.0 * umag / he[j];
-        const double phi2 = d2 / (d2 + rr * rr + DBL_EPSILON);
-        mu_e[t[j]] = mu[j] * (fmax(phi - dmin, 0.0) * phi2 + dmin);
+        for (int j = 0; j < 5; ++j) {
+            if (j >= KA) break;
+            const double rr = 1.0 * um[j] / he[j];
+            const double phi2 = d2[j] / (d2[j] + rr * rr + DBL_EPSILON);
+            mu_e[t[j]] = mu[j] * (fmax(phi[j] - dmin, 0.0) * phi2 + dmin);
+        }
     }
 }
 
@@ -629,8 +636,10 @@ extern "C" int sodhip_smart_visc_spectral_imex_f64(
                        mue_l,
                    "null array argument");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
-    int rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double));
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
+    if ((rc = ensure_buf(ctx->scratch2, 2 * kRedBlocks * sizeof(double)))) return rc;
     double *part = (double *)ctx->scratch2.ptr;
     double *pmax = part + kRedBlocks;
     const bool normalise = prm->flag_normalise_entropy == 1;
@@ -650,6 +659,7 @@ extern "C" int sodhip_smart_visc_spectral_imex_f64(
     A.nnode = (porder + 1) * (porder + 1) * (porder + 1);
     A.porder = porder;
     A.connec = connec;
+    A.perm = m->perm;
     A.Reta = Reta;
     A.rho = rho;
     A.u = u;
@@ -665,7 +675,7 @@ extern "C" int sodhip_smart_visc_spectral_imex_f64(
     A.norm_min = high_mach ? 1 : 0;
     A.mu_e = mu_e;
     A.mue_l = mue_l;
-    k_visc_imex<<<(nelem + 255) / 256, 256, 0, ctx->stream>>>(A);
+    k_visc_imex<<<tpe_grid(nelem), 256, 0, ctx->stream>>>(A);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;
@@ -682,12 +692,18 @@ extern "C" int sodhip_ducros_visc_limit_f64(sodhip_ctx ctx, int porder, int nele
     SODHIP_REQUIRE(nelem > 0 && npoin > 0, "invalid mesh sizes");
     SODHIP_REQUIRE(connec && u && divU && vorti && Ml && helem && mu_e, "null array argument");
     SODHIP_TRY_HIP(hipSetDevice(ctx->device));
-    const size_t EN = (size_t)nelem * (porder + 1) * (porder + 1) * (porder + 1);
+    MeshCache *m = nullptr;
+    int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
+    if (rc) return rc;
+    // the node pass's three arrays in the element-node scratch (3 npoin doubles,
+    // far below the stage's records; the limiter runs between stages)
+    if ((rc = ensure_buf(ctx->scratch, 3 * (size_t)npoin * sizeof(double)))) return rc;
+    double *nq = (double *)ctx->scratch.ptr;
     int tok = prof_begin(ctx, 2);
-    const size_t nthr = (((size_t)nelem + 63) / 64) * 64 * ((EN / nelem + 3) / 4);
-    k_ducros<<<(unsigned)((nthr + 255) / 256), 256, 0, ctx->stream>>>(
-        nelem, (porder + 1) * (porder + 1) * (porder + 1), npoin, connec, u, divU, vorti, Ml, helem,
-        ducros_min_val, mu_e);
+    k_ducros_node<<<(npoin + 255) / 256, 256, 0, ctx->stream>>>(npoin, u, divU, vorti, Ml, nq);
+    k_ducros<<<tpe_grid(nelem), 256, 0, ctx->stream>>>(nelem, nnode_of(porder), porder, npoin,
+                                                       connec, m->perm, nq, helem, ducros_min_val,
+                                                       mu_e);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "element.hpp"
 
 namespace sodhip {
@@ -441,9 +442,14 @@ static void *noalloc_ptr(int device)
 }
 
 // ---------------------------------------------------------------- CFL time step
-// adapt_dt_cfl (mod_time_ops.f90:11-63): per element, then a min over blocks.
+// adapt_dt_cfl (mod_time_ops.f90:11-63): one thread per element over the
+// reference's element-fastest connec and mu_e (for a node position the lanes
+// of a wave read consecutive entries), the nodes visited an i-line at a time
+// (build_alex) with all loads of a line issued before its arithmetic (fmax is
+// exact, so the order does not change the result); then the min over blocks.
 __global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder,
-                                                const int *__restrict__ connecT,
+                                                const int *__restrict__ connec,
+                                                const int *__restrict__ perm,
                                                 const double *__restrict__ helem,
                                                 const double *__restrict__ u,
                                                 const double *__restrict__ csound, int npoin,
@@ -455,26 +461,25 @@ __global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder
                                                 double *__restrict__ part)
 {
     __shared__ double sh[256];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int alex[125];
+    build_alex(perm, nnode, alex);
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
+    const int KA = porder + 1; // one i-line per group
     double dtl = 1.0e14;
     if (e < nelem) {
-        // nodes KA at a time, all loads of a group issued before its
-        // arithmetic (fmax is exact, so the grouping does not change the
-        // result); a group past nnode repeats the last node
-        constexpr int KA = 4;
         double L3 = 0.0, maxmu = 0.0;
-        for (int a0 = 0; a0 < nnode; a0 += KA) {
-            int n[KA];
-            size_t ea[KA];
+        for (int l0 = 0; l0 < nnode; l0 += KA) {
+            int n[5];
+            size_t ea[5];
 #pragma unroll
-            for (int j = 0; j < KA; ++j) {
-                const int a = min(a0 + j, nnode - 1);
-                n[j] = connecT[(size_t)e * nnode + a];
+            for (int j = 0; j < 5; ++j) {
+                const int a = alex[min(l0 + min(j, KA - 1), nnode - 1)];
                 ea[j] = e + (size_t)nelem * a;
+                n[j] = connec[ea[j]] - 1;
             }
-            double v0[KA], v1[KA], v2[KA], cs[KA], mf[KA], me[KA], sg[KA], rh[KA];
+            double v0[5], v1[5], v2[5], cs[5], mf[5], me[5], sg[5], rh[5];
 #pragma unroll
-            for (int j = 0; j < KA; ++j) {
+            for (int j = 0; j < 5; ++j) {
                 v0[j] = u[n[j]];
                 v1[j] = u[n[j] + npoin];
                 v2[j] = u[n[j] + 2 * (size_t)npoin];
@@ -487,7 +492,7 @@ __global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder
                 }
             }
 #pragma unroll
-            for (int j = 0; j < KA; ++j) {
+            for (int j = 0; j < 5; ++j) {
                 double um = fabs(v0[j]);
                 um = fmax(um, fabs(v1[j]));
                 um = fmax(um, fabs(v2[j]));
@@ -510,50 +515,37 @@ __global__ __launch_bounds__(256) void k_dt_cfl(int nelem, int nnode, int porder
 
 // ---------------------------------------------------------------- diagnostics
 // volAvg_EK (mod_analysis.f90:10-44): per element sum_g gpvol 1/2 rho |u|^2 in
-// FP64 (the shape functions are the identity at the GLL points), then fixed-order
-// block sums; the host adds the block partials in order.
-__global__ __launch_bounds__(256) void k_ek(int nelem, int nnode, const int *__restrict__ connecT,
-                                            const double *__restrict__ gpvol,
-                                            const double *__restrict__ rho,
-                                            const double *__restrict__ u, int npoin,
-                                            double *__restrict__ part)
+// FP64 (the shape functions are the identity at the GLL points), one lane per
+// node (gpvol and connecT rows read whole), the element's sum by a fixed tree
+// into elK; then fixed-order block sums.
+template <int P>
+__global__ __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) void k_ek(
+    int nelem, int npoin, const int *__restrict__ connecT, const double *__restrict__ gpvol,
+    const double *__restrict__ rho, const double *__restrict__ u, double *__restrict__ elK)
 {
-    __shared__ double sh[256];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    double r1 = 0.0;
-    if (e < nelem) {
-        // nodes KA at a time with all loads of a group issued first; the sum
-        // keeps the node order of the plain loop
-        constexpr int KA = 4;
-        for (int a0 = 0; a0 < nnode; a0 += KA) {
-            int n[KA];
-            double w[KA], r[KA], u0[KA], u1[KA], u2[KA];
-#pragma unroll
-            for (int j = 0; j < KA; ++j) {
-                const size_t ea = (size_t)e * nnode + min(a0 + j, nnode - 1);
-                n[j] = connecT[ea];
-                w[j] = gpvol[ea];
-            }
-#pragma unroll
-            for (int j = 0; j < KA; ++j) {
-                r[j] = rho[n[j]];
-                u0[j] = u[n[j]];
-                u1[j] = u[n[j] + npoin];
-                u2[j] = u[n[j] + 2 * (size_t)npoin];
-            }
-#pragma unroll
-            for (int j = 0; j < KA; ++j)
-                if (a0 + j < nnode)
-                    r1 += w[j] * 0.5 * r[j] * (u0[j] * u0[j] + u1[j] * u1[j] + u2[j] * u2[j]);
+    constexpr int NN = (P + 1) * (P + 1) * (P + 1), NT = (NN + 63) / 64 * 64;
+    __shared__ double rs[NT];
+    const int lane = threadIdx.x;
+    const bool act = lane < NN;
+    const XcdElems xe = xcd_elems(nelem);
+    for (int e = xe.first; e < xe.end; e += xe.stride) {
+        double v = 0.0;
+        if (act) {
+            const size_t ea = (size_t)e * NN + lane;
+            const int n = connecT[ea];
+            const double w = gpvol[ea], r = rho[n];
+            const double u0 = u[n], u1 = u[n + npoin], u2 = u[n + 2 * (size_t)npoin];
+            v = w * 0.5 * r * (u0 * u0 + u1 * u1 + u2 * u2);
         }
-    }
-    sh[threadIdx.x] = r1;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        rs[lane] = v;
+        __syncthreads();
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (lane < s) rs[lane] += rs[lane + s];
+            __syncthreads();
+        }
+        if (lane == 0) elK[e] = rs[0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 // visc_dissipationRate (mod_analysis.f90:46-144): per quadrature point the
@@ -576,7 +568,8 @@ __global__ __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) void 
     const int a = act ? lane : 0;
     const int own[3] = {min(max(gI[a] - 1, 0), P), min(max(gJ[a] - 1, 0), P),
                         min(max(gK[a] - 1, 0), P)};
-    for (int e = blockIdx.x; e < nelem; e += gridDim.x) {
+    const XcdElems xe = xcd_elems(nelem);
+    for (int e = xe.first; e < xe.end; e += xe.stride) {
         const size_t ea = (size_t)e * NN + a;
         const int n = act ? connecT[ea] : 0;
         if (act)
@@ -838,12 +831,13 @@ extern "C" int sodhip_adapt_dt_cfl_f64(sodhip_ctx ctx, int porder, int nelem, in
     MeshCache *m = nullptr;
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
-    const int nb = (nelem + 255) / 256;
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
+    const int nb = tpe_grid(nelem);
     if ((rc = ensure_buf(ctx->diag, (size_t)nb * sizeof(double)))) return rc;
     double *part = (double *)ctx->diag.ptr;
     int tok = prof_begin(ctx, 3);
-    k_dt_cfl<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), porder, m->connecT, helem, u,
-                                          csound, npoin, cfl_conv, cfl_diff, mu_fluid, mu_sgs,
+    k_dt_cfl<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), porder, connec, m->perm, helem,
+                                          u, csound, npoin, cfl_conv, cfl_diff, mu_fluid, mu_sgs,
                                           mu_e, rho, part);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
@@ -864,11 +858,17 @@ extern "C" int sodhip_volavg_ek_f64(sodhip_ctx ctx, int porder, int nelem, int n
     MeshCache *m = nullptr;
     int rc = get_mesh(ctx, porder, nelem, npoin, connec, &m);
     if (rc) return rc;
+    SODHIP_REQUIRE(porder >= 2 && porder <= kMaxP, "porder %d not supported (2..4)", porder);
     const int nb = (nelem + 255) / 256;
-    if ((rc = ensure_buf(ctx->diag, (size_t)nb * sizeof(double)))) return rc;
-    double *part = (double *)ctx->diag.ptr;
-    k_ek<<<nb, 256, 0, ctx->stream>>>(nelem, nnode_of(porder), m->connecT, gpvol, rho, u, npoin,
-                                      part);
+    if ((rc = ensure_buf(ctx->diag, ((size_t)nelem + nb) * sizeof(double)))) return rc;
+    double *elK = (double *)ctx->diag.ptr, *part = elK + nelem;
+    const int grid = diag_grid(ctx, nelem);
+    switch (porder) {
+    case 2: k_ek<2><<<grid, 64, 0, ctx->stream>>>(nelem, npoin, m->connecT, gpvol, rho, u, elK); break;
+    case 3: k_ek<3><<<grid, 64, 0, ctx->stream>>>(nelem, npoin, m->connecT, gpvol, rho, u, elK); break;
+    case 4: k_ek<4><<<grid, 128, 0, ctx->stream>>>(nelem, npoin, m->connecT, gpvol, rho, u, elK); break;
+    }
+    k_sum_blocks<<<nb, 256, 0, ctx->stream>>>(nelem, elK, part);
     SODHIP_TRY_HIP(hipGetLastError());
     std::vector<double> h;
     if ((rc = fetch_parts(ctx, part, nb, h))) return rc;
@@ -899,7 +899,7 @@ extern "C" int sodhip_visc_dissipation_f64(sodhip_ctx ctx, int porder, int nelem
     const int nb = (nelem + 255) / 256;
     if ((rc = ensure_buf(ctx->diag, (2 * (size_t)nelem + 2 * nb) * sizeof(double)))) return rc;
     double *elS = (double *)ctx->diag.ptr, *elD = elS + nelem, *pS = elD + nelem, *pD = pS + nb;
-    const int grid = std::min(nelem, ctx->num_cu * 16);
+    const int grid = diag_grid(ctx, nelem, 16); // 114 VGPRs: 4 waves per SIMD
     switch (porder) {
     case 2:
         k_dissipation<2><<<grid, 64, 0, ctx->stream>>>(nelem, npoin, m->connecT, He, gpvol,

@@ -46,6 +46,12 @@ int ensure_buf(DevBuf &b, size_t bytes)
     return SODHIP_OK;
 }
 
+int diag_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu)
+{
+    const int g = std::min((nelem + 7) / 8 * 8, ctx->num_cu * blocks_per_cu);
+    return std::max(8, g / 8 * 8);
+}
+
 int elem_grid(sodhip_ctx ctx, int nelem, int blocks_per_cu)
 {
     // a multiple of 8 so that every XCD receives the same number of blocks
