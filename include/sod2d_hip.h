@@ -60,6 +60,14 @@ int sodhip_last_error(char *buf, int len);
 int sodhip_basis_tables(int porder, int *a2ijk, int *invAtoIJK, int *gmshAtoI, int *gmshAtoJ,
                         int *gmshAtoK, double *xgp, double *wgp, double *dNgp, double *dlxigp_ip);
 
+/* The /order tables of a SOD2D mesh file of order porder, as the partitioner
+ * writes them: a2ijk = gmsh2ijk and vtk2ijk (nnode), a2ij = gmsh2ij and vtk2ij
+ * (npbou = (p+1)^2) (set_allocate_hexahedronHO_ijk_indices and
+ * set_allocate_quadrilateralHO_ij_indices, mod_ijk_indices.f90:61-154;
+ * mod_meshConversorTool.f90:161-162).  Any output pointer may be NULL. */
+int sodhip_order_tables(int porder, int *a2ijk, int *gmsh2ijk, int *vtk2ijk, int *a2ij,
+                        int *gmsh2ij, int *vtk2ij);
+
 /* Structured box of nx*ny*nz HEX((p+1)^3) elements on [0,L]^3 in SOD2D's partition
  * format (mod_mpi_mesh.f90:15-48): npoin nodes including periodic slaves,
  * connecParOrig/connecParWork (nelem,nnode) 1-based, workingNodesPar (npoin_w),

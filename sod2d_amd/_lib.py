@@ -52,6 +52,7 @@ _SIGS = {
     "sodhip_version": (I, []),
     "sodhip_last_error": (I, [C.c_char_p, I]),
     "sodhip_basis_tables": (I, [I, P, P, P, P, P, P, P, P, P]),
+    "sodhip_order_tables": (I, [I] + [P] * 6),
     "sodhip_box_mesh_sizes": (I, [I, I, I, I, I, P, P, P]),
     "sodhip_box_mesh": (I, [I, I, I, I, I, P, D, C.c_ulonglong, P, P, P, P]),
     "sodhip_elem_jacobian": (I, [I, I, I, P, P, P, P]),

@@ -57,6 +57,30 @@ def build_lib(jobs: int = 8) -> str:
     return LIB
 
 
+H5LIB = os.path.join(HERE, "libsod2d_h5.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+
+
+def build_h5() -> str | None:
+    """The HDF5 mesh/restart I/O library (host code only, sod2d_amd/libsod2d_h5.so),
+    linked against the HDF5 C library under HDF5_DIR.  Returns None when that
+    library's header is absent."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 not found under {HDF5_DIR}: libsod2d_h5.so not built", file=sys.stderr)
+        return None
+    cxx = shutil.which("g++") or "g++"
+    tmp = H5LIB + ".tmp"
+    cmd = [cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{inc}",
+           os.path.join(CSRC, "h5io.cpp"), f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}",
+           "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5LIB)
+    return H5LIB
+
+
 def build_fortran() -> str | None:
     """Compile the ISO_C_BINDING shim module and the Fortran host driver."""
     flang = shutil.which("flang") or os.path.join(ROCM, "llvm", "bin", "flang")
@@ -76,6 +100,7 @@ def main(argv=None):
     ap.add_argument("--no-fortran", action="store_true")
     a = ap.parse_args(argv)
     print(build_lib(a.jobs))
+    print(build_h5())
     if not a.no_fortran:
         build_fortran()
 

@@ -266,6 +266,85 @@ extern "C" int sodhip_basis_tables(int porder, int *a2ijk, int *invAtoIJK, int *
     return SODHIP_OK;
 }
 
+namespace {
+
+// VTK Lagrange point number (1-based) of the node at position (i,j,k) of a
+// hexahedron of order p, in the convention of the reference's
+// vtkHigherOrderHexahedron_pointIndexFromIJK (mod_ijk_indices.f90:360-494):
+// the 8 vertices, the edges along i, then along j (4 each, edge nodes by
+// increasing parameter), the 4 edges along k (in vertex order), the faces
+// normal to i, j, k (lower face first, in-face lexicographic) and the body.
+int corner_no(int i, int j) { return i ? (j ? 2 : 1) : (j ? 3 : 0); }
+
+int vtk_hex_no(int p, int i, int j, int k)
+{
+    const int ib = i == 0 || i == p, jb = j == 0 || j == p, kb = k == 0 || k == p;
+    const int m = p - 1;
+    if (ib + jb + kb == 3) return 1 + corner_no(i, j) + (k ? 4 : 0);
+    int n = 9;
+    if (ib + jb + kb == 2) {
+        if (!ib) return n + (i - 1) + (j ? 2 * m : 0) + (k ? 4 * m : 0);
+        if (!jb) return n + (j - 1) + (i ? m : 3 * m) + (k ? 4 * m : 0);
+        return n + 8 * m + (k - 1) + m * corner_no(i, j);
+    }
+    n += 12 * m;
+    if (ib + jb + kb == 1) {
+        if (ib) return n + (j - 1) + m * (k - 1) + (i ? m * m : 0);
+        n += 2 * m * m;
+        if (jb) return n + (i - 1) + m * (k - 1) + (j ? m * m : 0);
+        n += 2 * m * m;
+        return n + (i - 1) + m * (j - 1) + (k ? m * m : 0);
+    }
+    n += 6 * m * m;
+    return n + (i - 1) + m * ((j - 1) + m * (k - 1));
+}
+
+// the same for a quadrilateral (vtkHigherOrderQuadrilateral_pointIndexFromIJ,
+// mod_ijk_indices.f90:496-570)
+int vtk_quad_no(int p, int i, int j)
+{
+    const int ib = i == 0 || i == p, jb = j == 0 || j == p, m = p - 1;
+    if (ib + jb == 2) return 1 + corner_no(i, j);
+    if (ib + jb == 1) return 5 + (!ib ? (i - 1) + (j ? 2 * m : 0) : (j - 1) + (i ? m : 3 * m));
+    return 5 + 4 * m + (i - 1) + m * (j - 1);
+}
+
+} // namespace
+
+extern "C" int sodhip_order_tables(int porder, int *a2ijk, int *gmsh2ijk, int *vtk2ijk, int *a2ij,
+                                   int *gmsh2ij, int *vtk2ij)
+{
+    Tables t;
+    int rc = make_tables(porder, t);
+    if (rc) return rc;
+    const int p = porder, n1 = p + 1;
+    // hexahedra: positions visited k, i, j in sod2d's 1-D order
+    // (set_allocate_hexahedronHO_ijk_indices, mod_ijk_indices.f90:61-114)
+    int c = 0;
+    for (int k = 0; k < n1; ++k)
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n1; ++j, ++c) {
+                if (a2ijk) a2ijk[c] = t.a2ijk[c];
+                if (gmsh2ijk) gmsh2ijk[c] = t.a2ijk[c];
+                if (vtk2ijk) vtk2ijk[c] = vtk_hex_no(p, s2g(p, i), s2g(p, j), s2g(p, k));
+            }
+    // quadrilaterals: positions visited i, j (set_allocate_quadrilateralHO_ij_indices,
+    // :116-154); a2ij = gmsh2ij as the partitioner sets it (mod_meshConversorTool.f90:161-162)
+    std::vector<std::array<int, 2>> q;
+    gmsh_quad_order(p, q);
+    std::vector<int> qlocal(n1 * n1, -1);
+    for (size_t a = 0; a < q.size(); ++a) qlocal[q[a][0] + n1 * q[a][1]] = (int)a;
+    c = 0;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j, ++c) {
+            const int g = qlocal[s2g(p, i) + n1 * s2g(p, j)] + 1;
+            if (a2ij) a2ij[c] = g;
+            if (gmsh2ij) gmsh2ij[c] = g;
+            if (vtk2ij) vtk2ij[c] = vtk_quad_no(p, s2g(p, i), s2g(p, j));
+        }
+    return SODHIP_OK;
+}
+
 extern "C" int sodhip_box_mesh_sizes(int porder, int nx, int ny, int nz, int periodic, int *nelem,
                                      int *npoin, int *npoin_w)
 {
