@@ -461,7 +461,7 @@ def main(argv=None):
     for _ in range(prof_steps):
         run.step()
     torch.cuda.synchronize()
-    prof = {k: run.ctx.prof_get(k) for k in range(4)}
+    prof = {k: run.ctx.prof_get(k) for k in range(5)}
     run.ctx.prof_enable(False)
     diag = run.diagnostics()
     # the same run with the adaptive time step (CFL dt reduced and read back
@@ -494,8 +494,9 @@ def main(argv=None):
     k_ms, k_n = prof[0]
     g_ms, g_n = prof[1]
     if a.lps:
-        # two element kernels (convection, stabilised viscous operator) and one
-        # gather/update per stage; full_proj_ijk counts as other
+        # one element kernel (convection and the stabilised viscous operator,
+        # two waves per element at p=3) and one gather/update per stage;
+        # full_proj_ijk (its element kernel and projection gather) is timed apart
         t_elem = k_ms / (prof_steps * nrhs)
         t_gath = g_ms / (prof_steps * nrhs)
     else:
@@ -503,11 +504,15 @@ def main(argv=None):
         t_gath = g_ms / max(g_n, 1)
     t_stage = t_elem + t_gath
     B = stage_bytes(N, EN, 8 if a.precision == 8 else 4)
+    if a.lps:
+        # the LPS stage also reads each node's projection record (96 bytes) and
+        # each element's tau_stab
+        B += 96 * N + 8 * run.nelem
     achieved = B / (t_stage * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_rk_stage_tgv{a.n}_p{a.porder}_"
                       f"{'fp64' if a.precision == 8 else 'fp32op'}.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and not a.lps:
         try:
             traffic = json.load(open(tf)).get("bytes_per_stage")
         except Exception:
@@ -547,20 +552,21 @@ def main(argv=None):
                     == (16, 1, 256)), 5)})),
         },
         "roofline": {
-            "kernel": ("LPS stage (sodhip_rk_stage_lps_f64): element kernels (convection, "
-                       "stabilised viscous operator) + the gather/update, per stage; "
-                       "full_proj_ijk excluded" if a.lps else
-                       "fused RK stage: k_rhs (element) + k_gather_update (node)"),
+            "kernel": ("LPS stage (sodhip_rk_stage_lps_f64): k_rhs_split in LPS mode "
+                       "(convection and full_diff_stab_ijk) + k_gather_update, per stage; "
+                       "full_proj_ijk (ms_projection) excluded" if a.lps else
+                       "fused RK stage: k_rhs_split (element) + k_gather_update (node)"),
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_stage": B,
             "ms_element_kernel": round(t_elem, 5), "ms_gather_update": round(t_gath, 5),
             "stage_only_gdofs": round(5 * N / (t_stage * 1e-3) / 1e9, 4),
-        },
+        } | ({"ms_projection": round(prof[4][0] / (prof_steps * nrhs), 5)} if a.lps else {}),
         "per_step_ms": {"element_kernels": round(k_ms / prof_steps, 4),
                         "gather_update": round(g_ms / prof_steps, 4),
                         "sensor": round(prof[2][0] / prof_steps, 4),
-                        "other": round(prof[3][0] / prof_steps, 4)},
+                        "other": round(prof[3][0] / prof_steps, 4)}
+        | ({"projection": round(prof[4][0] / prof_steps, 4)} if a.lps else {}),
         "diagnostics": {k: diag[k] for k in ("time", "EK", "eps_T", "maxmach")},
         "adaptive_dt": adaptive,
     }

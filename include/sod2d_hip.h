@@ -164,7 +164,8 @@ int sodhip_copy_present(sodhip_ctx ctx, const void *src_host, void *dst_host, si
  * launch of kernel class `kind` is bracketed by events; sodhip_prof_get returns
  * the summed duration (ms) and the number of launches since the last reset.
  * kind: 0 element operator kernel, 1 node gather/update kernel, 2 sensor,
- * 3 other nodal kernels. */
+ * 3 other nodal kernels, 4 the LPS projections (full_proj_ijk's element
+ * kernel and gather). */
 /* Launch shape of the fused RK stage on this context: blocks per CU of the
  * element kernel (1..64; with a grid-stride walk this sets the elements each
  * workgroup takes), whether the geometry moves through LDS in 16-byte lane
@@ -709,7 +710,9 @@ int sodhip_rk_stage_noalloc_f64(sodhip_ctx ctx, int porder, int nelem, int npoin
  * stage combine y <- lam1*y1 + lams*y + gam*dt*R, as sodhip_rk_stage_f64 does
  * for the plain path (FP64).  u, e_int, pr, aux_h, Tem receive the EOS of the
  * current state (working nodes), mu_fluid Sutherland's viscosity (all nodes),
- * the Proj arrays the projections; tau_stab is comp_tau's.  mu_sgs is zero.
+ * the Proj arrays (all or none; NULL keeps the projections inside the call,
+ * where the stabilised operator reads each node's 15 values as one 128-byte
+ * record) the projections; tau_stab is comp_tau's.  mu_sgs is zero.
  * The convection and the stabilised viscous terms are summed per element
  * before the node-ordered gather.  Rmass/Rmom/Rener (all or none) receive
  * R*invMl; rho_next/q_next/E_next (all or none) the next step's first stage

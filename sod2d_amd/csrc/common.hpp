@@ -33,6 +33,11 @@ int set_error(int code, const char *fmt, ...);
 // ---------------------------------------------------------------- element sizes
 constexpr int kMaxP = 4; // device kernels are instantiated for porder 2..4
 
+// values per node projection record of the fused LPS stage: the upper triangle
+// t11 t12 t13 t22 t23 t33 of the projected tensor 2S - 2/3 div u I, then
+// ProjMass (grad rho) and ProjEner (grad T); 96 bytes
+constexpr int kProjRec = 12;
+
 // size of the allocation-free stage's record scratch, per GPU
 #ifndef SODHIP_NOALLOC_SCRATCH_MIB
 #define SODHIP_NOALLOC_SCRATCH_MIB 64
@@ -112,7 +117,7 @@ struct ProfSlot {
     long long count = 0;
 };
 
-constexpr int kProfKinds = 4;
+constexpr int kProfKinds = 5;
 
 } // namespace sodhip
 
@@ -200,6 +205,16 @@ int gather_update_on(sodhip_ctx ctx, int npoin, int nw, const int *list, const i
                      double dt, double lam1, double lams, double gam, const double *rho1,
                      const double *q1, const double *E1, double *rho, double *q, double *E,
                      double *Rmass, double *Rmom, double *Rener, const NextStage *next);
+// convection and the LPS-stabilised viscous operator (full_diff_stab_ijk) of
+// the fused LPS stage at p=3, FP64, from the conserved state, with the node
+// projection records prj (kProjRec per node) and comp_tau's tau; records of
+// five per element-node into out (ops_element.hip)
+int rhs_lps_elements(sodhip_ctx ctx, const MeshCache *m, const double *He, const double *gpvol,
+                     const double *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                     const double *rho, const double *q, const double *E, const double *mu_e,
+                     const double *prj, const double *tau, double gamma, double Rgas, double Cp,
+                     double Pr, double mu_factor, double c_rho, double c_ener, double c_lps,
+                     double *out);
 // the fused p=3 FP64 stage's element kernel in cluster mode: interior nodes
 // updated in place, surface partial records into rec (ops_element.hip)
 int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *cl,

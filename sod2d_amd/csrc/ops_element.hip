@@ -120,6 +120,10 @@ template <typename T> struct RhsArgs {
     double *Rm, *Rq, *Re;
     double nx_l1, nx_gdt;
     double *nxr, *nxq, *nxE;
+    // LPS stage (the viscous wave evaluates full_diff_stab_ijk): node
+    // projection records (kProjRec values), comp_tau's tau_stab, c_lps
+    const double *prj, *tau;
+    double c_lps;
 };
 
 // Sutherland's law (mod_fluid_viscosity.f90:21-41)
@@ -893,7 +897,8 @@ __global__ __launch_bounds__(256) void k_gather_entropy(int npoin, const int *__
 // 20 % larger records cost the element kernel more than the gather gains.
 template <typename T> constexpr int split_rw() { return sizeof(T) == 4 ? 6 : 5; }
 
-template <int P, typename T, bool G16, int RUNV = SODHIP_SPLIT_RUN> struct SplitLDS {
+template <int P, typename T, bool G16, int RUNV = SODHIP_SPLIT_RUN, bool LPSV = false>
+struct SplitLDS {
     static constexpr int NN = (P + 1) * (P + 1) * (P + 1);
     static constexpr int AH = sizeof(T) == 8 ? SODHIP_SPLIT_AHEAD64 : SODHIP_SPLIT_AHEAD32;
     // geometry, He then gpvol, copied in 64-lane rows of GW bytes per lane
@@ -909,7 +914,13 @@ template <int P, typename T, bool G16, int RUNV = SODHIP_SPLIT_RUN> struct Split
     static constexpr int RUN = RUNV;
     static constexpr bool MUBLK = sizeof(T) == 8;
     static constexpr int ST = 5 * NN * 8, MU = MUBLK ? 0 : NN * (int)sizeof(T);
-    static constexpr int SLOT = GEO + ST + MU;
+    // LPS: the projection records of the element's nodes (kProjRec doubles
+    // per node, lexicographic order: 6 rows of 64 16-byte pieces) and its
+    // tau_stab (the first 8 bytes of one 4-byte-per-lane row)
+    static constexpr int PJ_AT = GEO + ST + MU, PJ = LPSV ? NN * kProjRec * 8 : 0;
+    static constexpr int PJ_ROWS = PJ / 1024, TAU_AT = PJ_AT + PJ, TAU = LPSV ? 256 : 0;
+    static_assert(!LPSV || (NN == 64 && PJ == PJ_ROWS * 1024), "LPS records: whole rows");
+    static constexpr int SLOT = GEO + ST + MU + PJ + TAU;
     // connectivity rows: AH + 1 in use at a time
     static constexpr int CROWS = AH + 1, CROW = NN * 4;
     static constexpr int MUB = RUN * NN * (int)sizeof(T), MUBS = MUBLK ? 3 : 0;
@@ -920,7 +931,9 @@ template <int P, typename T, bool G16, int RUNV = SODHIP_SPLIT_RUN> struct Split
                   "one run's mu_e block in MU_OPS rows per element");
     // copies per element: wave 0 takes the connectivity row, the even
     // geometry rows and rho, q1, q2; wave 1 the odd rows, q3, E and mu_e
-    static constexpr int NOPS0 = 1 + (GROWS + 1) / 2 + 6, NOPS1 = GROWS / 2 + 4 + MU_OPS;
+    // (with LPS the viscous wave also copies the projection rows and tau)
+    static constexpr int NOPS0 = 1 + (GROWS + 1) / 2 + 6,
+                         NOPS1 = GROWS / 2 + 4 + MU_OPS + (LPSV ? PJ_ROWS + 1 : 0);
     static_assert(SLOT % 16 == 0, "ring slots must stay 16-byte aligned");
     static_assert(AH >= 2 && AH * NOPS0 <= 63 && AH * NOPS1 <= 63, "vmcnt range");
 };
@@ -1026,6 +1039,24 @@ __device__ __forceinline__ void mu_issue2(const RhsArgs<T> &A, unsigned blk, int
     }
 }
 
+// LPS copies of element e into the ring slot at LDS address slot (viscous
+// wave): row r, lane L moves 16-byte piece (64r+L)%6 of the projection record
+// of the node at lexicographic position (64r+L)/6, whose element-local index
+// ik[r] it holds (c: the element's connectivity row); then tau_stab(e) as two
+// 4-byte halves (lanes 0 and 1; the other lanes repeat them)
+template <typename T, int PJ_ROWS, int PJ_AT, int TAU_AT>
+__device__ __forceinline__ void lps_issue(const RhsArgs<T> &A, unsigned slot, const int *c,
+                                          const int (&ik)[PJ_ROWS], int e, int lane)
+{
+#pragma unroll
+    for (int r = 0; r < PJ_ROWS; ++r) {
+        const int piece = (64 * r + lane) % 6;
+        const int n = c[ik[r]];
+        dma_lds<16>(A.prj + kProjRec * (size_t)n + 2 * piece, slot + PJ_AT + r * 1024);
+    }
+    dma_lds<4>((const char *)(A.tau + e) + 4 * (lane & 1), slot + TAU_AT);
+}
+
 // reads ring slot sl into registers: He, gpvol, the conserved fields and mu_e
 // of node a (lexicographic position pa)
 template <int P, typename T, bool G16>
@@ -1103,11 +1134,17 @@ __device__ __forceinline__ void split_store(const RhsArgs<T> &A, const void *den
 // a cluster's last element finishes its interior nodes (mass inverse and
 // stage combine, as k_gather_update) and stores one partial record per
 // surface node.  mu_e runs are then pairs of elements.
-template <int P, typename T, bool G16, bool CL = false>
+// LPS: the viscous wave evaluates full_diff_stab_ijk (elem_stab.f90:207-438)
+// instead of full_diffusion_ijk: the stabilising terms -c_lps tau_e (Proj -
+// grad) enter the fluxes, nu_e and kappa_e multiply the gradients at each
+// point before the weak divergence, and the projections (node records) and
+// tau_e come through the ring with the element's other inputs.
+template <int P, typename T, bool G16, bool CL = false, bool LPS = false>
 __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_SPLIT_WAVES_F32) void k_rhs_split(RhsArgs<T> A)
 {
-    using L = SplitLDS<P, T, G16, CL ? 2 : SODHIP_SPLIT_RUN>;
+    using L = SplitLDS<P, T, G16, CL ? 2 : SODHIP_SPLIT_RUN, LPS>;
     static_assert(!CL || (sizeof(T) == 8 && P == 3), "cluster assembly: FP64, p=3");
+    static_assert(!LPS || (sizeof(T) == 8 && P == 3 && !CL), "LPS: FP64, p=3, element order");
     using V2 = T __attribute__((ext_vector_type(2)));
     // node records of REC values: 112 bytes in FP64, an odd multiple of 16,
     // so that 16 lanes reading 16 consecutive records with 16-byte reads hit
@@ -1215,6 +1252,9 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
     // lexicographic positions nd and 32 + nd.
     auto lex = [&](int r) { return min(max(A.perm[r], 0), NN - 1); };
     int a, ia0, ia1;
+    // LPS: element-local index of the node whose projection piece lane
+    // copies in row r (lexicographic position (64r + lane)/6)
+    int ik[LPS ? L::PJ_ROWS : 1];
     {
         int *inv = (int *)s1;
         if (role == 0) inv[lex(lane)] = lane;
@@ -1222,8 +1262,17 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
         a = min(max(inv[lane], 0), NN - 1);
         ia0 = min(max(inv[nd], 0), NN - 1);
         ia1 = min(max(inv[32 + nd], 0), NN - 1);
+        if constexpr (LPS)
+#pragma unroll
+            for (int r = 0; r < L::PJ_ROWS; ++r)
+                ik[r] = min(max(inv[(64 * r + lane) / 6], 0), NN - 1);
         lds_barrier();
     }
+    auto lps_at = [&](int t, const int *c) {
+        if constexpr (LPS)
+            lps_issue<T, L::PJ_ROWS, L::PJ_AT, L::TAU_AT>(A, ring_lds + slot_of(t), c, ik,
+                                                           elem_of(t), lane);
+    };
     const int pa = lane;
     LineTab<P, T> lt;
     build_linetab<P, T>(a, A.dl, A.inv, A.gI, A.gJ, A.gK, lt);
@@ -1252,6 +1301,7 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                                       elem_of(t + AH), n0, n1, lane);
             mu_at(t - AH);
             cloc_issue(t);
+            lps_at(t, c);
         } else
             split_issue<P, T, G16, 0>(A, ring_lds + slot_of(t), ring_lds + crow_of(t + AH), elem_of(t),
                                       elem_of(t + AH), n0, n1, lane);
@@ -1530,6 +1580,15 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             T He[3][3], vol;
             split_read<P, T, G16>(ring + slot_of(t), a, pa, true, He, vol, cur);
             if constexpr (L::MUBLK) cur.mue = mu_read(t, a);
+            // LPS: the node's projection record and tau_e, read before B2 (the
+            // copies of element t + AH then start into this slot)
+            double pj[LPS ? kProjRec : 1], taus = 0.0;
+            if constexpr (LPS) {
+                const double *pr = (const double *)(ring + slot_of(t) + L::PJ_AT) + kProjRec * pa;
+#pragma unroll
+                for (int j = 0; j < kProjRec; ++j) pj[j] = pr[j];
+                taus = *(const double *)(ring + slot_of(t) + L::TAU_AT);
+            }
             T rho, u[3], Tk, muf;
             const T mue = cur.mue;
             {
@@ -1554,6 +1613,7 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                                           elem_of(t + AH), elem_of(t + 2 * AH), c[ia0], c[ia1], lane);
                 mu_at(t);
                 cloc_issue(t + AH);
+                lps_at(t + AH, c);
             }
             // residual records of the previous element, completed in s2 by the
             // convection wave: their stores complete while this element runs
@@ -1593,7 +1653,65 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
                 gt[i] = He[i][0] * dT[0] + He[i][1] * dT[1] + He[i][2] * dT[2];
             }
             // contravariant fluxes G_k(F) = gpvol sum_d He(d,k) F_d into the flux record
-            {
+            if constexpr (LPS) {
+                // full_diff_stab_ijk's fluxes at the point (mu_sgs = 0), in
+                // the arithmetic of k_diff_stab; record per k: mass, energy,
+                // (unused), momentum 1..3
+                const double rhon = rho, mufl = muf, musgs = 0.0;
+                const double twoThirds = 2.0 / 3.0;
+                const double divU = gU[0][0] + gU[1][1] + gU[2][2];
+                const double nu_e = A.c_rho * mue / rhon;
+                const double mu_fgp = mufl + rhon * musgs;
+                const double kappa_e = mufl * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr +
+                                       A.Cp * rhon * musgs / 0.9;
+                const double cl = -A.c_lps * taus;
+                // projected tensor rows from its upper triangle t11 t12 t13 t22 t23 t33
+                const int at[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+                double F[5][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        double tt2 = gU[i][d] + gU[d][i];
+                        if (i == d) tt2 = tt2 - twoThirds * divU;
+                        F[1 + i][d] = cl * (pj[at[3 * i + d]] - tt2) * rhon;
+                    }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    F[4][d] = cl * (pj[9 + d] - gt[d]) * rhon * A.Cp / A.Pr;
+                    F[0][d] = cl * (pj[6 + d] - gR[d]);
+                }
+                double tauU[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        tauU[i] = tauU[i] + (mu_fgp + mue) * (gU[i][j] + gU[j][i]) * u[j];
+                        double tv = (mu_fgp + mue) * (gU[i][j] + gU[j][i]);
+                        if (i == j) tv = tv - mu_fgp * twoThirds * divU;
+                        F[1 + i][j] = F[1 + i][j] + tv;
+                    }
+                    tauU[i] = tauU[i] - mu_fgp * twoThirds * divU * u[i];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    F[4][d] = F[4][d] + kappa_e * gt[d] + tauU[d];
+                    F[0][d] = F[0][d] + nu_e * gR[d];
+                }
+                const int g = pa * GREC;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double Gf[5];
+#pragma unroll
+                    for (int f = 0; f < 5; ++f)
+                        Gf[f] = vol * (He[0][k] * F[f][0] + He[1][k] * F[f][1] + He[2][k] * F[f][2]);
+                    if (act) {
+                        s2[(g + 6 * k + 0) >> 1] = V2{Gf[0], Gf[4]};
+                        s2[(g + 6 * k + 2) >> 1] = V2{T(0), Gf[1]};
+                        s2[(g + 6 * k + 4) >> 1] = V2{Gf[2], Gf[3]};
+                    }
+                }
+            } else {
                 const T mut = muf + mue;
                 const T divU = gU[0][0] + gU[1][1] + gU[2][2];
                 const T tdiv = muf * T(2.0 / 3.0) * divU;
@@ -1656,9 +1774,16 @@ __global__ __launch_bounds__(128, sizeof(T) == 8 ? SODHIP_SPLIT_WAVES : SODHIP_S
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
             if (act) {
-                s2[3 * pa] = V2{nu_e * W6[0], W6[3]};
-                s2[3 * pa + 1] = V2{W6[4], W6[5]};
-                s2[3 * pa + 2] = V2{W6[2] + kappa * W6[1], T(0)};
+                if constexpr (LPS) {
+                    // nu_e and kappa_e are inside the fluxes
+                    s2[3 * pa] = V2{W6[0], W6[3]};
+                    s2[3 * pa + 1] = V2{W6[4], W6[5]};
+                    s2[3 * pa + 2] = V2{W6[1], T(0)};
+                } else {
+                    s2[3 * pa] = V2{nu_e * W6[0], W6[3]};
+                    s2[3 * pa + 1] = V2{W6[4], W6[5]};
+                    s2[3 * pa + 2] = V2{W6[2] + kappa * W6[1], T(0)};
+                }
             }
         }
         // the last element's records, completed by the convection wave
@@ -2037,6 +2162,55 @@ int rhs_cluster_stage(sodhip_ctx ctx, const MeshCache *m, const ClusterTables *c
         k_rhs_split<3, T, true, true><<<grid, 128, 0, ctx->stream>>>(A);
     else
         k_rhs_split<3, T, false, true><<<grid, 128, 0, ctx->stream>>>(A);
+    prof_end(ctx, tok);
+    SODHIP_TRY_HIP(hipGetLastError());
+    return SODHIP_OK;
+}
+
+int rhs_lps_elements(sodhip_ctx ctx, const MeshCache *m, const double *He, const double *gpvol,
+                     const double *dl, const int *inv, const int *gI, const int *gJ, const int *gK,
+                     const double *rho, const double *q, const double *E, const double *mu_e,
+                     const double *prj, const double *tau, double gamma, double Rgas, double Cp,
+                     double Pr, double mu_factor, double c_rho, double c_ener, double c_lps,
+                     double *out)
+{
+    using T = double;
+    RhsArgs<T> A{};
+    A.nelem = m->nelem;
+    A.npoin = m->npoin;
+    A.EN = (size_t)m->nelem * 64;
+    A.connecT = m->connecT;
+    A.perm = m->perm;
+    A.He = He;
+    A.gpvol = gpvol;
+    A.dl = dl;
+    A.inv = inv;
+    A.gI = gI;
+    A.gJ = gJ;
+    A.gK = gK;
+    A.cr = rho;
+    A.cq = q;
+    A.cE = E;
+    A.mu_e = mu_e;
+    A.gamma = gamma;
+    A.Rgas = Rgas;
+    A.mu_factor = mu_factor;
+    A.p_ref = 0.0;
+    A.Cp = Cp;
+    A.Pr = Pr;
+    A.c_rho = c_rho;
+    A.c_ener = c_ener;
+    A.prj = prj;
+    A.tau = tau;
+    A.c_lps = c_lps;
+    A.out = out;
+    A.geo16 = ctx->stage_geo16 && ((uintptr_t)He % 16 == 0) && ((uintptr_t)gpvol % 16 == 0);
+    const int grid = elem_grid(ctx, m->nelem, ctx->stage_bpc);
+    int tok = prof_begin(ctx, 0);
+    if (A.geo16)
+        k_rhs_split<3, T, true, false, true><<<grid, 128, 0, ctx->stream>>>(A);
+    else
+        k_rhs_split<3, T, false, false, true><<<grid, 128, 0, ctx->stream>>>(A);
     prof_end(ctx, tok);
     SODHIP_TRY_HIP(hipGetLastError());
     return SODHIP_OK;

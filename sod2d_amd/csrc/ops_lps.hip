@@ -15,6 +15,7 @@
 
 namespace sodhip {
 
+
 struct LpsArgs {
     int nelem, npoin;
     const int *connecT; // element-major, 0-based
@@ -23,6 +24,9 @@ struct LpsArgs {
     const int *inv, *gI, *gJ, *gK;
     const double *rho_n, *rho, *u, *Tem, *mu_fluid, *mu_e, *mu_sgs;
     const double *pMass, *pEner, *pMX, *pMY, *pMZ; // projections (npoin,3)
+    // or one record per node (kProjRec values: the upper triangle of the
+    // projected tensor, then ProjMass, then ProjEner)
+    const double *prj;
     const double *tau;                             // tau_stab(nelem)
     double Cp, Pr, c_rho, c_ener, c_lps;
     double *out;
@@ -158,7 +162,9 @@ __global__ __launch_bounds__(EG<P>::NT) void k_lps_proj(LpsArgs A)
 // sum_k sum_{b on the k-line} dl(b,k,I_k(a)) G_k(F)(b).  Records of 5 values
 // (mass, momentum 1..3, energy) per element-node; ACC adds them to the records
 // already there (the convection of the same stage).
-template <int P, bool ACC = false>
+// AOS (the fused stage): the projections come as node records (A.prj), 96
+// contiguous bytes per node, instead of 15 arrays.
+template <int P, bool ACC = false, bool AOS = false>
 __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
 {
     constexpr int NN = EG<P>::NN, NT = EG<P>::NT;
@@ -196,6 +202,22 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
         const double musgs = A.mu_sgs ? A.mu_sgs[eg] : 0.0;
         const double taus = A.tau[e];
         lds_barrier();
+        // the node's projections: MX MY MZ Mass Ener, three components each
+        double pj[15];
+        if constexpr (AOS) {
+            const double2 *r = (const double2 *)(A.prj + kProjRec * (size_t)n);
+            double v[12];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const double2 x = r[q];
+                v[2 * q] = x.x;
+                v[2 * q + 1] = x.y;
+            }
+            // rows of the symmetric tensor, ProjMass, ProjEner
+            const int at[15] = {0, 1, 2, 1, 3, 4, 2, 4, 5, 6, 7, 8, 9, 10, 11};
+#pragma unroll
+            for (int j = 0; j < 15; ++j) pj[j] = v[at[j]];
+        }
         double gR[3], gT[3], gU[3][3];
         lps_gradients<P>(lt, s, pa, hp, gR, gT, gU);
         const double divU = gU[0][0] + gU[1][1] + gU[2][2];
@@ -204,20 +226,26 @@ __global__ __launch_bounds__(EG<P>::NT) void k_diff_stab(LpsArgs A)
         const double kappa_e =
             mufl * A.Cp / A.Pr + A.c_ener * mue * A.Cp / A.Pr + A.Cp * rhon * musgs / 0.9;
         const double cl = -A.c_lps * taus;
+        if constexpr (!AOS) {
+            const double *pf[5] = {A.pMX, A.pMY, A.pMZ, A.pMass, A.pEner};
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) pj[3 * f + d] = pf[f][n + np * d];
+        }
         double F[5][3];
-        const double *pm[3] = {A.pMX, A.pMY, A.pMZ};
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 double t = gU[i][d] + gU[d][i];
                 if (i == d) t = t - twoThirds * divU;
-                F[1 + i][d] = cl * (pm[i][n + np * d] - t) * rhon;
+                F[1 + i][d] = cl * (pj[3 * i + d] - t) * rhon;
             }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            F[4][d] = cl * (A.pEner[n + np * d] - gT[d]) * rhon * A.Cp / A.Pr;
-            F[0][d] = cl * (A.pMass[n + np * d] - gR[d]);
+            F[4][d] = cl * (pj[12 + d] - gT[d]) * rhon * A.Cp / A.Pr;
+            F[0][d] = cl * (pj[9 + d] - gR[d]);
         }
         double tauU[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -360,7 +388,8 @@ __global__ __launch_bounds__(256) void k_gather5(int npoin, const int *__restric
 // invMl on the working nodes (lumped_solver_vect_opt, :576-581), and written
 // as ProjMX/MY/MZ (rows of the symmetric tensor), ProjMass, ProjEner.
 struct ProjOut {
-    double *MX, *MY, *MZ, *Mass, *Ener;
+    double *MX, *MY, *MZ, *Mass, *Ener; // the reference's arrays, or all NULL
+    double *prj;                        // one record of kProjRec values per node, or NULL
 };
 __global__ __launch_bounds__(256) void k_gather_proj(int npoin, const int *__restrict__ row,
                                                      const int *__restrict__ src,
@@ -401,22 +430,29 @@ __global__ __launch_bounds__(256) void k_gather_proj(int npoin, const int *__res
 #pragma unroll
         for (int f = 0; f < 12; ++f) acc[f] *= m;
     }
-    const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
-    out.MX[n] = acc[0];
-    out.MX[n1] = acc[1];
-    out.MX[n2] = acc[2];
-    out.MY[n] = acc[1];
-    out.MY[n1] = acc[3];
-    out.MY[n2] = acc[4];
-    out.MZ[n] = acc[2];
-    out.MZ[n1] = acc[4];
-    out.MZ[n2] = acc[5];
-    out.Mass[n] = acc[6];
-    out.Mass[n1] = acc[7];
-    out.Mass[n2] = acc[8];
-    out.Ener[n] = acc[9];
-    out.Ener[n1] = acc[10];
-    out.Ener[n2] = acc[11];
+    if (out.MX) {
+        const size_t n1 = n + (size_t)npoin, n2 = n + 2 * (size_t)npoin;
+        out.MX[n] = acc[0];
+        out.MX[n1] = acc[1];
+        out.MX[n2] = acc[2];
+        out.MY[n] = acc[1];
+        out.MY[n1] = acc[3];
+        out.MY[n2] = acc[4];
+        out.MZ[n] = acc[2];
+        out.MZ[n1] = acc[4];
+        out.MZ[n2] = acc[5];
+        out.Mass[n] = acc[6];
+        out.Mass[n1] = acc[7];
+        out.Mass[n2] = acc[8];
+        out.Ener[n] = acc[9];
+        out.Ener[n1] = acc[10];
+        out.Ener[n2] = acc[11];
+    }
+    if (out.prj) {
+        double2 *r = (double2 *)(out.prj + kProjRec * (size_t)n);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) r[q] = make_double2(acc[2 * q], acc[2 * q + 1]);
+    }
 }
 
 static int node_grid(int npoin) { return ((npoin + 255) / 256 + 7) / 8 * 8; }
@@ -436,7 +472,7 @@ static int lps_common(sodhip_ctx ctx, int porder, int nelem, int npoin, const in
 // full_proj_ijk into the caller's projections, records through out (12 x EN)
 static int proj_run(sodhip_ctx ctx, int porder, const MeshCache *m, LpsArgs A,
                     const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
-                    double *ProjMY, double *ProjMZ);
+                    double *ProjMY, double *ProjMZ, double *prj = nullptr);
 
 #define SODHIP_LPS_LAUNCH(KERNEL, GRID, ARGS)                                              \
     switch (porder) {                                                                      \
@@ -448,15 +484,15 @@ static int proj_run(sodhip_ctx ctx, int porder, const MeshCache *m, LpsArgs A,
 
 static int proj_run(sodhip_ctx ctx, int porder, const MeshCache *m, LpsArgs A,
                     const double *invMl, double *ProjMass, double *ProjEner, double *ProjMX,
-                    double *ProjMY, double *ProjMZ)
+                    double *ProjMY, double *ProjMZ, double *prj)
 {
     const int grid = elem_grid(ctx, m->nelem, 16);
-    int tok = prof_begin(ctx, 3);
+    int tok = prof_begin(ctx, 4);
     SODHIP_LPS_LAUNCH(k_lps_proj, grid, A)
     SODHIP_TRY_HIP(hipGetLastError());
     k_gather_proj<<<node_grid(m->npoin), 256, 0, ctx->stream>>>(
         m->npoin, m->row, m->src, A.out, m->wflag, invMl,
-        ProjOut{ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner});
+        ProjOut{ProjMX, ProjMY, ProjMZ, ProjMass, ProjEner, prj});
     SODHIP_TRY_HIP(hipGetLastError());
     prof_end(ctx, tok);
     return SODHIP_OK;
@@ -610,9 +646,11 @@ extern "C" int sodhip_rk_stage_lps_f64(
     if (rc) return rc;
     SODHIP_REQUIRE(lpoin_w && npoin_w > 0 && npoin_w <= npoin, "invalid working nodes");
     SODHIP_REQUIRE(mu_e && invMl && tau_stab && rho1 && q1 && E1 && rho && q && E && u &&
-                       e_int && pr && aux_h && Tem && mu_fluid && ProjMass && ProjEner &&
-                       ProjMX && ProjMY && ProjMZ,
+                       e_int && pr && aux_h && Tem && mu_fluid,
                    "null array argument");
+    SODHIP_REQUIRE(!!ProjMass == !!ProjEner && !!ProjEner == !!ProjMX && !!ProjMX == !!ProjMY &&
+                       !!ProjMY == !!ProjMZ,
+                   "ProjMass/ProjEner/ProjMX/ProjMY/ProjMZ: all or none");
     SODHIP_REQUIRE(!!Rmass == !!Rmom && !!Rmom == !!Rener, "Rmass/Rmom/Rener: all or none");
     SODHIP_REQUIRE(!!rho_next == !!q_next && !!q_next == !!E_next, "next stage: all or none");
     SODHIP_REQUIRE(!rho_next || (rho_next != rho && q_next != q && E_next != E),
@@ -626,11 +664,19 @@ extern "C" int sodhip_rk_stage_lps_f64(
     // the projection records start on a 256-byte boundary (their gathers read
     // them with 16-byte loads, and 5*EN doubles is odd when EN is)
     const size_t prec_at = (5 * EN + 31) & ~(size_t)31;
-    if ((rc = ensure_buf(ctx->scratch, (prec_at + 12 * EN) * sizeof(double)))) return rc;
-    double *rec = (double *)ctx->scratch.ptr, *prec = rec + prec_at;
-    // convection records (elem_convec.f90:227-470, E slot = enthalpy)
-    if ((rc = conv_elements<double>(ctx, porder, m, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI,
-                                    gmshAtoJ, gmshAtoK, u, q, rho, pr, aux_h, rec)))
+    // then the node projection records, on a 256-byte boundary
+    const size_t prj_at = (prec_at + 12 * EN + 31) & ~(size_t)31;
+    if ((rc = ensure_buf(ctx->scratch, (prj_at + kProjRec * (size_t)npoin) * sizeof(double))))
+        return rc;
+    double *rec = (double *)ctx->scratch.ptr, *prec = rec + prec_at, *prj = rec + prj_at;
+    // at p=3 the convection and the stabilised viscous operator run in one
+    // two-wave element kernel after the projections; otherwise the
+    // convection records (elem_convec.f90:227-470, E slot = enthalpy) come
+    // first and the stabilised operator adds to them
+    const bool split = porder == 3;
+    if (!split && (rc = conv_elements<double>(ctx, porder, m, He, gpvol, dlxigp_ip, invAtoIJK,
+                                              gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, aux_h,
+                                              rec)))
         return rc;
     LpsArgs A{};
     A.nelem = nelem;
@@ -651,11 +697,7 @@ extern "C" int sodhip_rk_stage_lps_f64(
     A.mu_fluid = mu_fluid;
     A.mu_e = mu_e;
     A.mu_sgs = nullptr;
-    A.pMass = ProjMass;
-    A.pEner = ProjEner;
-    A.pMX = ProjMX;
-    A.pMY = ProjMY;
-    A.pMZ = ProjMZ;
+    A.prj = prj;
     A.tau = tau_stab;
     A.Cp = prm->Cp;
     A.Pr = prm->Pr;
@@ -664,20 +706,28 @@ extern "C" int sodhip_rk_stage_lps_f64(
     A.c_lps = c_lps;
     // projections (elem_stab.f90:441-586)
     A.out = prec;
-    if ((rc = proj_run(ctx, porder, m, A, invMl, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ)))
+    if ((rc = proj_run(ctx, porder, m, A, invMl, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ,
+                       prj)))
         return rc;
     // stabilised viscous operator added to the convection records: both carry
     // the same fact (-1), applied once by the gather
-    A.out = rec;
-    const int grid = elem_grid(ctx, nelem, 16);
-    int tok = prof_begin(ctx, 0);
-    switch (porder) {
-    case 2: k_diff_stab<2, true><<<grid, EG<2>::NT, 0, ctx->stream>>>(A); break;
-    case 3: k_diff_stab<3, true><<<grid, EG<3>::NT, 0, ctx->stream>>>(A); break;
-    case 4: k_diff_stab<4, true><<<grid, EG<4>::NT, 0, ctx->stream>>>(A); break;
+    if (split) {
+        if ((rc = rhs_lps_elements(ctx, m, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ,
+                                   gmshAtoK, rho, q, E, mu_e, prj, tau_stab, prm->gamma_gas,
+                                   prm->Rgas, prm->Cp, prm->Pr, prm->mu_factor, prm->c_rho,
+                                   prm->c_ener, c_lps, rec)))
+            return rc;
+    } else {
+        A.out = rec;
+        const int grid = elem_grid(ctx, nelem, 16);
+        int tok = prof_begin(ctx, 0);
+        switch (porder) {
+        case 2: k_diff_stab<2, true, true><<<grid, EG<2>::NT, 0, ctx->stream>>>(A); break;
+        case 4: k_diff_stab<4, true, true><<<grid, EG<4>::NT, 0, ctx->stream>>>(A); break;
+        }
+        SODHIP_TRY_HIP(hipGetLastError());
+        prof_end(ctx, tok);
     }
-    SODHIP_TRY_HIP(hipGetLastError());
-    prof_end(ctx, tok);
     const NextStage nx{1.0, gam_next * dt, rho_next, q_next, E_next};
     return gather_update<double>(ctx, m, npoin, npoin_w, lpoin_w, rec, 5, EN, invMl, prm->fact, dt,
                                  lam1, lams, gam, rho1, q1, E1, rho, q, E, Rmass, Rmom, Rener,

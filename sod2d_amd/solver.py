@@ -214,7 +214,9 @@ class TGVRun:
         if sensor_path == "live":
             self.divU, self.vorti = z(n), z(3 * n)
         if lps:
-            self.proj = [z(3 * n) for _ in range(5)]  # ProjMass, ProjEner, ProjMX, MY, MZ
+            # the projections stay inside the stage call (ProjMass, ProjEner,
+            # ProjMX, MY, MZ are not kept)
+            self.proj = [None] * 5
             self.tau_stab = z(ne)
             self.aux_h = z(n)
         if integrator == "rk4":

@@ -164,3 +164,51 @@ def test_time_loop_lps_fixed_dt_next_stage(oracle):
         assert relerr(s["rho"][w], orc.rho[1][w]) < 1e-12, step
         assert relerr(s["q"][w], orc.q[1][w]) < 1e-12, step
         assert relerr(s["E"][w], orc.E[1][w]) < 1e-12, step
+
+
+@pytest.mark.parametrize("p,n", [(3, 3), (2, 3)])
+def test_fused_stage_projection_arrays_optional(oracle, p, n):
+    """sodhip_rk_stage_lps_f64 keeps the projections inside the call when the
+    Proj arrays are NULL (its stabilised operator reads them as node records):
+    the stage then gives the same bits as with the arrays, and the arrays
+    hold the drop-in full_proj_ijk of the stage's EOS output, bit for bit
+    (the drop-in is checked against the oracle in the projection test)."""
+    import ctypes as C
+
+    from sod2d_amd._lib import check, lib
+    from sod2d_amd.device import _ptr
+    from sod2d_amd.solver import C_LPS_COMP, TGVParams, TGVRun
+    g = make_case(p, n, periodic=True, lengths=(2 * np.pi,) * 3)
+    tp = TGVParams()
+    ic = oracle.tgv_ic(g.mesh.coord, oracle.tgv_params())
+    run = TGVRun(p, geometry=g, params=tp, initial=ic, sensor_path="live", lps=True)
+    run.step()
+    m, N = g.mesh, run.npoin
+    outs = []
+    for keep in (False, True):
+        y = [run.rho[1].clone(), run.q[1].clone(), run.E[1].clone()]
+        prim = [torch.zeros(k * N, dtype=torch.float64, device="cuda") for k in (3, 1, 1, 1, 1, 1)]
+        proj = [torch.zeros(3 * N, dtype=torch.float64, device="cuda") for _ in range(5)]
+        check(lib().sodhip_rk_stage_lps_f64(
+            run.ctx.h, p, run.nelem, N, run.npoin_w, _ptr(run.connec), _ptr(run.lpoin_w),
+            _ptr(run.He), _ptr(run.gpvol), _ptr(run.dl), *[_ptr(x) for x in run.tabs],
+            _ptr(run.mu_e), _ptr(run.invMl), _ptr(run.tau_stab), C.byref(run.sp), C_LPS_COMP,
+            run.dt, run.L[2, 0], run.L[2, 1], run.G[2, 1], _ptr(run.rho[0]), _ptr(run.q[0]),
+            _ptr(run.E[0]), *[_ptr(x) for x in y], *[_ptr(x) for x in prim],
+            *[_ptr(x) if keep else None for x in proj], None, None, None, 0.0, None, None, None),
+            "rk_stage_lps")
+        torch.cuda.synchronize()
+        outs.append((y, prim, proj))
+    (y0, _, _), (y1, prim, proj) = outs
+    for a, b in zip(y0, y1):
+        assert torch.equal(a, b)
+    # the arrays: the drop-in full_proj_ijk (pinned against the oracle above)
+    # of the stage's EOS output, bit for bit
+    from sod2d_amd import device as D
+    ref = [torch.full((3 * N,), 7.0, dtype=torch.float64, device="cuda") for _ in range(5)]
+    D.full_proj_ijk(m.nelem, N, m.npoin_w, run.connec, run.lpoin_w, None, run.He, run.gpvol,
+                    run.dl, *run.tabs, 1004.0, 0.71, run.rho[1], prim[0], prim[4], None,
+                    run.invMl, *ref)
+    torch.cuda.synchronize()
+    for a, b in zip(proj, ref):
+        assert torch.equal(a, b)
