@@ -212,3 +212,25 @@ def test_fused_stage_projection_arrays_optional(oracle, p, n):
     torch.cuda.synchronize()
     for a, b in zip(proj, ref):
         assert torch.equal(a, b)
+
+
+def test_lps_stage_geometry_row_layouts_agree(oracle):
+    """The p=3 LPS stage moves He and gpvol through LDS in 16-byte or 4-byte
+    lane rows (sodhip_set_stage_config); both give the same LDS image, so
+    the same bits, step after step."""
+    from sod2d_amd import autotune
+    from sod2d_amd.solver import TGVParams, TGVRun
+    g = make_case(3, 4, periodic=True, lengths=(2 * np.pi,) * 3)
+    tp = TGVParams()
+    ic = oracle.tgv_ic(g.mesh.coord, oracle.tgv_params())
+    runs = []
+    for geo16 in (1, 0):
+        run = TGVRun(3, geometry=g, params=tp, initial=ic, sensor_path="live", lps=True,
+                     dt_mode="fixed")
+        autotune.set_config(run.ctx, (8, geo16, 128))
+        for _ in range(3):
+            run.step()
+        runs.append(run.state())
+        autotune.set_config(run.ctx, (16, 1, 256))
+    for k in ("rho", "q", "E", "mu_e"):
+        assert np.array_equal(runs[0][k], runs[1][k]), k
