@@ -1,10 +1,10 @@
 """The isolated-operator workload at its full size: full_convec_ijk,
 full_diffusion_ijk and smart_visc_spectral on a 128^3-element structured box
 (2,097,152 hexahedra) at p=3 (HEX64) and p=4 (HEX125), node positions
-jittered per axis by U(-0.05, 0.05) x the local GLL spacing (the +-10 % of
-the workload's description folds one element of 2.1 M at p=3 for this seed,
-so the Jacobian-positivity requirement fixes +-5 %), fields from
-case.operator_bench_fields.
+jittered per axis by U(-j, j) x the local GLL spacing: the workload's
+j = 0.10 at p=3 (seed 2, for which every Jacobian stays positive), and
+j = 0.05 at p=4, where 0.10 folds an element for every seed tried (2..15),
+fields from case.operator_bench_fields.
 
 The device runs every element of the full mesh.  The oracle cannot evaluate
 2 M elements in a test's time, so it evaluates blocks of 6^3 elements (a
@@ -29,8 +29,9 @@ from helpers import fz, relerr
 pytestmark = pytest.mark.gpu
 
 N = 128
-JITTER = 0.05
-SEED = 20261202
+JITTER = {3: 0.10, 4: 0.05}
+SEED = {3: 2, 4: 20261202}
+FSEED = 20261202  # fields and sample draws
 BLK = 6
 BLOCKS = [(0, 0, 0), (N // 2 - BLK // 2,) * 3, (17, 90, 41)]
 
@@ -45,7 +46,7 @@ def _dev(a, dtype=None):
 def c3(request):
     p = request.param
     T = case.basis_tables(p)
-    m = case.box_mesh(p, N, periodic=False, jitter=JITTER, seed=SEED)
+    m = case.box_mesh(p, N, periodic=False, jitter=JITTER[p], seed=SEED[p])
     He, gpvol = case.elem_jacobian(p, m.connec_orig, m.coord)
     # element lengths: corner-edge minimum; per node the spectral minimum at
     # p=3, the element's value at p=4 (an input of the sensor only)
@@ -54,7 +55,7 @@ def c3(request):
     else:
         helem, _ = case.char_length(p, m.connec_orig, m.coord, spectral=False)
         helem_l = np.asfortranarray(np.repeat(helem[:, None], T.nnode, axis=1))
-    s = case.operator_bench_fields(m.npoin, m.nelem, T.nnode, seed=SEED + p)
+    s = case.operator_bench_fields(m.npoin, m.nelem, T.nnode, seed=FSEED + p)
     del m.coord
     g = dict(p=p, T=T, m=m, He=He, gpvol=gpvol, helem=helem, helem_l=helem_l, s=s)
     # device copies, uploaded once; FP32 copies made on the device
@@ -156,7 +157,7 @@ def test_c3_smart_visc_spectral(c3, oracle):
     p, m, T = g["p"], g["m"], g["T"]
     g["d"].pop(torch.float32, None)
     ne, n, nn = m.nelem, m.npoin, T.nnode
-    r = np.random.default_rng(SEED + 10 * p)
+    r = np.random.default_rng(FSEED + 10 * p)
     rho = r.uniform(0.9, 1.1, n)
     u = np.asfortranarray(r.normal(0, 0.3, (n, 3)))
     Tem = r.uniform(0.9, 1.1, n)
