@@ -8,7 +8,8 @@ non-periodic box are moved by U(-0.1, 0.1) x the local GLL spacing per axis
 node, and the device time loop (two-wave element kernel at p=3, its LPS mode,
 the single-wave kernel at p=2 and p=4) is compared with the oracle's
 restatement step by step, with the bars of the periodic time-loop tests:
-rho, rho*u, E within 1e-12 normwise relative, mu_e within 1e-10."""
+rho, rho*u, E within 1e-12 normwise relative, mu_e within 1e-10; the
+FP32-operator mode with the bounds of its periodic test."""
 import numpy as np
 import pytest
 
@@ -45,3 +46,25 @@ def test_time_loop_on_jittered_box(oracle, p, n, lps):
         assert relerr(s["mu_e"], orc.mu_e) < 1e-10, step
     # the flow moved
     assert relerr(s["q"][w], ic["q"][w]) > 1e-8
+
+
+def test_time_loop_fp32_operators_on_jittered_box(oracle):
+    """The FP32-operator mode of the two-wave kernel (FP64 state, gather, mass
+    inverse and combine) on the non-affine box against the FP64 oracle after 4
+    steps, with the bounds of the periodic FP32-operator test (rho 2e-7,
+    rho*u 1e-5, E 2e-7)."""
+    from sod2d_amd.solver import TGVParams, TGVRun
+    g = make_case(3, 4, periodic=False, jitter=0.1, lengths=(2 * np.pi,) * 3, seed=11)
+    tp = TGVParams()
+    to = oracle.tgv_params()
+    ic = oracle.tgv_ic(g.mesh.coord, to)
+    dev = TGVRun(3, geometry=g, params=tp, initial=ic, op_precision=4)
+    orc = oracle.TGVOracle(3, g, ic, to, cfl=(tp.cfl_conv, tp.cfl_diff))
+    for _ in range(4):
+        dev.step()
+        orc.step()
+    s = dev.state()
+    w = orc.lpoin_w - 1
+    e = {k: relerr(s[k][w], getattr(orc, k)[1][w]) for k in ("rho", "q", "E")}
+    print("FP32 operators on the jittered box:", e, flush=True)
+    assert e["rho"] < 2e-7 and e["q"] < 1e-5 and e["E"] < 2e-7, e
