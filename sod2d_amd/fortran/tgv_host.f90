@@ -15,6 +15,10 @@
 !   * stages 2..S: rk_stage_fused (updateF, lumped inverse and combine);
 !   * end of step: eos_entropy_step, entropy_convec_residual and
 !     smart_visc_spectral on the new state (:389-473).
+! With the LPS stabilisation (flag_lps_stab, meta.txt's optional fifth line
+! set to 1) comp_tau runs at the start of every step on the level-1 velocity
+! (:266-268) and every updateF is rk_stage_lps_fused (full_proj_ijk and
+! full_diff_stab_ijk in place of full_diffusion_ijk).
 ! The level copy 2 -> 1 of the conserved and primitive fields is an exchange
 ! of the level index (level 2 of the primitives is rewritten before it is
 ! read, and stage 1 reads the conserved level 1 only); eta(:,1) = eta(:,2) is
@@ -31,7 +35,7 @@ program tgv_host
    character(len=512) :: dir
    integer(4) :: nelem, npoin, npoin_w, nsteps, nstage, istep, s, l1, l2, lt
    real(8) :: gamma_gas, Rgas, Cp, Prt, mu_factor, po, rho0, dt, ce, cmax, cglob
-   integer(4) :: flag_norm
+   integer(4) :: flag_norm, lps, ios
    integer(4), allocatable :: connec(:,:), lpoin_w(:), invAtoIJK(:,:,:)
    integer(4), allocatable :: gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
    real(8), allocatable :: He(:,:,:,:), gpvol(:,:,:), dlxigp_ip(:,:,:), invMl(:)
@@ -41,6 +45,9 @@ program tgv_host
    real(8), allocatable :: mu_e(:,:), mue_l(:,:), Reta(:,:), auxReta(:), f_eta(:,:)
    real(8), allocatable :: Rmass(:), Rmom(:,:), Rener(:)
    real(8), allocatable :: rkL(:,:), rkG(:,:)
+   ! the LPS path: tau per element, the projections and the enthalpy work array
+   real(8), allocatable :: tau_stab(:), aux_h(:), ProjMass(:,:), ProjEner(:,:)
+   real(8), allocatable :: ProjMX(:,:), ProjMY(:,:), ProjMZ(:,:)
    ! arguments of the reference's interfaces that the device path does not read
    real(8) :: Ngp(nnode,nnode), dNgp(ndime,nnode,nnode), xgp(nnode,ndime), wgp(nnode)
    real(8), allocatable :: coord(:,:), Ml(:)
@@ -57,6 +64,8 @@ program tgv_host
    read(10,*) gamma_gas, Rgas, Cp, Prt, mu_factor, po, rho0
    read(10,*) dt
    read(10,*) ce, cmax, cglob, flag_norm
+   read(10,*,iostat=ios) lps
+   if (ios /= 0) lps = 0
    close(10)
 
    allocate(connec(nelem,nnode), lpoin_w(npoin_w), invAtoIJK(p+1,p+1,p+1))
@@ -68,6 +77,10 @@ program tgv_host
    allocate(mu_fluid(npoin), mu_e(nelem,nnode), mue_l(nelem,nnode), Reta(npoin,2))
    allocate(auxReta(npoin), f_eta(npoin,ndime), Rmass(npoin), Rmom(npoin,ndime), Rener(npoin))
    allocate(rkL(0:nstage,0:nstage), rkG(0:nstage,0:nstage), coord(npoin,ndime), Ml(npoin))
+   allocate(tau_stab(nelem), aux_h(npoin), ProjMass(npoin,ndime), ProjEner(npoin,ndime))
+   allocate(ProjMX(npoin,ndime), ProjMY(npoin,ndime), ProjMZ(npoin,ndime))
+   tau_stab = 0.0d0; aux_h = 0.0d0; ProjMass = 0.0d0; ProjEner = 0.0d0
+   ProjMX = 0.0d0; ProjMY = 0.0d0; ProjMZ = 0.0d0
    Ngp = 0.0d0; dNgp = 0.0d0; xgp = 0.0d0; wgp = 0.0d0; coord = 0.0d0; Ml = 0.0d0
 
    call read_i4(dir, 'connec', connec, size(connec))
@@ -137,6 +150,15 @@ program tgv_host
    call sod2d_hip_enter_data(Rmass)
    call sod2d_hip_enter_data(Rmom)
    call sod2d_hip_enter_data(Rener)
+   if (lps /= 0) then
+      call sod2d_hip_enter_data(tau_stab)
+      call sod2d_hip_enter_data(aux_h)
+      call sod2d_hip_enter_data(ProjMass)
+      call sod2d_hip_enter_data(ProjEner)
+      call sod2d_hip_enter_data(ProjMX)
+      call sod2d_hip_enter_data(ProjMY)
+      call sod2d_hip_enter_data(ProjMZ)
+   end if
 
    l1 = 1; l2 = 2
    first = .true.
@@ -144,6 +166,7 @@ program tgv_host
       ! level 2 -> level 1 (an exchange of the level index) and eta(:,1) = eta(:,2)
       lt = l1; l1 = l2; l2 = lt
       call sod2d_hip_copy(eta(:,2), eta(:,1))
+      if (lps /= 0) call comp_tau(nelem, npoin, connec, csound, u(:,:,l1), helem, dt, tau_stab)
       if (first) then
          ! Reta(:,1) of f_eta = u(:,:,1) eta(:,1); the sensor; updateF at y^n
          call generic_scalar_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, &
@@ -152,29 +175,15 @@ program tgv_host
          call smart_visc_spectral(nelem, npoin, npoin_w, connec, lpoin_w, Reta(:,1), Ngp, coord, &
             dNgp, gpvol, wgp, gamma_gas, rho(:,l1), u(:,:,l1), csound, Tem(:,l1), eta(:,1), &
             helem_l, helem, Ml, mu_e, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mue_l)
-         call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
-            invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
-            mu_factor, dt, 0.0d0, 1.0d0, 0.0d0, rho(:,l1), q(:,:,l1), E(:,l1), rho(:,l2), &
-            q(:,:,l2), E(:,l2), Rmass, Rmom, Rener)
+         call stage(0.0d0, 1.0d0, 0.0d0, .true., .false.)
          ! stage 1 with the kept residual (time_integ_ls.f90:323-331)
          call rk_stage_update(npoin, npoin_w, lpoin_w, dt, 1.0d0, 0.0d0, rkG(1,0), rho(:,l1), &
             q(:,:,l1), E(:,l1), rho(:,l2), q(:,:,l2), E(:,l2), Rmass, Rmom, Rener)
          first = .false.
       end if
       do s = 2, nstage
-         if (s == nstage) then
-            ! the last stage keeps R and stores the next step's stage 1 in level 1
-            call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
-               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
-               mu_factor, dt, rkL(s,0), rkL(s,s-1), rkG(s,s-1), rho(:,l1), q(:,:,l1), E(:,l1), &
-               rho(:,l2), q(:,:,l2), E(:,l2), Rmass, Rmom, Rener, rkG(1,0), rho(:,l1), &
-               q(:,:,l1), E(:,l1))
-         else
-            call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
-               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
-               mu_factor, dt, rkL(s,0), rkL(s,s-1), rkG(s,s-1), rho(:,l1), q(:,:,l1), E(:,l1), &
-               rho(:,l2), q(:,:,l2), E(:,l2))
-         end if
+         ! the last stage keeps R and stores the next step's stage 1 in level 1
+         call stage(rkL(s,0), rkL(s,s-1), rkG(s,s-1), s == nstage, s == nstage)
       end do
       ! end of step: EOS, entropy and Sutherland; the entropy residual; the sensor
       call eos_entropy_step(npoin, npoin_w, lpoin_w, gamma_gas, Rgas, po, rho0, mu_factor, &
@@ -200,6 +209,52 @@ program tgv_host
    write(*,'(a,i0,a)') 'tgv_host: ', nsteps, ' steps done'
 
 contains
+
+   ! one updateF, lumped solve and stage combine (rk_stage_fused, or
+   ! rk_stage_lps_fused with the LPS stabilisation) from level l1 (y^n) and level
+   ! l2 (the stage state) into level l2; keep_R stores the residual, next also
+   ! writes the next step's stage 1 into level l1
+   subroutine stage(lam1, lams, gam, keep_R, next)
+      real(8), intent(in) :: lam1, lams, gam
+      logical, intent(in) :: keep_R, next
+      if (lps /= 0) then
+         if (next) then
+            call rk_stage_lps_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, &
+               dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, tau_stab, &
+               gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), &
+               E(:,l1), rho(:,l2), q(:,:,l2), E(:,l2), u(:,:,l2), e_int(:,l2), pr(:,l2), aux_h, &
+               Tem(:,l2), mu_fluid, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ, Rmass, Rmom, &
+               Rener, rkG(1,0), rho(:,l1), q(:,:,l1), E(:,l1))
+         else if (keep_R) then
+            call rk_stage_lps_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, &
+               dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, tau_stab, &
+               gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), &
+               E(:,l1), rho(:,l2), q(:,:,l2), E(:,l2), u(:,:,l2), e_int(:,l2), pr(:,l2), aux_h, &
+               Tem(:,l2), mu_fluid, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ, Rmass, Rmom, Rener)
+         else
+            call rk_stage_lps_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, &
+               dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, tau_stab, &
+               gamma_gas, Rgas, Cp, Prt, mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), &
+               E(:,l1), rho(:,l2), q(:,:,l2), E(:,l2), u(:,:,l2), e_int(:,l2), pr(:,l2), aux_h, &
+               Tem(:,l2), mu_fluid, ProjMass, ProjEner, ProjMX, ProjMY, ProjMZ)
+         end if
+      else if (next) then
+         call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
+            invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
+            mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), E(:,l1), rho(:,l2), &
+            q(:,:,l2), E(:,l2), Rmass, Rmom, Rener, rkG(1,0), rho(:,l1), q(:,:,l1), E(:,l1))
+      else if (keep_R) then
+         call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
+            invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
+            mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), E(:,l1), rho(:,l2), &
+            q(:,:,l2), E(:,l2), Rmass, Rmom, Rener)
+      else
+         call rk_stage_fused(nelem, npoin, npoin_w, connec, lpoin_w, He, gpvol, dlxigp_ip, &
+            invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, mu_e, invMl, gamma_gas, Rgas, Cp, Prt, &
+            mu_factor, dt, lam1, lams, gam, rho(:,l1), q(:,:,l1), E(:,l1), rho(:,l2), &
+            q(:,:,l2), E(:,l2))
+      end if
+   end subroutine stage
 
    subroutine read_i4(d, name, a, n)
       character(*), intent(in) :: d, name

@@ -5,7 +5,9 @@ case entirely through mod_sod2d_hip, at fixed dt, and writes each step's
 conserved fields and mu_e.  Every step is checked against the oracle's
 restatement of the same loop: rho, rho*u and E within 1e-12 normwise relative
 on the working nodes, mu_e within 1e-10 (its min/max limiter), the bars of the
-Python-hosted time loop."""
+Python-hosted time loop.  With ``lps`` the program runs the LPS-stabilised
+loop (comp_tau at the start of every step, rk_stage_lps_fused for every
+updateF) against the oracle's LPS restatement with the same bars."""
 import os
 import subprocess
 
@@ -27,8 +29,9 @@ def _r(d, name, shape):
     return np.fromfile(os.path.join(d, name + ".bin"), dtype=np.float64).reshape(shape, order="F")
 
 
-@pytest.mark.parametrize("p,n,nsteps,stages", [(3, 4, 6, 5), (2, 4, 5, 4)])
-def test_fortran_host_time_loop(oracle, tmp_path, p, n, nsteps, stages):
+@pytest.mark.parametrize("p,n,nsteps,stages,lps", [(3, 4, 6, 5, False), (2, 4, 5, 4, False),
+                                                   (3, 4, 4, 5, True), (2, 4, 4, 4, True)])
+def test_fortran_host_time_loop(oracle, tmp_path, p, n, nsteps, stages, lps):
     exe = os.path.join(FDIR, f"_build_p{p}", "tgv_host")
     if not os.path.exists(exe):
         pytest.fail(f"{exe} is not built (python -m sod2d_amd.build)")
@@ -36,7 +39,7 @@ def test_fortran_host_time_loop(oracle, tmp_path, p, n, nsteps, stages):
     m, T = g.mesh, oracle.Tables(p)
     t = oracle.tgv_params(M=0.1)
     ic = oracle.tgv_ic(m.coord, t)
-    orc = oracle.TGVOracle(p, g, ic, t, stages=stages, parity=True, dt_mode="fixed")
+    orc = oracle.TGVOracle(p, g, ic, t, stages=stages, parity=True, dt_mode="fixed", lps=lps)
     prm = oracle.visc_params(t.Rgas)
     L, G = oracle.rk_ls_coeffs(stages, 1)  # 6 x 6; the leading (S+1) x (S+1) block is used
     L, G = L[:stages + 1, :stages + 1], G[:stages + 1, :stages + 1]
@@ -47,6 +50,7 @@ def test_fortran_host_time_loop(oracle, tmp_path, p, n, nsteps, stages):
                 f"{t.rho0!r}\n")
         f.write(f"{orc.dt!r}\n")
         f.write(f"{prm.ce!r} {prm.cmax!r} {prm.cglob!r} {prm.flag_normalise_entropy}\n")
+        f.write(f"{int(lps)}\n")
     for name, a in [("connec", m.connec), ("lpoin_w", m.lpoin_w), ("invAtoIJK", T.invAtoIJK),
                     ("gmshAtoI", T.gI), ("gmshAtoJ", T.gJ),
                     ("gmshAtoK", T.gK)]:
@@ -57,7 +61,7 @@ def test_fortran_host_time_loop(oracle, tmp_path, p, n, nsteps, stages):
         _w(d, name, a)
     for name in ("rho", "q", "E", "u", "pr", "Tem", "e_int", "eta", "csound", "machno"):
         _w(d, name, ic[name])
-    print(f"fortran host TGV p={p} n={n} steps={nsteps} stages={stages}", flush=True)
+    print(f"fortran host TGV p={p} n={n} steps={nsteps} stages={stages} lps={lps}", flush=True)
     res = subprocess.run([exe, d], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert f"tgv_host: {nsteps} steps done" in res.stdout
