@@ -799,9 +799,10 @@ __global__ __launch_bounds__(256) void k_gather(int npoin, const int *__restrict
 #pragma unroll
                 for (int f = 0; f < NF; ++f) v[u][f] = TR(w[f]);
             } else {
+                // an absent slot reads record 0 unconditionally (cheaper than
+                // masking each load) and is not added
 #pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    v[u][f] = sj[u] >= 0 ? TR(scr[(size_t)sj[u] * NF + f]) : TR(0);
+                for (int f = 0; f < NF; ++f) v[u][f] = TR(scr[(size_t)max(sj[u], 0) * NF + f]);
             }
         }
 #pragma unroll
@@ -844,7 +845,7 @@ __global__ __launch_bounds__(256) void k_gather_entropy(int npoin, const int *__
         for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? wsrc[j + u] : -1;
         double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = sj[u] >= 0 ? scr[sj[u]] : 0.0;
+        for (int u = 0; u < 4; ++u) v[u] = scr[max(sj[u], 0)]; // absent: record 0, not added
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (sj[u] >= 0) acc += fact * v[u];

@@ -404,19 +404,22 @@ __global__ __launch_bounds__(256) void k_gather_proj(int npoin, const int *__res
     double acc[12];
 #pragma unroll
     for (int f = 0; f < 12; ++f) acc[f] = 0.0;
-    for (int j = j0; j < j1; j += 4) {
-        int sj[4];
+    // two slots per step; the loads of an absent slot read record 0 (always
+    // present) unconditionally, which is cheaper than masking each load, and
+    // are not added
+    for (int j = j0; j < j1; j += 2) {
+        int sj[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
-        double2 v[4][6];
+        for (int u = 0; u < 2; ++u) sj[u] = j + u < j1 ? src[j + u] : -1;
+        double2 v[2][6];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             const double2 *r = (const double2 *)(scr + 12 * (size_t)max(sj[u], 0));
 #pragma unroll
-            for (int q = 0; q < 6; ++q) v[u][q] = sj[u] >= 0 ? r[q] : make_double2(0.0, 0.0);
+            for (int q = 0; q < 6; ++q) v[u][q] = r[q];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 2; ++u)
             if (sj[u] >= 0) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
