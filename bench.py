@@ -391,7 +391,8 @@ def main(argv=None):
     ap.add_argument("--stage-config", default=None, metavar="B,G,T",
                     help="use this launch shape of the fused stage (element blocks per CU, "
                          "16-byte geometry rows 0/1, gather threads) instead of autotuning, "
-                         "e.g. to profile the shape a tuned run chose")
+                         "e.g. to profile the shape a tuned run chose; with --lps the element "
+                         "kernel rounds B down to whole rounds of the blocks a CU holds")
     ap.add_argument("--op-n", type=int, default=128,
                     help="elements per direction of the operator workload's box")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -439,7 +440,7 @@ def main(argv=None):
         run.step()
     torch.cuda.synchronize()
     tune = None
-    if a.stage_config and not a.lps:
+    if a.stage_config:
         from sod2d_amd import autotune
         cfg = tuple(int(x) for x in a.stage_config.split(","))
         autotune.set_config(run.ctx, cfg)
