@@ -1897,7 +1897,7 @@ int conv_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *He, c
     A.E = E;
     A.out = out;
     int tok = prof_begin(ctx, 0);
-    const int rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, m->nelem, 16));
+    const int rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, m->nelem, 128));
     prof_end(ctx, tok);
     return rc;
 }
@@ -1943,7 +1943,7 @@ static int full_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const i
     A.E = E;
     A.out = (T *)ctx->scratch.ptr;
     int tok = prof_begin(ctx, 0);
-    rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, nelem, 16));
+    rc = launch_rhs<T>(ctx, porder, true, false, false, A, elem_grid(ctx, nelem, 128));
     prof_end(ctx, tok);
     if (rc) return rc;
     tok = prof_begin(ctx, 1);
@@ -1997,7 +1997,7 @@ static int full_diffusion(sodhip_ctx ctx, int porder, int nelem, int npoin, cons
     A.c_ener = c_ener;
     A.out = (T *)ctx->scratch.ptr;
     int tok = prof_begin(ctx, 0);
-    rc = launch_rhs<T>(ctx, porder, false, true, false, A, elem_grid(ctx, nelem, 16));
+    rc = launch_rhs<T>(ctx, porder, false, true, false, A, elem_grid(ctx, nelem, 128));
     prof_end(ctx, tok);
     if (rc) return rc;
     tok = prof_begin(ctx, 1);
@@ -2080,7 +2080,7 @@ int rhs_fused_elements(sodhip_ctx ctx, int porder, const MeshCache *m, const T *
     }
     if (!done)
 #endif
-        rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 16));
+        rc = launch_rhs<T>(ctx, porder, true, true, true, A, elem_grid(ctx, m->nelem, 128));
     prof_end(ctx, tok);
     return rc;
 }
@@ -2275,7 +2275,7 @@ static int scalar_convec(sodhip_ctx ctx, int porder, int nelem, int npoin, const
     A.gK = gK;
     A.u = u;
     A.out = (T *)ctx->scratch.ptr;
-    const int grid = elem_grid(ctx, nelem, 16);
+    const int grid = elem_grid(ctx, nelem, 128);
     int tok = prof_begin(ctx, 3);
 #define SODHIP_SC(PP)                                                                       \
     if (q)                                                                                  \
@@ -2344,7 +2344,7 @@ static int div_curl(sodhip_ctx ctx, int porder, int nelem, int npoin, int npoin_
     A.gK = gK;
     A.u = u;
     A.out = (T *)ctx->scratch.ptr;
-    const int grid = elem_grid(ctx, nelem, 16);
+    const int grid = elem_grid(ctx, nelem, 128);
     int tok = prof_begin(ctx, 3);
 #define SODHIP_DC(PP, DV, CU) \
     k_div_curl<PP, T, DV, CU><<<grid, EG<PP>::NT, 0, ctx->stream>>>(A)

@@ -619,7 +619,7 @@ extern "C" int sodhip_full_diff_stab_ijk_f64(
     A.c_ener = c_ener;
     A.c_lps = c_lps;
     A.out = (double *)ctx->scratch.ptr;
-    const int grid = elem_grid(ctx, nelem, 16);
+    const int grid = elem_grid(ctx, nelem, 128);
     int tok = prof_begin(ctx, 0);
     SODHIP_LPS_LAUNCH(k_diff_stab, grid, A)
     SODHIP_TRY_HIP(hipGetLastError());
@@ -725,7 +725,7 @@ extern "C" int sodhip_rk_stage_lps_f64(
             return rc;
     } else {
         A.out = rec;
-        const int grid = elem_grid(ctx, nelem, 16);
+        const int grid = elem_grid(ctx, nelem, 128);
         int tok = prof_begin(ctx, 0);
         switch (porder) {
         case 2: k_diff_stab<2, true, true><<<grid, EG<2>::NT, 0, ctx->stream>>>(A); break;
